@@ -1,0 +1,220 @@
+#!/usr/bin/env python3
+"""Benchmark of the point-cloud hologram path on one MI355X.
+
+Default workload (one "step"): the 100,000-point cloud (x,y uniform in
++-8 mm, z in [0.45, 0.55] m, amplitude in [0.1, 1], SplitMix64 seed
+20261202+2) rendered at 3840x2160, 3.74 um pitch, three channels
+(638/532/473 nm) with both RS and Fresnel diffraction on the FP32 direct
+sum, then phase-encoded and normalised to 8 bits -- everything resident in
+HBM before the timed region starts.
+
+value = point-pixel pairs per second (G/s) over the whole step.
+Also reported: field relative L2 against the library's FP64 probe on sampled
+pixels, p50/p99 ms per hologram (one diffraction mode, 3 channels + encode),
+the dominant kernel's roofline (VALU issue model, HIP events on the
+context's stream) and the CPU baseline (the reference's OpenMP CPU path,
+restated in oracle/, timed on a bounded sample on this host).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+# FP32 VALU issue slots per (point, pixel) pair of the direct kernels as
+# implemented (transcendentals v_sin/v_cos count 4 slots): see DESIGN.md.
+SLOTS_PER_PAIR = {0: 15, 1: 12}
+PEAK_FP32_TFLOPS = 157.3          # published MI355X FP32 vector peak
+PEAK_HBM_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", type=int, default=2, choices=[1, 2, 3, 5])
+    ap.add_argument("--points", type=int, default=0, help="override point count (0 = workload's)")
+    ap.add_argument("--mode", default="fp32", choices=["fp32", "fp64"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--probe-pixels", type=int, default=512)
+    return ap.parse_args()
+
+
+def cpu_baseline(w, target_s: float):
+    """The reference's OpenMP CPU path (oracle/ restatement), bounded sample:
+    a leading subset of the workload's points on the full pixel grid, all of
+    the workload's channels and diffraction modes.  Cost is linear in the
+    point count (the mask covers the whole hologram), so pairs/s is exact."""
+    from oracle import oracle as orc
+    from openholo_amd import synth
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    g = orc.make_geom(w.pn, w.pp)
+    # the reference parallelises over points: give every thread several
+    n = min(w.n_points, 4 * threads)
+    while True:
+        v = synth.points(w, n)
+        t0 = time.perf_counter()
+        for mode in w.modes:
+            orc.gen_restate(g, v, w.lambdas, mode, n_threads=threads)
+        t = time.perf_counter() - t0
+        if t >= target_s / 2 or n >= w.n_points:
+            break
+        n = min(w.n_points, max(n + threads, int(n * target_s / max(t, 1e-3))))
+    pairs = n * w.n_pixels * len(w.lambdas) * len(w.modes)
+    return {"value": pairs / t / 1e9, "unit": "Gpairs/s", "cores": threads, "kind": "port",
+            "sample": f"first {n} of {w.n_points} points x {w.pn[0]}x{w.pn[1]} px x "
+                      f"{len(w.lambdas)} ch x modes {list(w.modes)}: {pairs:.3e} pairs in {t:.2f} s "
+                      f"(oracle/ restatement of genCghPointCloudCPU, g++ -O3 -fopenmp)"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "multi-GPU sharding is a design note, not implemented", file=sys.stderr)
+        sys.exit(2)
+
+    from openholo_amd import capi, synth
+    from openholo_amd.hip_events import Event
+
+    w = synth.WORKLOADS[args.workload]
+    n = args.points or w.n_points
+    mode = capi.MODE_GPU | (capi.MODE_FLOAT if args.mode == "fp32" else 0)
+    npx = w.n_pixels
+    nch = len(w.lambdas)
+    pts = synth.points(w, n)
+
+    ctx = capi.Context(0)
+    stream = ctx.stream
+    dv = ctx.to_device(pts)
+    fields = [ctx.alloc(16 * npx) for _ in range(nch)]
+    encs = [ctx.alloc(8 * npx) for _ in range(nch)]
+    u8s = [ctx.alloc(npx) for _ in range(nch)]
+    params = [capi.params(w.pn, w.pp, lam, channel=ch, use_phase=w.complex_amplitude)
+              for ch, lam in enumerate(w.lambdas)]
+
+    dom = 0 if 0 in w.modes else 1          # dominant kernel: RS when present
+    kev = {}                                # per-launch events of the dominant kernel
+    holo_ev = []
+
+    def hologram(diff, timed):
+        e0 = Event()
+        e0.record(stream)
+        for ch in range(nch):
+            ctx.field_zero(fields[ch], npx)
+            if timed and diff == dom:
+                a, b = Event(), Event()
+                a.record(stream)
+                ctx.diffract(dv, n, fields[ch], params[ch], diff, mode)
+                b.record(stream)
+                kev.setdefault(diff, []).append((a, b))
+            else:
+                ctx.diffract(dv, n, fields[ch], params[ch], diff, mode)
+        for ch in range(nch):
+            ctx.encode(fields[ch], npx, encs[ch], capi.ENCODE_PHASE)
+        ctx.normalize_u8(encs, npx, u8s)
+        e1 = Event()
+        e1.record(stream)
+        if timed:
+            holo_ev.append((e0, e1))
+
+    def step(timed):
+        for diff in w.modes:
+            hologram(diff, timed)
+
+    for i in range(args.warmup):
+        step(False)
+        print(f"# warmup step {i + 1}/{args.warmup} done", file=sys.stderr, flush=True)
+    ctx.sync()
+    t_start, t_stop = Event(), Event()
+    wall0 = time.perf_counter()
+    t_start.record(stream)
+    for i in range(args.steps):
+        step(True)
+    t_stop.record(stream)
+    ctx.sync()
+    wall = time.perf_counter() - wall0
+    ms_total = t_start.elapsed_ms(t_stop)
+    ms_step = ms_total / args.steps
+    print(f"# timed {args.steps} steps: {ms_step:.1f} ms/step (wall {wall:.2f} s)", file=sys.stderr, flush=True)
+
+    pairs_step = n * npx * nch * len(w.modes)
+    value = pairs_step / (ms_step / 1e3) / 1e9
+
+    # dominant kernel: average launch time from events on the launch stream
+    launches = kev.get(dom, [])
+    k_ms = float(np.mean([a.elapsed_ms(b) for a, b in launches])) if launches else float("nan")
+    pairs_launch = n * npx
+    slots = SLOTS_PER_PAIR[dom] if mode != capi.MODE_GPU else None
+    if slots:
+        achieved = pairs_launch / (k_ms / 1e3) * slots * 2 / 1e12
+        roofline = {"bound": "valu", "achieved": round(achieved, 2), "peak": PEAK_FP32_TFLOPS,
+                    "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
+                    "kernel": f"k_pc_direct<{'RS' if dom == 0 else 'Fresnel'},FP32>",
+                    "avg_launch_ms": round(k_ms, 3),
+                    "model": f"{slots} FP32 issue slots/pair x 2 FLOP/slot; packed FP32 is not used, "
+                             f"so 0.5 of the published peak is the ceiling"}
+    else:
+        roofline = {"bound": "valu", "achieved": None, "peak": 78.6, "unit": "TFLOP/s", "frac": None,
+                    "traffic": None, "avg_launch_ms": round(k_ms, 3)}
+
+    # p50 / p99 per hologram (one mode, all channels, encode + normalise)
+    hol = sorted(a.elapsed_ms(b) for a, b in holo_ev)
+    p50 = float(np.percentile(hol, 50)) if hol else None
+    p99 = float(np.percentile(hol, 99)) if hol else None
+
+    # accuracy: FP32 field vs the library's FP64 probe on sampled pixels
+    rng = np.random.default_rng(1234)
+    pix = rng.choice(npx, min(args.probe_pixels, npx), replace=False)
+    rel = {}
+    for diff in w.modes:
+        num = den = 0.0
+        for ch in range(nch):
+            ctx.field_zero(fields[ch], npx)
+            ctx.diffract(dv, n, fields[ch], params[ch], diff, mode)
+            f = fields[ch].download(np.complex128, npx)[pix]
+            ref = ctx.probe_f64(dv, n, params[ch], diff, pix)
+            num += float(np.sum(np.abs(f - ref) ** 2))
+            den += float(np.sum(np.abs(ref) ** 2))
+        rel["RS" if diff == 0 else "Fresnel"] = float(np.sqrt(num / den))
+
+    for b in [dv] + fields + encs + u8s:
+        b.free()
+    ctx.close()
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(w, args.cpu_seconds)
+        print(f"# cpu baseline: {cpu['value']:.4f} Gpairs/s on {cpu['cores']} threads", file=sys.stderr, flush=True)
+
+    line = {
+        "metric": "point-pixel pairs/s (G/s) + field rel-L2 vs FP64; p50/p99 ms per hologram",
+        "value": round(value, 3), "unit": "Gpairs/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_step, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32" if args.mode == "fp32" else "f64",
+        "data": "synthetic (SplitMix64 point cloud)",
+        "config": {"workload": w.name, "points": n, "resolution": list(w.pn), "pitch_m": list(w.pp),
+                   "lambdas_m": list(w.lambdas), "modes": ["RS" if m == 0 else "Fresnel" for m in w.modes],
+                   "kernel": "direct sum FP32 (FP64 per-point phase constants)" if args.mode == "fp32"
+                   else "direct sum FP64", "parallelism": "single GPU"},
+        "field_rel_l2_vs_fp64": {k: float(f"{v:.3e}") for k, v in rel.items()},
+        "probe_pixels": int(len(pix)),
+        "p50_ms_per_hologram": round(p50, 3) if p50 else None,
+        "p99_ms_per_hologram": round(p99, 3) if p99 else None,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,156 @@
+/*
+ * openholo_amd.h -- C ABI of the MI355X point-cloud hologram library
+ * (libopenholo_amd.so, gfx950).
+ *
+ * Plain C: opaque handles, plain pointers and sizes, int status codes, no
+ * C++ or framework types.  Device pointers ("d_" prefix) are HIP device
+ * addresses obtained from ohx_malloc (or any hipMalloc of the same device).
+ * Every call is stream-ordered on the context's stream; calls that return
+ * host-visible results synchronize that stream before returning.
+ *
+ * Each entry point names the reference interface it replaces
+ * (paths relative to the Openholo source tree).
+ */
+#ifndef OPENHOLO_AMD_H
+#define OPENHOLO_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OHX_ABI_VERSION 1
+
+/* Status codes.  The reference reports CUDA errors by exit(EXIT_FAILURE)
+ * (ophgen/src/ophPointCloud_GPU.h:71-81); this library never exits. */
+typedef enum ohx_status {
+    OHX_OK = 0,
+    OHX_ERR_INVALID_ARG = -1,
+    OHX_ERR_OUT_OF_MEMORY = -2,
+    OHX_ERR_DEVICE = -3,
+    OHX_ERR_ADMISSION = -4,   /* serving: a job can never fit the memory budget */
+    OHX_ERR_NOT_FOUND = -5,
+    OHX_ERR_TIMEOUT = -6,
+    OHX_ERR_UNSUPPORTED = -7,
+    OHX_ERR_CANCELLED = -8
+} ohx_status;
+
+/* Diffraction flags: ophPointCloud::PC_DIFF_FLAG (ophgen/src/ophPointCloud.h:550-553). */
+enum { OHX_DIFF_RS = 0, OHX_DIFF_FRESNEL = 1 };
+
+/* Mode bits: MODE_GPU/MODE_FLOAT/MODE_FASTMATH (Openholo/src/define.h:155-159),
+ * plus matrix-core variants of the Fresnel sum (no reference counterpart).
+ *   0 (MODE_DOUBLE)         : FP64 direct sum (every term in FP64)
+ *   MODE_FLOAT [|FASTMATH]   : FP32 direct sum with FP64 per-point phase constants
+ *   MFMA_*  (Fresnel only)   : H = U diag(a) V^T on matrix cores; RS ignores them
+ *                              and runs the FP32 direct sum. */
+enum {
+    OHX_MODE_GPU = 1,
+    OHX_MODE_FLOAT = 2,
+    OHX_MODE_FASTMATH = 4,
+    OHX_MODE_MFMA_BF16X1 = 0x10,
+    OHX_MODE_MFMA_BF16X3 = 0x20,
+    OHX_MODE_MFMA_FP16X3 = 0x40
+};
+
+/* Encode flags: ophGen::ENCODE_FLAG (ophgen/src/ophGen.h:84-96), the
+ * per-pixel encoders on this path. */
+enum {
+    OHX_ENCODE_PHASE = 0,
+    OHX_ENCODE_AMPLITUDE = 1,
+    OHX_ENCODE_REAL = 2,
+    OHX_ENCODE_IMAGINARY = 3
+};
+
+typedef struct ohx_ctx ohx_ctx;
+
+/* One launch's geometry.  Replaces CudaPointCloudConfig / ...RS / ...Fresnel
+ * (ophgen/src/ophPointCloud_GPU.h:89-217): the same fields passed by value;
+ * k, half_ss, det_tx/ty and tx/ty are derived inside as the reference does. */
+typedef struct ohx_pc_params {
+    int32_t pn_x, pn_y;       /* pixel_number                                  */
+    int32_t offset_x, offset_y; /* start offset (OphConfig::offset)            */
+    double pp_x, pp_y;        /* pixel_pitch (m)                               */
+    double scale[3];          /* OphPointCloudConfig::scale                    */
+    double distance;          /* OphPointCloudConfig::distance (offset_depth)  */
+    double lambda;            /* wave length of this channel (m)               */
+    int32_t channel;          /* index into Vertex::color (0..2)               */
+    int32_t use_phase;        /* 1: amplitude = color[channel]*exp(j*phase); the
+                                 reference ignores Vertex::phase (0)           */
+} ohx_pc_params;
+
+/* ---- context and memory -------------------------------------------------
+ * Replaces the cudaWrapper singleton (ophgen/src/cudaWrapper.h:50-87) and the
+ * per-call cudaMalloc/cudaMemcpy of genCghPointCloudGPU
+ * (ophgen/src/ophPointCloud_GPU.cpp:218-285). */
+int ohx_ctx_create(int device, ohx_ctx** out);
+int ohx_ctx_destroy(ohx_ctx* ctx);
+void* ohx_ctx_stream(ohx_ctx* ctx);            /* the hipStream_t used for launches */
+int ohx_ctx_device(ohx_ctx* ctx);
+const char* ohx_status_string(int status);
+const char* ohx_last_error(ohx_ctx* ctx);      /* text of the last failure on ctx */
+int ohx_mem_info(ohx_ctx* ctx, size_t* free_bytes, size_t* total_bytes);
+
+int ohx_malloc(ohx_ctx* ctx, size_t bytes, void** d_ptr);
+int ohx_free(ohx_ctx* ctx, void* d_ptr);
+int ohx_memcpy_h2d(ohx_ctx* ctx, void* d_dst, const void* src, size_t bytes);
+int ohx_memcpy_d2h(ohx_ctx* ctx, void* dst, const void* d_src, size_t bytes);
+int ohx_memcpy_d2d(ohx_ctx* ctx, void* d_dst, const void* d_src, size_t bytes);
+int ohx_synchronize(ohx_ctx* ctx);
+
+/* Zero n_px complex FP64 pixels with a fill kernel (resetBuffer,
+ * ophgen/src/ophGen.cpp:801-814; no memset node, safe under graph capture). */
+int ohx_field_zero(ohx_ctx* ctx, double* d_field, int64_t n_px);
+
+/* ---- the hot path --------------------------------------------------------
+ * Accumulate one channel of the point-cloud hologram into d_field
+ * (interleaved re,im FP64, pn_x*pn_y complex, row 0 = +y), i.e.
+ * d_field += sum_p (1/j) a_p exp(j phi_p(x,y)) over the reference's
+ * anti-aliasing mask.  d_vertices: n_points 56-byte Vertex records
+ * (Openholo/src/struct.h:86-106) in device memory.
+ * Replaces cudaPointCloud_RS and cudaPointCloud_Fresnel
+ * (ophgen/src/ophPCKernel.cu:893-933, declared ophgen/src/ophPointCloud_GPU.h:225-233),
+ * selected by diff_flag; mode as above. */
+int ohx_pc_diffract(ohx_ctx* ctx, const void* d_vertices, int64_t n_points,
+                    double* d_field, const ohx_pc_params* params,
+                    uint32_t diff_flag, uint32_t mode);
+
+/* d_dst += d_src over n_px complex FP64 pixels.
+ * Replaces sum_Kernel (ophgen/src/ophPCKernel.cu:874-891). */
+int ohx_field_accumulate(ohx_ctx* ctx, double* d_dst, const double* d_src, int64_t n_px);
+
+/* FP64 field at listed pixels (d_pix: row*pn_x+col, int64), every term in FP64
+ * with the phase reduced in cycles and compensated summation; d_out is
+ * interleaved re,im.  The accuracy probe behind the stated relative-L2
+ * errors; no reference counterpart (the reference evaluates in place). */
+int ohx_pc_probe_f64(ohx_ctx* ctx, const void* d_vertices, int64_t n_points,
+                     const ohx_pc_params* params, uint32_t diff_flag,
+                     const int64_t* d_pix, int64_t n_pix, double* d_out);
+
+/* ---- encoding and 8-bit normalisation ------------------------------------
+ * ophGen::encoding(ENCODE_FLAG) per channel (ophgen/src/ophGen.cpp:816-849,
+ * encoders :1821-1864): d_enc[i] = f(d_field[i]). */
+int ohx_encode(ohx_ctx* ctx, const double* d_field, int64_t n_px, double* d_enc,
+               uint32_t encode_flag);
+
+/* ophGen::normalize (ophgen/src/ophGen.cpp:654-687): global min/max over all
+ * n_channels encoded buffers, then u8 = (uint8)(((v-min)/gap)*255 + 0.5).
+ * d_enc and d_out are host arrays of n_channels device pointers. */
+int ohx_normalize_u8(ohx_ctx* ctx, const double* const* d_enc, int n_channels,
+                     int64_t n_px, uint8_t* const* d_out);
+
+/* Shape of the last ohx_pc_diffract on ctx (for throughput accounting). */
+typedef struct ohx_launch_stats {
+    int64_t pairs_nominal;   /* n_points * pn_x * pn_y                        */
+    int64_t workgroups;      /* workgroups of the diffraction kernel          */
+    int32_t kernel_kind;     /* 0 FP32 direct, 1 FP64 direct, 2 MFMA GEMM     */
+    int32_t reserved;
+} ohx_launch_stats;
+int ohx_last_launch_stats(ohx_ctx* ctx, ohx_launch_stats* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* OPENHOLO_AMD_H */

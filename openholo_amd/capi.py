@@ -1,0 +1,240 @@
+"""ctypes binding of the C ABI in include/openholo_amd.h.
+
+This is the reference-side binding a Python caller would add (the reference
+itself has no Python binding for this path): plain C calls on
+``libopenholo_amd.so``.  Loading fails loudly when the library has not been
+built -- there is no fallback implementation.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libopenholo_amd.so")
+
+OK = 0
+DIFF_RS, DIFF_FRESNEL = 0, 1
+MODE_GPU, MODE_FLOAT, MODE_FASTMATH = 1, 2, 4
+MODE_DOUBLE = 0
+MODE_MFMA_BF16X1, MODE_MFMA_BF16X3, MODE_MFMA_FP16X3 = 0x10, 0x20, 0x40
+ENCODE_PHASE, ENCODE_AMPLITUDE, ENCODE_REAL, ENCODE_IMAGINARY = 0, 1, 2, 3
+
+# every symbol include/openholo_amd.h declares (checked by the CPU test suite)
+EXPORTS = [
+    "ohx_ctx_create", "ohx_ctx_destroy", "ohx_ctx_stream", "ohx_ctx_device", "ohx_status_string",
+    "ohx_last_error", "ohx_mem_info", "ohx_malloc", "ohx_free", "ohx_memcpy_h2d", "ohx_memcpy_d2h",
+    "ohx_memcpy_d2d", "ohx_synchronize", "ohx_field_zero", "ohx_pc_diffract", "ohx_field_accumulate",
+    "ohx_pc_probe_f64", "ohx_encode", "ohx_normalize_u8", "ohx_last_launch_stats",
+]
+
+
+class PcParams(ctypes.Structure):
+    _fields_ = [
+        ("pn_x", ctypes.c_int32), ("pn_y", ctypes.c_int32),
+        ("offset_x", ctypes.c_int32), ("offset_y", ctypes.c_int32),
+        ("pp_x", ctypes.c_double), ("pp_y", ctypes.c_double),
+        ("scale", ctypes.c_double * 3), ("distance", ctypes.c_double),
+        ("lambda_", ctypes.c_double),
+        ("channel", ctypes.c_int32), ("use_phase", ctypes.c_int32),
+    ]
+
+
+class LaunchStats(ctypes.Structure):
+    _fields_ = [("pairs_nominal", ctypes.c_int64), ("workgroups", ctypes.c_int64),
+                ("kernel_kind", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+def params(pn, pp, lam, channel=0, scale=(1.0, 1.0, 1.0), distance=0.0, offset=(0, 0),
+           use_phase=False) -> PcParams:
+    p = PcParams()
+    p.pn_x, p.pn_y = int(pn[0]), int(pn[1])
+    p.offset_x, p.offset_y = int(offset[0]), int(offset[1])
+    p.pp_x, p.pp_y = float(pp[0]), float(pp[1])
+    for i in range(3):
+        p.scale[i] = float(scale[i])
+    p.distance = float(distance)
+    p.lambda_ = float(lam)
+    p.channel = int(channel)
+    p.use_phase = int(bool(use_phase))
+    return p
+
+
+class OhxError(RuntimeError):
+    def __init__(self, status, what, detail=""):
+        super().__init__(f"{what} failed: status {status} ({detail})")
+        self.status = status
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise OhxError(-7, "load libopenholo_amd.so",
+                           f"{LIB_PATH} not built; run __graft_entry__.build() or make -C openholo_amd")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, i64, u32, i32 = ctypes.c_void_p, ctypes.c_int64, ctypes.c_uint32, ctypes.c_int
+        sz = ctypes.c_size_t
+        sig = {
+            "ohx_ctx_create": ([i32, ctypes.POINTER(vp)], i32),
+            "ohx_ctx_destroy": ([vp], i32),
+            "ohx_ctx_stream": ([vp], vp),
+            "ohx_ctx_device": ([vp], i32),
+            "ohx_status_string": ([i32], ctypes.c_char_p),
+            "ohx_last_error": ([vp], ctypes.c_char_p),
+            "ohx_mem_info": ([vp, ctypes.POINTER(sz), ctypes.POINTER(sz)], i32),
+            "ohx_malloc": ([vp, sz, ctypes.POINTER(vp)], i32),
+            "ohx_free": ([vp, vp], i32),
+            "ohx_memcpy_h2d": ([vp, vp, vp, sz], i32),
+            "ohx_memcpy_d2h": ([vp, vp, vp, sz], i32),
+            "ohx_memcpy_d2d": ([vp, vp, vp, sz], i32),
+            "ohx_synchronize": ([vp], i32),
+            "ohx_field_zero": ([vp, vp, i64], i32),
+            "ohx_pc_diffract": ([vp, vp, i64, vp, ctypes.POINTER(PcParams), u32, u32], i32),
+            "ohx_field_accumulate": ([vp, vp, vp, i64], i32),
+            "ohx_pc_probe_f64": ([vp, vp, i64, ctypes.POINTER(PcParams), u32, vp, i64, vp], i32),
+            "ohx_encode": ([vp, vp, i64, vp, u32], i32),
+            "ohx_normalize_u8": ([vp, ctypes.POINTER(vp), i32, i64, ctypes.POINTER(vp)], i32),
+            "ohx_last_launch_stats": ([vp, ctypes.POINTER(LaunchStats)], i32),
+        }
+        for name, (args, res) in sig.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = L
+    return _lib
+
+
+class DeviceBuffer:
+    """A device allocation owned by a Context."""
+
+    def __init__(self, ctx: "Context", nbytes: int):
+        self.ctx, self.nbytes = ctx, int(nbytes)
+        p = ctypes.c_void_p()
+        ctx._check(lib().ohx_malloc(ctx.h, self.nbytes, ctypes.byref(p)), "ohx_malloc")
+        self.ptr = p.value
+
+    def free(self):
+        if self.ptr:
+            self.ctx._check(lib().ohx_free(self.ctx.h, self.ptr), "ohx_free")
+            self.ptr = None
+
+    def upload(self, arr: np.ndarray):
+        a = np.ascontiguousarray(arr)
+        assert a.nbytes <= self.nbytes
+        self.ctx._check(lib().ohx_memcpy_h2d(self.ctx.h, self.ptr, a.ctypes.data, a.nbytes), "h2d")
+
+    def download(self, dtype, count) -> np.ndarray:
+        out = np.empty(count, dtype)
+        assert out.nbytes <= self.nbytes
+        self.ctx._check(lib().ohx_memcpy_d2h(self.ctx.h, out.ctypes.data, self.ptr, out.nbytes), "d2h")
+        return out
+
+
+class Context:
+    def __init__(self, device: int = 0):
+        h = ctypes.c_void_p()
+        st = lib().ohx_ctx_create(int(device), ctypes.byref(h))
+        if st != OK:
+            raise OhxError(st, "ohx_ctx_create", lib().ohx_status_string(st).decode())
+        self.h = h.value
+
+    def _check(self, st, what):
+        if st != OK:
+            raise OhxError(st, what, lib().ohx_last_error(self.h).decode())
+
+    def close(self):
+        if self.h:
+            lib().ohx_ctx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def stream(self):
+        return lib().ohx_ctx_stream(self.h)
+
+    def alloc(self, nbytes) -> DeviceBuffer:
+        return DeviceBuffer(self, nbytes)
+
+    def to_device(self, arr: np.ndarray) -> DeviceBuffer:
+        a = np.ascontiguousarray(arr)
+        b = DeviceBuffer(self, max(a.nbytes, 1))
+        if a.nbytes:
+            b.upload(a)
+        return b
+
+    def sync(self):
+        self._check(lib().ohx_synchronize(self.h), "ohx_synchronize")
+
+    def field_zero(self, field: DeviceBuffer, n_px: int):
+        self._check(lib().ohx_field_zero(self.h, field.ptr, int(n_px)), "ohx_field_zero")
+
+    def diffract(self, d_vertices: DeviceBuffer, n_points: int, field: DeviceBuffer, p: PcParams,
+                 diff: int, mode: int):
+        self._check(lib().ohx_pc_diffract(self.h, d_vertices.ptr, int(n_points), field.ptr,
+                                          ctypes.byref(p), int(diff), int(mode)), "ohx_pc_diffract")
+
+    def accumulate(self, dst: DeviceBuffer, src: DeviceBuffer, n_px: int):
+        self._check(lib().ohx_field_accumulate(self.h, dst.ptr, src.ptr, int(n_px)), "ohx_field_accumulate")
+
+    def probe_f64(self, d_vertices: DeviceBuffer, n_points: int, p: PcParams, diff: int,
+                  pix: np.ndarray) -> np.ndarray:
+        pix = np.ascontiguousarray(pix, np.int64)
+        dpix = self.to_device(pix)
+        out = self.alloc(16 * max(len(pix), 1))
+        try:
+            self._check(lib().ohx_pc_probe_f64(self.h, d_vertices.ptr, int(n_points), ctypes.byref(p),
+                                               int(diff), dpix.ptr, len(pix), out.ptr), "ohx_pc_probe_f64")
+            return out.download(np.complex128, len(pix))
+        finally:
+            dpix.free()
+            out.free()
+
+    def encode(self, field: DeviceBuffer, n_px: int, enc: DeviceBuffer, flag: int):
+        self._check(lib().ohx_encode(self.h, field.ptr, int(n_px), enc.ptr, int(flag)), "ohx_encode")
+
+    def normalize_u8(self, encs, n_px: int, outs):
+        a = (ctypes.c_void_p * len(encs))(*[e.ptr for e in encs])
+        b = (ctypes.c_void_p * len(outs))(*[o.ptr for o in outs])
+        self._check(lib().ohx_normalize_u8(self.h, a, len(encs), int(n_px), b), "ohx_normalize_u8")
+
+    def last_stats(self) -> LaunchStats:
+        s = LaunchStats()
+        self._check(lib().ohx_last_launch_stats(self.h, ctypes.byref(s)), "ohx_last_launch_stats")
+        return s
+
+    def mem_info(self):
+        f, t = ctypes.c_size_t(), ctypes.c_size_t()
+        self._check(lib().ohx_mem_info(self.h, ctypes.byref(f), ctypes.byref(t)), "ohx_mem_info")
+        return f.value, t.value
+
+
+def generate(ctx: Context, vertices: np.ndarray, pn, pp, lambdas, diff: int, mode: int,
+             scale=(1.0, 1.0, 1.0), distance=0.0, offset=(0, 0), use_phase=False) -> list[np.ndarray]:
+    """Convenience: upload vertices, generate each channel, download complex128 fields."""
+    n = len(vertices)
+    dv = ctx.to_device(vertices)
+    npx = int(pn[0]) * int(pn[1])
+    fld = ctx.alloc(16 * npx)
+    out = []
+    try:
+        for ch, lam in enumerate(lambdas):
+            ctx.field_zero(fld, npx)
+            p = params(pn, pp, lam, channel=ch, scale=scale, distance=distance, offset=offset,
+                       use_phase=use_phase)
+            ctx.diffract(dv, n, fld, p, diff, mode)
+            out.append(fld.download(np.complex128, npx).reshape(int(pn[1]), int(pn[0])))
+    finally:
+        dv.free()
+        fld.free()
+    return out

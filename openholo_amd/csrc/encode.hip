@@ -1,0 +1,169 @@
+// Per-pixel encoders and 8-bit normalisation (HBM-bound).
+//
+// ophGen::encoding(ENCODE_FLAG) -> Phase / Amplitude / RealPart /
+// ImaginaryPart (ophgen/src/ophGen.cpp:816-849, 1821-1864):
+//   phase = (atan2(im, re) + pi) / (2 pi), amplitude = |H|, real, imag.
+// ophGen::normalize (ophgen/src/ophGen.cpp:654-687) with minOfArr / maxOfArr
+// (Openholo/src/function.h:79-99): the minimum starts at DBL_MAX and the
+// maximum at DBL_MIN (the smallest *positive* normal, a reference quirk kept
+// here), both taken over all channels; then
+//   u8 = (uint8)(((v - min) / gap) * 255 + 0.5)   (truncating conversion).
+// Min/max are reduced per block (wave shuffles) and merged with 64-bit
+// atomics on an order-preserving integer image of the doubles; NaN is
+// skipped as the reference's < / > comparisons skip it.
+#include <cfloat>
+
+#include "ohx_internal.h"
+
+namespace ohx {
+
+constexpr int EB = 256;
+constexpr int MAXCH = 8;
+
+__device__ inline unsigned long long order_key(double d)
+{
+    const unsigned long long b = __double_as_longlong(d);
+    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+__device__ inline double key_value(unsigned long long k)
+{
+    const unsigned long long b = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
+    return __longlong_as_double((long long)b);
+}
+
+template <int FLAG>
+__global__ __launch_bounds__(EB) void k_encode(const double2* __restrict__ f, int64_t n, double* __restrict__ e)
+{
+    const double pi = 3.14159265358979323846;
+    const double pi2 = pi * 2;
+    int64_t i = (int64_t)blockIdx.x * EB + threadIdx.x;
+    const int64_t stride = (int64_t)gridDim.x * EB;
+    for (; i < n; i += stride) {
+        const double2 h = f[i];
+        double v;
+        if (FLAG == OHX_ENCODE_PHASE) v = (atan2(h.y, h.x) + pi) / pi2;
+        else if (FLAG == OHX_ENCODE_AMPLITUDE) v = sqrt(h.x * h.x + h.y * h.y);
+        else if (FLAG == OHX_ENCODE_REAL) v = h.x;
+        else v = h.y;
+        e[i] = v;
+    }
+}
+
+struct ChanPtrs {
+    const double* p[MAXCH];
+    uint8_t* o[MAXCH];
+};
+
+__global__ void k_minmax_init(unsigned long long* mm)
+{
+    if (threadIdx.x == 0) {
+        mm[0] = order_key(DBL_MAX);
+        mm[1] = order_key(DBL_MIN);
+    }
+}
+
+__global__ __launch_bounds__(EB) void k_minmax(ChanPtrs cp, int nch, int64_t n, unsigned long long* mm)
+{
+    unsigned long long kmin = ~0ull, kmax = 0ull;
+    const int64_t stride = (int64_t)gridDim.x * EB;
+    for (int c = 0; c < nch; ++c) {
+        const double* __restrict__ e = cp.p[c];
+        for (int64_t i = (int64_t)blockIdx.x * EB + threadIdx.x; i < n; i += stride) {
+            const double v = e[i];
+            if (v == v) {
+                const unsigned long long k = order_key(v);
+                kmin = k < kmin ? k : kmin;
+                kmax = k > kmax ? k : kmax;
+            }
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long a = __shfl_xor(kmin, off);
+        const unsigned long long b = __shfl_xor(kmax, off);
+        kmin = a < kmin ? a : kmin;
+        kmax = b > kmax ? b : kmax;
+    }
+    __shared__ unsigned long long smin[EB / 64], smax[EB / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) {
+        smin[wave] = kmin;
+        smax[wave] = kmax;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < EB / 64; ++w) {
+            kmin = smin[w] < kmin ? smin[w] : kmin;
+            kmax = smax[w] > kmax ? smax[w] : kmax;
+        }
+        if (kmin != ~0ull) atomicMin(&mm[0], kmin);
+        if (kmax != 0ull) atomicMax(&mm[1], kmax);
+    }
+}
+
+__global__ __launch_bounds__(EB) void k_to_u8(ChanPtrs cp, int nch, int64_t n, const unsigned long long* mm)
+{
+    const double mn = key_value(mm[0]), mx = key_value(mm[1]);
+    const double gap = mx - mn;
+    const int64_t stride = (int64_t)gridDim.x * EB;
+    for (int c = 0; c < nch; ++c) {
+        const double* __restrict__ e = cp.p[c];
+        uint8_t* __restrict__ o = cp.o[c];
+        for (int64_t i = (int64_t)blockIdx.x * EB + threadIdx.x; i < n; i += stride) {
+            const double t = ((e[i] - mn) / gap) * 255 + 0.5;
+            o[i] = (t >= 0.0 && t < 256.0) ? (uint8_t)(int)t : (uint8_t)0;
+        }
+    }
+}
+
+static unsigned egrid(int64_t n)
+{
+    int64_t g = (n + EB - 1) / EB;
+    if (g > 8192) g = 8192;
+    return (unsigned)(g < 1 ? 1 : g);
+}
+
+}  // namespace ohx
+
+using namespace ohx;
+
+extern "C" int ohx_encode(ohx_ctx* c, const double* d_field, int64_t n, double* d_enc, uint32_t flag)
+{
+    if (!c) return OHX_ERR_INVALID_ARG;
+    if (n < 0 || (n > 0 && (!d_field || !d_enc)))
+        return set_error(c, OHX_ERR_INVALID_ARG, "null buffer or negative size");
+    if (flag > OHX_ENCODE_IMAGINARY)
+        return set_error(c, OHX_ERR_UNSUPPORTED, "encode flag not on the point-cloud path");
+    if (n == 0) return OHX_OK;
+    (void)hipSetDevice(c->device);
+    const double2* f = (const double2*)d_field;
+    const dim3 g(egrid(n)), b(EB);
+    switch (flag) {
+    case OHX_ENCODE_PHASE: hipLaunchKernelGGL(k_encode<OHX_ENCODE_PHASE>, g, b, 0, c->stream, f, n, d_enc); break;
+    case OHX_ENCODE_AMPLITUDE: hipLaunchKernelGGL(k_encode<OHX_ENCODE_AMPLITUDE>, g, b, 0, c->stream, f, n, d_enc); break;
+    case OHX_ENCODE_REAL: hipLaunchKernelGGL(k_encode<OHX_ENCODE_REAL>, g, b, 0, c->stream, f, n, d_enc); break;
+    default: hipLaunchKernelGGL(k_encode<OHX_ENCODE_IMAGINARY>, g, b, 0, c->stream, f, n, d_enc); break;
+    }
+    return check_hip(c, hipGetLastError(), "k_encode");
+}
+
+extern "C" int ohx_normalize_u8(ohx_ctx* c, const double* const* d_enc, int nch, int64_t n,
+                                uint8_t* const* d_out)
+{
+    if (!c) return OHX_ERR_INVALID_ARG;
+    if (nch < 1 || nch > MAXCH || n < 0 || !d_enc || !d_out)
+        return set_error(c, OHX_ERR_INVALID_ARG, "bad channel count or buffers");
+    ChanPtrs cp{};
+    for (int i = 0; i < nch; ++i) {
+        if (n > 0 && (!d_enc[i] || !d_out[i])) return set_error(c, OHX_ERR_INVALID_ARG, "null channel buffer");
+        cp.p[i] = d_enc[i];
+        cp.o[i] = d_out[i];
+    }
+    if (n == 0) return OHX_OK;
+    std::lock_guard<std::mutex> lk(c->mu);
+    (void)hipSetDevice(c->device);
+    hipLaunchKernelGGL(k_minmax_init, dim3(1), dim3(64), 0, c->stream, c->minmax);
+    hipLaunchKernelGGL(k_minmax, dim3(egrid(n)), dim3(EB), 0, c->stream, cp, nch, n, c->minmax);
+    hipLaunchKernelGGL(k_to_u8, dim3(egrid(n)), dim3(EB), 0, c->stream, cp, nch, n,
+                       (const unsigned long long*)c->minmax);
+    return check_hip(c, hipGetLastError(), "k_normalize");
+}

@@ -1,0 +1,64 @@
+// Internal declarations shared by the HIP sources of libopenholo_amd.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <mutex>
+#include <string>
+
+#include "../../include/openholo_amd.h"
+
+struct ohx_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::mutex mu;                 // serialises use of the scratch buffers
+    std::string last_error;
+    // per-point SoA scratch (grown on demand)
+    void* pts = nullptr;
+    size_t pts_bytes = 0;
+    // 16-byte min/max scratch of the normaliser
+    unsigned long long* minmax = nullptr;
+    ohx_launch_stats last{};
+};
+
+namespace ohx {
+
+// Per-point constants produced by the prep kernel (structure of arrays).
+struct PointSoA {
+    double* px;     // scaled x
+    double* py;     // scaled y
+    double* pz;     // scaled z + distance
+    double* cyc0;   // frac(z/lambda [+ phase/2pi]) in cycles
+    double* kamp;   // Fresnel: a/(lambda z); RS: a z / lambda
+    double* q;      // Fresnel: 1/(2 lambda z); RS: unused (0)
+    int4* box;      // [x0,x1) x [y0,y1) mask box, clamped
+    int64_t n;
+};
+
+// Derived launch constants, computed on the host with the reference's
+// operation order (contraction off) so that box bounds are bit-identical.
+struct Geometry {
+    int32_t pn_x, pn_y, off_x, off_y;
+    double pp_x, pp_y;
+    double x0, y0;       // -ss_x/2, -ss_y/2
+    double lambda, inv_lambda;
+    double hx_coef, hy_coef;  // RS: tx/sqrt(1-tx^2); Fresnel: unused
+    double scale[3];
+    double distance;
+    int32_t channel, use_phase;
+    int32_t diff;
+};
+
+int set_error(ohx_ctx* ctx, int status, const std::string& msg);
+int check_hip(ohx_ctx* ctx, hipError_t e, const char* what);
+int geometry_from_params(ohx_ctx* ctx, const ohx_pc_params* p, uint32_t diff, Geometry* g);
+int ensure_points(ohx_ctx* ctx, int64_t n, PointSoA* soa);
+
+// launchers (pc_direct.hip)
+int launch_prep(ohx_ctx* ctx, const void* d_vertices, const Geometry& g, const PointSoA& soa);
+int launch_direct(ohx_ctx* ctx, const Geometry& g, const PointSoA& soa, double* d_field, bool fp64);
+int launch_probe(ohx_ctx* ctx, const Geometry& g, const PointSoA& soa, const int64_t* d_pix,
+                 int64_t n_pix, double* d_out);
+
+}  // namespace ohx

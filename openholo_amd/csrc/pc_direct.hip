@@ -1,0 +1,520 @@
+// Direct point-cloud diffraction sums for gfx950.
+//
+// Replaces the eight per-pixel kernels of ophgen/src/ophPCKernel.cu:66-872
+// (RS / Fresnel x double / single x plain / round-to-zero), whose maths is
+// ophGen::RS_Diffraction / Fresnel_Diffraction (ophgen/src/ophGen.cpp:361-514).
+//
+// Field model (per point p, pixel (x, y), cycles = radians / 2pi):
+//   RS:      H += a z / (lambda r^2) * (sin 2pi c - j cos 2pi c),  c = r / lambda
+//   Fresnel: H += a / (lambda z)     * (sin 2pi c - j cos 2pi c),  c = z/lambda + rho^2/(2 lambda z)
+// over the reference's anti-aliasing mask (integer box of ophGen.cpp:384-407 /
+// 467-490, plus the RS per-pixel range test of ophGen.cpp:415-429).
+//
+// Design (one workgroup = one 64 x 32 pixel tile, 4 waves, 8 rows per lane):
+//  * k_pc_prep turns the 56-byte Vertex records into FP64 structure-of-arrays
+//    constants, including the exact integer mask box (same FP64 operations,
+//    in the same order, as the reference, contraction off) and the per-point
+//    phase constant z/lambda reduced mod 1 cycle with an exact remainder.
+//  * Per 128-point chunk, one thread per point classifies the (point, tile)
+//    pair in FP64 -- OUT (mask empty on the tile), FAST (whole tile inside
+//    the mask, with margin), MASKED (per-pixel mask needed) or EXACT (FP32
+//    series not accurate enough: FP64 terms) -- and writes FP32 constants
+//    *recentred on the tile centre* into LDS, compacted in point order
+//    (ballot + mbcnt, deterministic).  Recentring keeps every FP32 phase
+//    below ~10 cycles, so FP32 rounding costs ~1e-6 rad instead of the ~1 rad
+//    a direct FP32 evaluation of k r ~ 6e6 rad would.
+//  * Fresnel phase is separable: c = phi_c + u (g_x + q u) + v (g_y + q v);
+//    the row term is tabulated in LDS per chunk, the column term is one FMA
+//    pair per lane per point, so a pair costs add, fract, v_sin, v_cos and
+//    two FMAs (12 VALU issue slots).
+//  * RS uses N = r^2 - r_c^2 = (2 dx_c u + u^2) + (2 dy_c v + v^2) and the
+//    series r - r_c = N/(2 r_c) - N^2/(8 r_c^3) (valid bound checked per pair
+//    in FP64), amplitude z/r^2 = (z/r_c^2)(1 - N/r_c^2): 15 slots per pair,
+//    no sqrt or reciprocal in the loop.
+//  * FP32 partial sums per chunk are folded into FP64 accumulators; the
+//    field (interleaved FP64 complex, the reference's Complex<Real> layout)
+//    is read-modify-written once per pixel at the end, 1 KB coalesced per row.
+#include "ohx_internal.h"
+
+namespace ohx {
+
+constexpr int TW = 64;            // tile width = lanes of a wave
+constexpr int WAVES = 4;
+constexpr int RPT = 8;            // rows per lane
+constexpr int TH = WAVES * RPT;   // tile height
+constexpr int NT = TW * WAVES;    // threads per workgroup
+constexpr int PCH = 128;          // points per LDS chunk
+
+enum { CLS_OUT = 0, CLS_FAST = 1, CLS_MASKED = 2, CLS_EXACT = 3 };
+
+constexpr double INV_2PI = 0.15915494309189533576888376337251436;
+// Largest phase excursion (cycles) over a tile that the FP32 path accepts:
+// FP32 rounding then stays below ~4e-6 cycles per term; larger spans (points
+// a few mm from the hologram) take the FP64 path.
+constexpr double MAX_CYC = 64.0;
+
+// ---------------------------------------------------------------------------
+// reference pixel coordinates, ophGen.cpp:413 and :422 (int expression times
+// pitch, then added; no contraction)
+__device__ inline double pix_x(const Geometry& g, int col)
+{
+#pragma clang fp contract(off)
+    return g.x0 + (double)(col - 1 + g.off_x) * g.pp_x;
+}
+__device__ inline double pix_y(const Geometry& g, int row)
+{
+#pragma clang fp contract(off)
+    return g.y0 + (double)(g.pn_y - row + g.off_y) * g.pp_y;
+}
+
+// RS per-pixel range test, ophGen.cpp:415-429, bit-identical operation order
+__device__ inline bool rs_range_ok(double px, double py, double zz, double hxc, double hyc,
+                                   double xxx, double yyy)
+{
+#pragma clang fp contract(off)
+    const double sy = sqrt((yyy - py) * (yyy - py) + zz);
+    const double sx = sqrt((xxx - px) * (xxx - px) + zz);
+    const double rx0 = px + fabs(hxc * sy), rx1 = px - fabs(hxc * sy);
+    const double ry0 = py + fabs(hyc * sx), ry1 = py - fabs(hyc * sx);
+    return ((xxx < rx0) && (xxx > rx1)) && ((yyy < ry0) && (yyy > ry1));
+}
+
+// ---------------------------------------------------------------------------
+// per-point constants: Vertex (56 B) -> SoA
+__global__ __launch_bounds__(256) void k_pc_prep(const double* __restrict__ vtx, Geometry g, PointSoA s)
+{
+#pragma clang fp contract(off)
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= s.n) return;
+    const double* v = vtx + 7 * i;
+    // genCghPointCloudCPU: scale, then distance (ophPointCloud.cpp:304-306, ophGen.cpp:379)
+    const double px = v[0] * g.scale[0];
+    const double py = v[1] * g.scale[1];
+    const double z = v[2] * g.scale[2] + g.distance;
+    const double a = v[3 + g.channel];
+    double hx, hy;
+    if (g.diff == OHX_DIFF_RS) {
+        hx = fabs(g.hx_coef * z);
+        hy = fabs(g.hy_coef * z);
+    } else {
+        const double operand = g.lambda * z;
+        hx = fabs(operand / (2 * g.pp_x));
+        hy = fabs(operand / (2 * g.pp_y));
+    }
+    // box bounds, ophGen.cpp:394-407 (x = -ss/2 = g.x0)
+    double X0 = floor(((px + hx) - g.x0) / g.pp_x) + 1;       // exclusive upper column
+    double X1 = floor(((px - hx) - g.x0) / g.pp_x) + 1;       // inclusive lower column
+    double Y0 = g.pn_y - floor(((py - hy) - g.y0) / g.pp_y);  // exclusive upper row
+    double Y1 = g.pn_y - floor(((py + hy) - g.y0) / g.pp_y);  // inclusive lower row
+    if (X0 > g.pn_x) X0 = g.pn_x;
+    if (X1 < 0) X1 = 0;
+    if (Y0 > g.pn_y) Y0 = g.pn_y;
+    if (Y1 < 0) Y1 = 0;
+    int4 b = make_int4(0, 0, 0, 0);
+    if (X1 < X0 && Y1 < Y0) b = make_int4((int)X1, (int)X0, (int)Y1, (int)Y0);
+
+    // z / lambda mod 1 with the exact remainder of the division
+    const double qz = z / g.lambda;
+    const double rem = fma(-qz, g.lambda, z);
+    double c = (qz - floor(qz)) + rem * g.inv_lambda;
+    if (g.use_phase) c += v[6] * INV_2PI;
+    c -= floor(c);
+
+    s.px[i] = px;
+    s.py[i] = py;
+    s.pz[i] = z;
+    s.cyc0[i] = c;
+    if (g.diff == OHX_DIFF_RS) {
+        s.kamp[i] = a * z / g.lambda;
+        s.q[i] = 0.0;
+    } else {
+        s.kamp[i] = a / (g.lambda * z);
+        s.q[i] = 0.5 / (g.lambda * z);
+    }
+    s.box[i] = b;
+}
+
+// ---------------------------------------------------------------------------
+__device__ inline float fract32(float x) { return __builtin_amdgcn_fractf(x); }
+__device__ inline float sin_rev(float x) { return __builtin_amdgcn_sinf(x); }  // sin(2 pi x)
+__device__ inline float cos_rev(float x) { return __builtin_amdgcn_cosf(x); }  // cos(2 pi x)
+
+// FP64 term (1/j) amp exp(j 2pi c) of one (point, pixel) pair, added to (re, im)
+template <int DIFF>
+__device__ inline void exact_term(double dx, double dy, double z, double cyc0, double kamp,
+                                  double q, double inv_lambda, double& re, double& im)
+{
+    const double rho2 = dx * dx + dy * dy;
+    double cyc, amp;
+    if (DIFF == OHX_DIFF_RS) {
+        const double r2 = rho2 + z * z;
+        const double r = sqrt(r2);
+        cyc = cyc0 + (rho2 / (r + z)) * inv_lambda;   // r/lambda = z/lambda + (r-z)/lambda
+        amp = kamp / r2;                               // a z / (lambda r^2)
+    } else {
+        cyc = cyc0 + rho2 * q;                         // z/lambda + rho^2/(2 lambda z)
+        amp = kamp;                                    // a / (lambda z)
+    }
+    cyc -= floor(cyc);
+    double sn, cs;
+    sincospi(2.0 * cyc, &sn, &cs);
+    re += amp * sn;
+    im -= amp * cs;
+}
+
+template <int DIFF, bool FP64>
+__global__ __launch_bounds__(NT) void k_pc_direct(Geometry g, PointSoA s, double2* __restrict__ field,
+                                                  int tiles_x)
+{
+    __shared__ float4 s_c0[PCH];
+    __shared__ float4 s_c1[PCH];
+    __shared__ float4 s_R[WAVES][PCH][2];
+    __shared__ int4 s_box[PCH];
+    __shared__ double s_d[6][PCH];   // px, py, z, cyc0, kamp, q (slow list)
+    __shared__ int s_cls[PCH];
+    __shared__ int s_cnt[2][WAVES];
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tx = blockIdx.x % tiles_x, ty = blockIdx.x / tiles_x;
+    const int x0 = tx * TW, y0 = ty * TH;
+    const int cx1 = min(x0 + TW, g.pn_x), ry1 = min(y0 + TH, g.pn_y);
+    const int col = x0 + lane;
+    const int rowb = y0 + wave * RPT;
+
+    // tile centre and this lane's offset from it
+    const double xc = pix_x(g, x0 + TW / 2), yc = pix_y(g, y0 + TH / 2);
+    const float u = (float)((double)(lane - TW / 2) * g.pp_x);
+    const float uu = u * u;
+    const double xxx = pix_x(g, col);
+    const float umax = (float)(TW / 2) * (float)g.pp_x, vmax = (float)(TH / 2) * (float)g.pp_y;
+
+    double accr[RPT], acci[RPT];
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) accr[k] = acci[k] = 0.0;
+
+    for (int64_t base = 0; base < s.n; base += PCH) {
+        // ---- classify and prepare one point per thread --------------------
+        int cls = CLS_OUT;
+        float4 c0 = make_float4(0, 0, 0, 0), c1 = make_float4(0, 0, 0, 0);
+        float rowc = 0.f;   // Fresnel: g_y; RS: 2 dy_c
+        int4 b = make_int4(0, 0, 0, 0);
+        double px = 0, py = 0, z = 0, cyc0 = 0, kamp = 0, q = 0;
+        const int64_t p = base + tid;
+        if (tid < PCH && p < s.n) {
+            b = s.box[p];
+            if (b.x < cx1 && b.y > x0 && b.z < ry1 && b.w > y0) {
+                const bool full = b.x <= x0 && b.y >= cx1 && b.z <= y0 && b.w >= ry1;
+                px = s.px[p];
+                py = s.py[p];
+                z = s.pz[p];
+                cyc0 = s.cyc0[p];
+                kamp = s.kamp[p];
+                q = s.q[p];
+                const double dxc = xc - px, dyc = yc - py;
+                if (FP64) {
+                    cls = CLS_EXACT;
+                } else if (DIFF == OHX_DIFF_FRESNEL) {
+                    // FP32 phase terms must stay below MAX_CYC cycles on the tile
+                    const double span = (2.0 * fabs(dxc) * umax + (double)umax * umax +
+                                         2.0 * fabs(dyc) * vmax + (double)vmax * vmax) * q;
+                    cls = span > MAX_CYC ? CLS_EXACT : (full ? CLS_FAST : CLS_MASKED);
+                    double ph = cyc0 + (dxc * dxc + dyc * dyc) * q;
+                    ph -= floor(ph);
+                    const float qf = (float)q;
+                    rowc = (float)(2.0 * dyc * q);
+                    c0 = make_float4((float)ph, (float)(2.0 * dxc * q), qf, (float)kamp);
+                } else {
+                    const double zz = z * z;
+                    const double rho2 = dxc * dxc + dyc * dyc;
+                    const double rc2 = rho2 + zz;
+                    const double rc = sqrt(rc2);
+                    // bound on |N| = |r^2 - r_c^2| over the tile
+                    const double nmax = 2.0 * umax * fabs(dxc) + (double)umax * umax +
+                                        2.0 * vmax * fabs(dyc) + (double)vmax * vmax;
+                    const double e = nmax / rc2;
+                    // third-order phase term e^3 r_c/(16 lambda) and second-order
+                    // amplitude term e^2 must be negligible
+                    const bool series_ok = e * e * e * rc * g.inv_lambda < 16 * 2e-7 && e * e < 1e-7 &&
+                                           nmax * 0.5 * g.inv_lambda / rc <= MAX_CYC;
+                    if (!series_ok) {
+                        cls = CLS_EXACT;
+                    } else {
+                        bool range_ok = false;
+                        if (full) {
+                            // reference range test holds on the whole tile with margin
+                            const double xl = pix_x(g, x0), xh = pix_x(g, cx1 - 1);
+                            const double yh = pix_y(g, y0), yl = pix_y(g, ry1 - 1);
+                            const double dxmax = fmax(fabs(xl - px), fabs(xh - px));
+                            const double dymax = fmax(fabs(yl - py), fabs(yh - py));
+                            const double dxmin = (px >= xl && px <= xh) ? 0.0 : fmin(fabs(xl - px), fabs(xh - px));
+                            const double dymin = (py >= yl && py <= yh) ? 0.0 : fmin(fabs(yl - py), fabs(yh - py));
+                            const double tol = 1e-12;
+                            range_ok = g.hx_coef * sqrt(dymin * dymin + zz) - dxmax > tol &&
+                                       g.hy_coef * sqrt(dxmin * dxmin + zz) - dymax > tol;
+                        }
+                        cls = range_ok ? CLS_FAST : CLS_MASKED;
+                        double ph = cyc0 + (rho2 / (rc + z)) * g.inv_lambda;
+                        ph -= floor(ph);
+                        const double hp = 0.5 * g.inv_lambda / rc;          // 1/(2 r_c lambda)
+                        const double c2 = hp / (4.0 * rc2);                  // 1/(8 r_c^3 lambda)
+                        const double az = kamp / rc2;                        // a z/(lambda r_c^2)
+                        const double m = az / rc2;
+                        c0 = make_float4((float)ph, (float)(2.0 * dxc), (float)(-c2), (float)hp);
+                        c1 = make_float4((float)(-m), (float)az, 0.f, 0.f);
+                        rowc = (float)(2.0 * dyc);
+                    }
+                }
+            }
+        }
+        // ---- deterministic compaction (point order) ------------------------
+        const unsigned long long bf = __ballot(cls == CLS_FAST);
+        const unsigned long long bs = __ballot(cls >= CLS_MASKED);
+        if (lane == 0) {
+            s_cnt[0][wave] = __popcll(bf);
+            s_cnt[1][wave] = __popcll(bs);
+        }
+        __syncthreads();
+        int nF = 0, nS = 0, offF = 0, offS = 0;
+#pragma unroll
+        for (int w = 0; w < WAVES; ++w) {
+            if (w < wave) {
+                offF += s_cnt[0][w];
+                offS += s_cnt[1][w];
+            }
+            nF += s_cnt[0][w];
+            nS += s_cnt[1][w];
+        }
+        if (cls != CLS_OUT) {
+            const unsigned long long m = (cls == CLS_FAST) ? bf : bs;
+            const int rank = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                                       __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+            const int slot = (cls == CLS_FAST) ? offF + rank : PCH - 1 - (offS + rank);
+            if (cls != CLS_EXACT) {
+                s_c0[slot] = c0;
+                s_c1[slot] = c1;
+                // row terms for the 32 rows of the tile: Fresnel v (g_y + q v),
+                // RS v (2 dy_c + v), with v = y_row - y_c
+#pragma unroll 2
+                for (int h = 0; h < TH / 4; ++h) {
+                    float t[4];
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        const float v = (float)(-(double)(4 * h + e - TH / 2) * g.pp_y);
+                        t[e] = (DIFF == OHX_DIFF_FRESNEL) ? v * fmaf(c0.z, v, rowc) : v * (rowc + v);
+                    }
+                    s_R[h / 2][slot][h & 1] = make_float4(t[0], t[1], t[2], t[3]);
+                }
+            }
+            if (cls >= CLS_MASKED) {
+                s_box[slot] = b;
+                s_cls[slot] = cls;
+                s_d[0][slot] = px;
+                s_d[1][slot] = py;
+                s_d[2][slot] = z;
+                s_d[3][slot] = cyc0;
+                s_d[4][slot] = kamp;
+                s_d[5][slot] = q;
+            }
+        }
+        __syncthreads();
+
+        // ---- fast list: whole tile inside the mask --------------------------
+        float pr[RPT], pim[RPT];
+#pragma unroll
+        for (int k = 0; k < RPT; ++k) pr[k] = pim[k] = 0.f;
+        for (int j = 0; j < nF; ++j) {
+            const float4 c = s_c0[j];
+            const float4 ra = s_R[wave][j][0], rb = s_R[wave][j][1];
+            const float Rv[RPT] = {ra.x, ra.y, ra.z, ra.w, rb.x, rb.y, rb.z, rb.w};
+            if (DIFF == OHX_DIFF_FRESNEL) {
+                const float bse = fmaf(fmaf(c.z, u, c.y), u, c.x);
+#pragma unroll
+                for (int k = 0; k < RPT; ++k) {
+                    const float f = fract32(bse + Rv[k]);
+                    pr[k] = fmaf(c.w, sin_rev(f), pr[k]);
+                    pim[k] = fmaf(c.w, cos_rev(f), pim[k]);
+                }
+            } else {
+                const float4 d = s_c1[j];
+                const float nx = fmaf(c.y, u, uu);
+#pragma unroll
+                for (int k = 0; k < RPT; ++k) {
+                    const float N = nx + Rv[k];
+                    const float f = fract32(fmaf(fmaf(c.z, N, c.w), N, c.x));
+                    const float amp = fmaf(d.x, N, d.y);
+                    pr[k] = fmaf(amp, sin_rev(f), pr[k]);
+                    pim[k] = fmaf(amp, cos_rev(f), pim[k]);
+                }
+            }
+        }
+        // ---- slow list: per-pixel mask and/or FP64 terms --------------------
+        for (int j = 0; j < nS; ++j) {
+            const int slot = PCH - 1 - j;
+            const int cl = s_cls[slot];
+            const int4 bb = s_box[slot];
+            const double qpx = s_d[0][slot], qpy = s_d[1][slot], qz = s_d[2][slot];
+            const bool colin = col >= bb.x && col < bb.y;
+            if (cl == CLS_MASKED) {
+                const float4 c = s_c0[slot];
+                const float* Rs = (const float*)&s_R[wave][slot][0];
+                const float4 d = s_c1[slot];
+                const double zz = qz * qz;
+#pragma unroll
+                for (int k = 0; k < RPT; ++k) {
+                    const int row = rowb + k;
+                    bool in = colin && row >= bb.z && row < bb.w;
+                    if (DIFF == OHX_DIFF_RS)
+                        in = in && rs_range_ok(qpx, qpy, zz, g.hx_coef, g.hy_coef, xxx, pix_y(g, row));
+                    float f, amp;
+                    if (DIFF == OHX_DIFF_FRESNEL) {
+                        f = fract32(fmaf(fmaf(c.z, u, c.y), u, c.x) + Rs[k]);
+                        amp = c.w;
+                    } else {
+                        const float N = fmaf(c.y, u, uu) + Rs[k];
+                        f = fract32(fmaf(fmaf(c.z, N, c.w), N, c.x));
+                        amp = fmaf(d.x, N, d.y);
+                    }
+                    amp = in ? amp : 0.f;
+                    pr[k] = fmaf(amp, sin_rev(f), pr[k]);
+                    pim[k] = fmaf(amp, cos_rev(f), pim[k]);
+                    __builtin_amdgcn_sched_barrier(0);  // keep one row live at a time
+                }
+            } else {
+                const double zz = qz * qz;
+                const double qc = s_d[3][slot], qk = s_d[4][slot], qq = s_d[5][slot];
+#pragma unroll
+                for (int k = 0; k < RPT; ++k) {
+                    const int row = rowb + k;
+                    const double yyy = pix_y(g, row);
+                    bool in = colin && row >= bb.z && row < bb.w;
+                    if (DIFF == OHX_DIFF_RS)
+                        in = in && rs_range_ok(qpx, qpy, zz, g.hx_coef, g.hy_coef, xxx, yyy);
+                    if (in) exact_term<DIFF>(xxx - qpx, yyy - qpy, qz, qc, qk, qq, g.inv_lambda, accr[k], acci[k]);
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < RPT; ++k) {
+            accr[k] += (double)pr[k];
+            acci[k] -= (double)pim[k];
+        }
+        __syncthreads();
+    }
+
+    if (col < g.pn_x) {
+#pragma unroll
+        for (int k = 0; k < RPT; ++k) {
+            const int row = rowb + k;
+            if (row < g.pn_y) {
+                const int64_t idx = (int64_t)row * g.pn_x + col;
+                double2 f = field[idx];
+                f.x += accr[k];
+                f.y += acci[k];
+                field[idx] = f;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// FP64 accuracy probe: one thread per listed pixel, Neumaier summation
+__device__ inline void neumaier(double& s, double& c, double x)
+{
+    const double t = s + x;
+    if (fabs(s) >= fabs(x)) c += (s - t) + x;
+    else c += (x - t) + s;
+    s = t;
+}
+
+template <int DIFF>
+__global__ __launch_bounds__(256) void k_pc_probe(Geometry g, PointSoA s, const int64_t* __restrict__ pix,
+                                                  int64_t npix, double2* __restrict__ out)
+{
+    __shared__ double sd[6][256];
+    __shared__ int4 sb[256];
+    const int tid = threadIdx.x;
+    const int64_t qi = (int64_t)blockIdx.x * 256 + tid;
+    const int64_t npx = (int64_t)g.pn_x * g.pn_y;
+    int64_t id = (qi < npix) ? pix[qi] : -1;
+    const bool valid = id >= 0 && id < npx;
+    const int col = valid ? (int)(id % g.pn_x) : 0;
+    const int row = valid ? (int)(id / g.pn_x) : 0;
+    const double xxx = pix_x(g, col), yyy = pix_y(g, row);
+    double sr = 0, cr = 0, si = 0, ci = 0;
+    for (int64_t base = 0; base < s.n; base += 256) {
+        const int64_t p = base + tid;
+        if (p < s.n) {
+            sd[0][tid] = s.px[p];
+            sd[1][tid] = s.py[p];
+            sd[2][tid] = s.pz[p];
+            sd[3][tid] = s.cyc0[p];
+            sd[4][tid] = s.kamp[p];
+            sd[5][tid] = s.q[p];
+            sb[tid] = s.box[p];
+        }
+        __syncthreads();
+        const int cnt = (int)min((int64_t)256, s.n - base);
+        for (int j = 0; j < cnt; ++j) {
+            const int4 b = sb[j];
+            bool in = valid && col >= b.x && col < b.y && row >= b.z && row < b.w;
+            if (DIFF == OHX_DIFF_RS && in)
+                in = rs_range_ok(sd[0][j], sd[1][j], sd[2][j] * sd[2][j], g.hx_coef, g.hy_coef, xxx, yyy);
+            if (in) {
+                double re = 0, im = 0;
+                exact_term<DIFF>(xxx - sd[0][j], yyy - sd[1][j], sd[2][j], sd[3][j], sd[4][j], sd[5][j],
+                                 g.inv_lambda, re, im);
+                neumaier(sr, cr, re);
+                neumaier(si, ci, im);
+            }
+        }
+        __syncthreads();
+    }
+    if (qi < npix) {
+        const double nan = __builtin_nan("");
+        out[qi] = valid ? make_double2(sr + cr, si + ci) : make_double2(nan, nan);
+    }
+}
+
+// ---------------------------------------------------------------------------
+int launch_prep(ohx_ctx* ctx, const void* d_vertices, const Geometry& g, const PointSoA& soa)
+{
+    const unsigned grid = (unsigned)((soa.n + 255) / 256);
+    hipLaunchKernelGGL(k_pc_prep, dim3(grid), dim3(256), 0, ctx->stream,
+                       (const double*)d_vertices, g, soa);
+    return check_hip(ctx, hipGetLastError(), "k_pc_prep");
+}
+
+int launch_direct(ohx_ctx* ctx, const Geometry& g, const PointSoA& soa, double* d_field, bool fp64)
+{
+    const int tiles_x = (g.pn_x + TW - 1) / TW, tiles_y = (g.pn_y + TH - 1) / TH;
+    const int64_t nwg = (int64_t)tiles_x * tiles_y;
+    if (nwg > 0x7fffffff) return set_error(ctx, OHX_ERR_INVALID_ARG, "hologram too large");
+    double2* f = (double2*)d_field;
+    const dim3 grid((unsigned)nwg), block(NT);
+    if (g.diff == OHX_DIFF_RS) {
+        if (fp64) hipLaunchKernelGGL((k_pc_direct<OHX_DIFF_RS, true>), grid, block, 0, ctx->stream, g, soa, f, tiles_x);
+        else hipLaunchKernelGGL((k_pc_direct<OHX_DIFF_RS, false>), grid, block, 0, ctx->stream, g, soa, f, tiles_x);
+    } else {
+        if (fp64) hipLaunchKernelGGL((k_pc_direct<OHX_DIFF_FRESNEL, true>), grid, block, 0, ctx->stream, g, soa, f, tiles_x);
+        else hipLaunchKernelGGL((k_pc_direct<OHX_DIFF_FRESNEL, false>), grid, block, 0, ctx->stream, g, soa, f, tiles_x);
+    }
+    ctx->last.workgroups = nwg;
+    ctx->last.kernel_kind = fp64 ? 1 : 0;
+    return check_hip(ctx, hipGetLastError(), "k_pc_direct");
+}
+
+int launch_probe(ohx_ctx* ctx, const Geometry& g, const PointSoA& soa, const int64_t* d_pix,
+                 int64_t n_pix, double* d_out)
+{
+    const unsigned grid = (unsigned)((n_pix + 255) / 256);
+    if (g.diff == OHX_DIFF_RS)
+        hipLaunchKernelGGL((k_pc_probe<OHX_DIFF_RS>), dim3(grid), dim3(256), 0, ctx->stream, g, soa, d_pix, n_pix,
+                           (double2*)d_out);
+    else
+        hipLaunchKernelGGL((k_pc_probe<OHX_DIFF_FRESNEL>), dim3(grid), dim3(256), 0, ctx->stream, g, soa, d_pix,
+                           n_pix, (double2*)d_out);
+    return check_hip(ctx, hipGetLastError(), "k_pc_probe");
+}
+
+}  // namespace ohx

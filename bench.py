@@ -28,11 +28,17 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-# FP32 VALU issue slots per (point, pixel) pair of the direct kernels as
-# implemented (transcendentals v_sin/v_cos count 4 slots): see DESIGN.md.
-SLOTS_PER_PAIR = {0: 15, 1: 12}
+# FP32 FLOPs per (point, pixel) pair executed by the direct kernels' fast
+# path (FMA = 2, every other VALU op incl. v_sin/v_cos/v_fract = 1):
+#   RS:      add, 2 fma (phase series), fract, sin, cos, fma (amplitude), 2 fma (accumulate) = 14
+#   Fresnel: add, fract, sin, cos, 2 fma (accumulate)                                      = 8
+FLOP_PER_PAIR = {0: 14, 1: 8}
 PEAK_FP32_TFLOPS = 157.3          # published MI355X FP32 vector peak
-PEAK_HBM_GBS = 8000.0
+PEAK_BF16_TFLOPS = 2500.0         # published dense BF16/FP16 matrix peak
+MODE_GPU, MODE_FLOAT = 1, 2
+KERNELS = {"direct": MODE_GPU | MODE_FLOAT, "fp64": MODE_GPU, "bf16x1": MODE_GPU | MODE_FLOAT | 0x10,
+           "bf16x3": MODE_GPU | MODE_FLOAT | 0x20, "fp16x3": MODE_GPU | MODE_FLOAT | 0x40}
+MFMA_TERMS = {"bf16x1": 1, "bf16x3": 3, "fp16x3": 3}
 
 
 def parse():
@@ -42,7 +48,9 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", type=int, default=2, choices=[1, 2, 3, 5])
     ap.add_argument("--points", type=int, default=0, help="override point count (0 = workload's)")
-    ap.add_argument("--mode", default="fp32", choices=["fp32", "fp64"])
+    ap.add_argument("--kernel", default="direct", choices=list(KERNELS))
+    ap.add_argument("--variants", type=int, default=1,
+                    help="also time the 1M-point Fresnel workload on every Fresnel kernel (0 = off)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--probe-pixels", type=int, default=512)
@@ -76,6 +84,57 @@ def cpu_baseline(w, target_s: float):
                       f"(oracle/ restatement of genCghPointCloudCPU, g++ -O3 -fopenmp)"}
 
 
+# HBM bytes per launch of the dominant direct kernels at the default workload
+# (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, scaled by the counters' reading of
+# a copy kernel of known size); see profiles/direct_100k_2160p_pmc.txt.
+TRAFFIC = {}
+
+
+def fresnel_variants(ctx, capi, synth, Event, launches=2):
+    """The 1M-point complex-amplitude Fresnel workload (3840x2160, 532 nm) on
+    the FP32 direct kernel and every matrix-core variant: pairs/s (HIP events
+    on the launch stream) and field rel-L2 against the FP64 probe."""
+    w = synth.WORKLOADS[3]
+    n, npx = w.n_points, w.n_pixels
+    pts = synth.points(w, n)
+    dv = ctx.to_device(pts)
+    fld = ctx.alloc(16 * npx)
+    p = capi.params(w.pn, w.pp, w.lambdas[0], use_phase=True)
+    pix = np.random.default_rng(99).choice(npx, 256, replace=False)
+    ref = ctx.probe_f64(dv, n, p, 1, pix)
+    out = {"workload": w.name, "points": n, "resolution": list(w.pn), "probe_pixels": len(pix)}
+    for name in ("direct", "bf16x1", "bf16x3", "fp16x3"):
+        mode = KERNELS[name]
+        ctx.field_zero(fld, npx)
+        ctx.diffract(dv, n, fld, p, 1, mode)          # warm-up, also the accuracy sample
+        ctx.sync()
+        f = fld.download(np.complex128, npx)[pix]
+        rel = float(np.linalg.norm(f - ref) / np.linalg.norm(ref))
+        ts = []
+        for _ in range(launches):
+            a, b = Event(), Event()
+            a.record(ctx.stream)
+            ctx.diffract(dv, n, fld, p, 1, mode)
+            b.record(ctx.stream)
+            ts.append((a, b))
+        ctx.sync()
+        ms = float(np.mean([a.elapsed_ms(b) for a, b in ts]))
+        gps = n * npx / (ms / 1e3) / 1e9
+        entry = {"ms": round(ms, 2), "Gpairs_per_s": round(gps, 1), "rel_l2_vs_fp64": float(f"{rel:.3e}")}
+        if name in MFMA_TERMS:
+            tf = n * npx * 8 * MFMA_TERMS[name] / (ms / 1e3) / 1e12
+            entry["mfma_tflops"] = round(tf, 1)
+            entry["mfma_frac"] = round(tf / PEAK_BF16_TFLOPS, 4)
+        out[name] = entry
+        print(f"# variant {name}: {ms:.1f} ms, {gps:.0f} Gpairs/s, rel-L2 {rel:.2e}", file=sys.stderr, flush=True)
+    d = out["direct"]["Gpairs_per_s"]
+    for name in MFMA_TERMS:
+        out[name]["speedup_vs_direct"] = round(out[name]["Gpairs_per_s"] / d, 2)
+    dv.free()
+    fld.free()
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -88,7 +147,7 @@ def main():
 
     w = synth.WORKLOADS[args.workload]
     n = args.points or w.n_points
-    mode = capi.MODE_GPU | (capi.MODE_FLOAT if args.mode == "fp32" else 0)
+    mode = KERNELS[args.kernel]
     npx = w.n_pixels
     nch = len(w.lambdas)
     pts = synth.points(w, n)
@@ -154,15 +213,20 @@ def main():
     launches = kev.get(dom, [])
     k_ms = float(np.mean([a.elapsed_ms(b) for a, b in launches])) if launches else float("nan")
     pairs_launch = n * npx
-    slots = SLOTS_PER_PAIR[dom] if mode != capi.MODE_GPU else None
-    if slots:
-        achieved = pairs_launch / (k_ms / 1e3) * slots * 2 / 1e12
+    if args.kernel == "direct":
+        fl = FLOP_PER_PAIR[dom]
+        achieved = pairs_launch / (k_ms / 1e3) * fl / 1e12
         roofline = {"bound": "valu", "achieved": round(achieved, 2), "peak": PEAK_FP32_TFLOPS,
-                    "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
+                    "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": TRAFFIC.get(dom),
                     "kernel": f"k_pc_direct<{'RS' if dom == 0 else 'Fresnel'},FP32>",
                     "avg_launch_ms": round(k_ms, 3),
-                    "model": f"{slots} FP32 issue slots/pair x 2 FLOP/slot; packed FP32 is not used, "
-                             f"so 0.5 of the published peak is the ceiling"}
+                    "model": f"{fl} FP32 FLOP/pair executed (FMA=2, sin/cos/fract/add=1)"}
+    elif args.kernel in MFMA_TERMS and dom == 1:
+        achieved = pairs_launch / (k_ms / 1e3) * 8 * MFMA_TERMS[args.kernel] / 1e12
+        roofline = {"bound": "mfma", "achieved": round(achieved, 2), "peak": PEAK_BF16_TFLOPS,
+                    "unit": "TFLOP/s", "frac": round(achieved / PEAK_BF16_TFLOPS, 4), "traffic": None,
+                    "kernel": f"k_fresnel_gemm<{args.kernel}>", "avg_launch_ms": round(k_ms, 3),
+                    "model": f"8 FLOP per complex MAC x {MFMA_TERMS[args.kernel]} term(s)"}
     else:
         roofline = {"bound": "valu", "achieved": None, "peak": 78.6, "unit": "TFLOP/s", "frac": None,
                     "traffic": None, "avg_launch_ms": round(k_ms, 3)}
@@ -189,6 +253,7 @@ def main():
 
     for b in [dv] + fields + encs + u8s:
         b.free()
+    var = fresnel_variants(ctx, capi, synth, Event) if args.variants else None
     ctx.close()
 
     cpu = None
@@ -200,12 +265,11 @@ def main():
         "metric": "point-pixel pairs/s (G/s) + field rel-L2 vs FP64; p50/p99 ms per hologram",
         "value": round(value, 3), "unit": "Gpairs/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms_step, 3), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32" if args.mode == "fp32" else "f64",
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64" if args.kernel == "fp64" else "f32",
         "data": "synthetic (SplitMix64 point cloud)",
         "config": {"workload": w.name, "points": n, "resolution": list(w.pn), "pitch_m": list(w.pp),
                    "lambdas_m": list(w.lambdas), "modes": ["RS" if m == 0 else "Fresnel" for m in w.modes],
-                   "kernel": "direct sum FP32 (FP64 per-point phase constants)" if args.mode == "fp32"
-                   else "direct sum FP64", "parallelism": "single GPU"},
+                   "kernel": args.kernel, "parallelism": "single GPU"},
         "field_rel_l2_vs_fp64": {k: float(f"{v:.3e}") for k, v in rel.items()},
         "probe_pixels": int(len(pix)),
         "p50_ms_per_hologram": round(p50, 3) if p50 else None,
@@ -213,6 +277,8 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+    if var:
+        line["variants"] = var
     print(json.dumps(line), flush=True)
 
 

@@ -74,7 +74,7 @@ int ensure_points(ohx_ctx* ctx, int64_t n, PointSoA* soa)
         if (ctx->pts) {
             int st = check_hip(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
             if (st) return st;
-            hipFree(ctx->pts);
+            (void)hipFree(ctx->pts);
             ctx->pts = nullptr;
             ctx->pts_bytes = 0;
         }
@@ -97,6 +97,22 @@ int ensure_points(ohx_ctx* ctx, int64_t n, PointSoA* soa)
     soa->q = (double*)take(n * sizeof(double));
     soa->box = (int4*)take(n * sizeof(int4));
     soa->n = n;
+    return OHX_OK;
+}
+
+int ensure_gemm_scratch(ohx_ctx* ctx, size_t bytes)
+{
+    if (bytes <= ctx->gemm_scratch_bytes) return OHX_OK;
+    if (ctx->gemm_scratch) {
+        int st = check_hip(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+        if (st) return st;
+        (void)hipFree(ctx->gemm_scratch);
+        ctx->gemm_scratch = nullptr;
+        ctx->gemm_scratch_bytes = 0;
+    }
+    int st = check_hip(ctx, hipMalloc(&ctx->gemm_scratch, bytes), "hipMalloc(gemm scratch)");
+    if (st) return st;
+    ctx->gemm_scratch_bytes = bytes;
     return OHX_OK;
 }
 
@@ -166,7 +182,7 @@ int ohx_ctx_create(int device, ohx_ctx** out)
     if (hipSetDevice(device) != hipSuccess ||
         hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipMalloc(&c->minmax, 4 * sizeof(unsigned long long)) != hipSuccess) {
-        if (c->stream) hipStreamDestroy(c->stream);
+        if (c->stream) (void)hipStreamDestroy(c->stream);
         delete c;
         return OHX_ERR_DEVICE;
     }
@@ -178,10 +194,11 @@ int ohx_ctx_destroy(ohx_ctx* c)
 {
     if (!c) return OHX_ERR_INVALID_ARG;
     (void)hipSetDevice(c->device);
-    hipStreamSynchronize(c->stream);
-    if (c->pts) hipFree(c->pts);
-    if (c->minmax) hipFree(c->minmax);
-    hipStreamDestroy(c->stream);
+    (void)hipStreamSynchronize(c->stream);
+    if (c->pts) (void)hipFree(c->pts);
+    if (c->minmax) (void)hipFree(c->minmax);
+    if (c->gemm_scratch) (void)hipFree(c->gemm_scratch);
+    (void)hipStreamDestroy(c->stream);
     delete c;
     return OHX_OK;
 }
@@ -273,8 +290,9 @@ int ohx_pc_diffract(ohx_ctx* c, const void* d_vertices, int64_t n, double* d_fie
     PointSoA soa;
     if ((st = ensure_points(c, n, &soa))) return st;
     if ((st = launch_prep(c, d_vertices, g, soa))) return st;
-    const bool fp64 = (mode & (OHX_MODE_FLOAT | OHX_MODE_FASTMATH)) == 0 &&
-                      (mode & (OHX_MODE_MFMA_BF16X1 | OHX_MODE_MFMA_BF16X3 | OHX_MODE_MFMA_FP16X3)) == 0;
+    const uint32_t mfma = mode & (OHX_MODE_MFMA_BF16X1 | OHX_MODE_MFMA_BF16X3 | OHX_MODE_MFMA_FP16X3);
+    if (mfma && diff == OHX_DIFF_FRESNEL) return launch_fresnel_gemm(c, g, soa, d_field, mode);
+    const bool fp64 = (mode & (OHX_MODE_FLOAT | OHX_MODE_FASTMATH)) == 0 && mfma == 0;
     return launch_direct(c, g, soa, d_field, fp64);
 }
 

@@ -19,6 +19,9 @@ struct ohx_ctx {
     size_t pts_bytes = 0;
     // 16-byte min/max scratch of the normaliser
     unsigned long long* minmax = nullptr;
+    // split-K partial sums of the matrix-core Fresnel path (grown on demand)
+    void* gemm_scratch = nullptr;
+    size_t gemm_scratch_bytes = 0;
     ohx_launch_stats last{};
 };
 
@@ -54,10 +57,12 @@ int set_error(ohx_ctx* ctx, int status, const std::string& msg);
 int check_hip(ohx_ctx* ctx, hipError_t e, const char* what);
 int geometry_from_params(ohx_ctx* ctx, const ohx_pc_params* p, uint32_t diff, Geometry* g);
 int ensure_points(ohx_ctx* ctx, int64_t n, PointSoA* soa);
+int ensure_gemm_scratch(ohx_ctx* ctx, size_t bytes);
 
 // launchers (pc_direct.hip)
 int launch_prep(ohx_ctx* ctx, const void* d_vertices, const Geometry& g, const PointSoA& soa);
 int launch_direct(ohx_ctx* ctx, const Geometry& g, const PointSoA& soa, double* d_field, bool fp64);
+int launch_fresnel_gemm(ohx_ctx* ctx, const Geometry& g, const PointSoA& soa, double* d_field, uint32_t mode);
 int launch_probe(ohx_ctx* ctx, const Geometry& g, const PointSoA& soa, const int64_t* d_pix,
                  int64_t n_pix, double* d_out);
 

@@ -128,19 +128,24 @@ def test_complex_amplitude_extension(ctx, orc):
 
 
 def test_partial_masks_large_geometry(ctx, orc):
-    """7680x4320 at 3.74 um with z down to 0.3 m: masks are partial, so many
-    tiles take the per-pixel mask path.  Mask exact, values within FP32."""
+    """7680x4320 at 3.74 um, lambda 473 nm, z down to 0.3 m: single-point
+    masks are partial, so tiles straddling a mask edge take the per-pixel
+    mask path.  Each point's support is bit-exact; values within FP32."""
     w = synth.WORKLOADS[5]
-    v = synth.points(w, 24)
     g = orc.make_geom(w.pn, w.pp)
     lam = w.lambdas[2]
+    v = np.concatenate([synth.points(w, 3), _one(11.0e-3, -9.5e-3, 0.30, 0.8),
+                        _one(-10.2e-3, 7.7e-3, 0.33, 0.5)])
+    partial = 0
     for diff in (0, 1):
-        want = orc.gen_restate(g, v, [lam], diff)[0]
-        got = capi.generate(ctx, v, w.pn, w.pp, [lam], diff, FP32, )[0]
-        nz_w, nz_g = want != 0, got != 0
-        assert nz_w.sum() < nz_w.size          # genuinely partial
-        assert np.array_equal(nz_w, nz_g)
-        assert rel_l2(got, want) < 1e-5
+        for i in range(len(v)):
+            want = orc.gen_restate(g, v[i:i + 1], [lam], diff)[0]
+            got = capi.generate(ctx, v[i:i + 1], w.pn, w.pp, [lam], diff, FP32)[0]
+            nz_w = want != 0
+            partial += int(nz_w.sum() < nz_w.size)
+            assert np.array_equal(nz_w, got != 0), (diff, i)
+            assert rel_l2(got, want) < 1e-5, (diff, i)
+    assert partial >= 4
 
 
 # ---------------------------------------------------------------------------

@@ -23,6 +23,8 @@
 //   FP16x3 : same split in FP16 (~2^-22), V scaled by 1/max|A| to stay in range
 // Split-K slices write FP32 partials; k_gemm_reduce sums them in FP64 in a fixed
 // order (deterministic) and adds -j S into the FP64 field.
+#include <algorithm>
+
 #include "ohx_internal.h"
 
 namespace ohx {
@@ -30,7 +32,7 @@ namespace ohx {
 namespace gemm {
 
 constexpr int BM = 128;           // tile rows (y)
-constexpr int BN = 128;           // tile columns (x), complex
+constexpr int BN = 128;           // tile columns (x), complex; wave w owns 32 of them
 constexpr int NT = 256;           // 4 waves
 constexpr int KP = 16;            // points per stage
 constexpr int KS = 2 * KP;        // realified K per stage (32)
@@ -53,49 +55,51 @@ struct Traits {
     static constexpr bool fp16 = VAR == V_FP16X3;
 };
 
-// pack two floats (round to nearest even) into one dword of two 16-bit values
+// pack two floats into one dword of two 16-bit values (BF16: round to nearest
+// even, v_cvt_pk_bf16_f32; FP16: round toward zero, v_cvt_pkrtz_f16_f32 -- the
+// 3-term split keeps the residual either way)
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 template <bool FP16>
 __device__ inline unsigned pack2(float a, float b)
 {
-    if (FP16) {
-        const _Float16 ha = (_Float16)a, hb = (_Float16)b;
-        return (unsigned)__builtin_bit_cast(unsigned short, ha) |
-               ((unsigned)__builtin_bit_cast(unsigned short, hb) << 16);
+    if constexpr (FP16) {
+        return __builtin_bit_cast(unsigned, __builtin_amdgcn_cvt_pkrtz(a, b));
     } else {
-        const __bf16 ha = (__bf16)a, hb = (__bf16)b;
-        return (unsigned)__builtin_bit_cast(unsigned short, ha) |
-               ((unsigned)__builtin_bit_cast(unsigned short, hb) << 16);
+        const bf16x2 v = {(__bf16)a, (__bf16)b};
+        return __builtin_bit_cast(unsigned, v);
     }
 }
 template <bool FP16>
 __device__ inline float lo16(unsigned w)
 {
-    if (FP16) return (float)__builtin_bit_cast(_Float16, (unsigned short)(w & 0xffffu));
+    if constexpr (FP16) return (float)__builtin_bit_cast(_Float16, (unsigned short)(w & 0xffffu));
     return __builtin_bit_cast(float, w << 16);
 }
 template <bool FP16>
 __device__ inline float hi16(unsigned w)
 {
-    if (FP16) return (float)__builtin_bit_cast(_Float16, (unsigned short)(w >> 16));
+    if constexpr (FP16) return (float)__builtin_bit_cast(_Float16, (unsigned short)(w >> 16));
     return __builtin_bit_cast(float, w & 0xffff0000u);
 }
 
 template <int VAR>
 __device__ inline f32x16 mfma(u32x4 a, u32x4 b, f32x16 c)
 {
-    if (Traits<VAR>::fp16)
+    if constexpr (Traits<VAR>::fp16)
         return __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(f16x8, a), __builtin_bit_cast(f16x8, b),
                                                       c, 0, 0, 0);
-    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b),
-                                                   c, 0, 0, 0);
+    else
+        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a),
+                                                       __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
 }
 
+// LDS: V[y][p] and U[x][p] as packed (Re, Im) 16-bit pairs, hi and (3-term) lo
+// parts, double buffered; rows padded to 80 bytes (conflict-free ds_read_b128).
 template <bool THREE>
 struct Smem {
     static constexpr int NH = THREE ? 2 : 1;
-    // [buffer][hi/lo][row][dword]
     unsigned v[2][NH][BM][ROWW];
-    unsigned u[2][NH][2 * BN][ROWW];
+    unsigned u[2][NH][BN][ROWW];
 };
 
 __device__ inline double gpix_x(const Geometry& g, int col)
@@ -109,10 +113,12 @@ __device__ inline double gpix_y(const Geometry& g, int row)
     return g.y0 + (double)(g.pn_y - row + g.off_y) * g.pp_y;
 }
 
-// generate one stage (KP points starting at p0, valid below p1) into buffer b
+// One stage: KP points from p0 (valid below p1) into buffer b.  Thread t
+// handles point t % 16, V rows and U columns t/16 + 16 i (i < 8).
 template <int VAR>
-__device__ inline void generate(Smem<Traits<VAR>::three>& sm, int b, const Geometry& g, const PointSoA& s, int64_t p0, int64_t p1,
-                                int x0, int y0, double xc, double yc, float vscale)
+__device__ inline void generate(Smem<Traits<VAR>::three>& sm, int b, const Geometry& g, const PointSoA& s,
+                                int64_t p0, int64_t p1, int x0, int y0, double xc, double yc, float vscale,
+                                float v0, float u0, float ppy16, float ppx16)
 {
     constexpr bool FP16 = Traits<VAR>::fp16;
     constexpr bool THREE = Traits<VAR>::three;
@@ -122,10 +128,14 @@ __device__ inline void generate(Smem<Traits<VAR>::three>& sm, int b, const Geome
     float ac = 0.f, gx = 0.f, bc = 0.f, gy = 0.f, qf = 0.f, amp = 0.f;
     double a64 = 0, b64 = 0, gx64 = 0, gy64 = 0, q64 = 0;
     bool big = false;
-    int4 box = make_int4(0, 0, 0, 0);
+    int rlo = 0, rhi = 0, clo = 0, chi = 0;
     if (pi < p1) {
         const double px = s.px[pi], py = s.py[pi], q = s.q[pi];
-        box = s.box[pi];
+        const int4 box = s.box[pi];
+        rlo = box.z - y0;
+        rhi = box.w - y0;
+        clo = box.x - x0;
+        chi = box.y - x0;
         const double dxc = xc - px, dyc = yc - py;
         a64 = dxc * dxc * q;
         a64 -= floor(a64);
@@ -145,22 +155,19 @@ __device__ inline void generate(Smem<Traits<VAR>::three>& sm, int b, const Geome
         const double um = (BN / 2) * g.pp_x, vm = (BM / 2) * g.pp_y;
         big = (fabs(gx64) * um + q * um * um + fabs(gy64) * vm + q * vm * vm) > MAX_SPAN;
     }
-    // V rows: r = grp + 16 i ; U columns: c = grp + 16 i
 #pragma unroll
     for (int i = 0; i < BM / 16; ++i) {
         const int r = grp + 16 * i;
-        const int row = y0 + r;
-        const double vd = -(double)(r - BM / 2) * g.pp_y;
-        const float v = (float)vd;
+        const float v = fmaf(-(float)i, ppy16, v0);          // y_r - y_c
         float f;
         if (big) {
-            double ph = b64 + vd * (gy64 + q64 * vd);
+            const double vd = (double)(BM / 2 - r) * g.pp_y;
+            const double ph = b64 + vd * (gy64 + q64 * vd);
             f = (float)(ph - floor(ph));
         } else {
             f = __builtin_amdgcn_fractf(fmaf(fmaf(qf, v, gy), v, bc));
         }
-        const bool in = row >= box.z && row < box.w;
-        const float a = in ? amp : 0.f;
+        const float a = (r >= rlo && r < rhi) ? amp : 0.f;
         const float vr = a * __builtin_amdgcn_cosf(f), vi = a * __builtin_amdgcn_sinf(f);
         const unsigned h = pack2<FP16>(vr, vi);
         sm.v[b][0][r][p] = h;
@@ -169,35 +176,38 @@ __device__ inline void generate(Smem<Traits<VAR>::three>& sm, int b, const Geome
 #pragma unroll
     for (int i = 0; i < BN / 16; ++i) {
         const int c = grp + 16 * i;
-        const int col = x0 + c;
-        const double ud = (double)(c - BN / 2) * g.pp_x;
-        const float u = (float)ud;
+        const float u = fmaf((float)i, ppx16, u0);            // x_c - x_c(tile)
         float f;
         if (big) {
-            double ph = a64 + ud * (gx64 + q64 * ud);
+            const double ud = (double)(c - BN / 2) * g.pp_x;
+            const double ph = a64 + ud * (gx64 + q64 * ud);
             f = (float)(ph - floor(ph));
         } else {
             f = __builtin_amdgcn_fractf(fmaf(fmaf(qf, u, gx), u, ac));
         }
-        const bool in = col >= box.x && col < box.y;
-        const float m = in ? 1.f : 0.f;
-        const float ur = m * __builtin_amdgcn_cosf(f), ui = m * __builtin_amdgcn_sinf(f);
-        // column 2c (Re S): (Re U, -Im U); column 2c+1 (Im S): (Im U, Re U)
-        const unsigned h0 = pack2<FP16>(ur, -ui), h1 = pack2<FP16>(ui, ur);
-        sm.u[b][0][2 * c][p] = h0;
-        sm.u[b][0][2 * c + 1][p] = h1;
-        if constexpr (THREE) {
-            const float lr = ur - lo16<FP16>(h0), li = ui - lo16<FP16>(h1);
-            sm.u[b][1][2 * c][p] = pack2<FP16>(lr, -li);
-            sm.u[b][1][2 * c + 1][p] = pack2<FP16>(li, lr);
-        }
+        const bool in = c >= clo && c < chi;
+        const float ur = in ? __builtin_amdgcn_cosf(f) : 0.f;
+        const float ui = in ? __builtin_amdgcn_sinf(f) : 0.f;
+        const unsigned h = pack2<FP16>(ur, ui);
+        sm.u[b][0][c][p] = h;
+        if constexpr (THREE) sm.u[b][1][c][p] = pack2<FP16>(ur - lo16<FP16>(h), ui - hi16<FP16>(h));
     }
 }
 
+// B' fragments from one U fragment: Re block (Re U, -Im U), Im block (Im U, Re U)
+__device__ inline u32x4 to_re(u32x4 w) { return w ^ 0x80000000u; }
+__device__ inline u32x4 to_im(u32x4 w)
+{
+    u32x4 r;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) r[i] = __builtin_amdgcn_alignbit(w[i], w[i], 16);
+    return r;
+}
+
 template <int VAR>
-__global__ __launch_bounds__(NT) void k_fresnel_gemm(Geometry g, PointSoA s, float* __restrict__ part,
-                                                     int tiles_x, int tiles, int64_t kslice,
-                                                     const float* __restrict__ amax)
+__global__ __launch_bounds__(NT, 2) void k_fresnel_gemm(Geometry g, PointSoA s, float2* __restrict__ part,
+                                                        int tiles_x, int tiles, int64_t kslice,
+                                                        const float* __restrict__ amax)
 {
     constexpr bool THREE = Traits<VAR>::three;
     __shared__ __align__(16) Smem<THREE> sm;
@@ -211,65 +221,70 @@ __global__ __launch_bounds__(NT) void k_fresnel_gemm(Geometry g, PointSoA s, flo
     const double xc = gpix_x(g, x0 + BN / 2), yc = gpix_y(g, y0 + BM / 2);
     const float am = amax[0];
     const float vscale = (Traits<VAR>::fp16 && am > 0.f) ? 1.0f / am : 1.0f;
+    const int grp = threadIdx.x >> 4;
+    const float v0 = (float)((double)(BM / 2 - grp) * g.pp_y);
+    const float u0 = (float)((double)(grp - BN / 2) * g.pp_x);
+    const float ppy16 = (float)(16.0 * g.pp_y), ppx16 = (float)(16.0 * g.pp_x);
 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     f32x16 acc[4][2];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j) acc[i][j] = f32x16{};
+    for (int i = 0; i < 4; ++i) acc[i][0] = acc[i][1] = f32x16{};
 
     const int nst = (int)((ke - kb + KP - 1) / KP);
-    if (nst > 0) generate<VAR>(sm, 0, g, s, kb, ke, x0, y0, xc, yc, vscale);
+    if (nst > 0) generate<VAR>(sm, 0, g, s, kb, ke, x0, y0, xc, yc, vscale, v0, u0, ppy16, ppx16);
     __syncthreads();
-    const int arow = lane & 31, koff = (lane >> 5) * 4;   // dword offset of this lane's 8 values
+    const int lrow = lane & 31, koff = (lane >> 5) * 4;   // dword offset of this lane's 8 values
+    const int ucol = wave * 32 + lrow;                      // this wave's 32 columns
     for (int st = 0; st < nst; ++st) {
         const int b = st & 1;
-        if (st + 1 < nst) generate<VAR>(sm, b ^ 1, g, s, kb + (int64_t)(st + 1) * KP, ke, x0, y0, xc, yc, vscale);
+        if (st + 1 < nst)
+            generate<VAR>(sm, b ^ 1, g, s, kb + (int64_t)(st + 1) * KP, ke, x0, y0, xc, yc, vscale, v0, u0,
+                          ppy16, ppx16);
 #pragma unroll
         for (int ks = 0; ks < 2; ++ks) {
             const int kd = ks * 8 + koff;
-            u32x4 ah[4], al[4], bh[2], bl[2];
+            u32x4 ah[4], al[4];
 #pragma unroll
             for (int mb = 0; mb < 4; ++mb) {
-                ah[mb] = *reinterpret_cast<const u32x4*>(&sm.v[b][0][mb * 32 + arow][kd]);
-                if constexpr (THREE) al[mb] = *reinterpret_cast<const u32x4*>(&sm.v[b][1][mb * 32 + arow][kd]);
+                ah[mb] = *reinterpret_cast<const u32x4*>(&sm.v[b][0][mb * 32 + lrow][kd]);
+                if constexpr (THREE) al[mb] = *reinterpret_cast<const u32x4*>(&sm.v[b][1][mb * 32 + lrow][kd]);
+            }
+            const u32x4 uh = *reinterpret_cast<const u32x4*>(&sm.u[b][0][ucol][kd]);
+            const u32x4 bhr = to_re(uh), bhi = to_im(uh);
+            u32x4 blr, bli;
+            if constexpr (THREE) {
+                const u32x4 ul = *reinterpret_cast<const u32x4*>(&sm.u[b][1][ucol][kd]);
+                blr = to_re(ul);
+                bli = to_im(ul);
             }
 #pragma unroll
-            for (int nb = 0; nb < 2; ++nb) {
-                const int col = wave * 64 + nb * 32 + arow;
-                bh[nb] = *reinterpret_cast<const u32x4*>(&sm.u[b][0][col][kd]);
-                if constexpr (THREE) bl[nb] = *reinterpret_cast<const u32x4*>(&sm.u[b][1][col][kd]);
-            }
-#pragma unroll
-            for (int mb = 0; mb < 4; ++mb)
-#pragma unroll
-                for (int nb = 0; nb < 2; ++nb) {
-                    acc[mb][nb] = mfma<VAR>(ah[mb], bh[nb], acc[mb][nb]);
-                    if constexpr (THREE) {
-                        acc[mb][nb] = mfma<VAR>(ah[mb], bl[nb], acc[mb][nb]);
-                        acc[mb][nb] = mfma<VAR>(al[mb], bh[nb], acc[mb][nb]);
-                    }
+            for (int mb = 0; mb < 4; ++mb) {
+                acc[mb][0] = mfma<VAR>(ah[mb], bhr, acc[mb][0]);
+                acc[mb][1] = mfma<VAR>(ah[mb], bhi, acc[mb][1]);
+                if constexpr (THREE) {
+                    acc[mb][0] = mfma<VAR>(ah[mb], blr, acc[mb][0]);
+                    acc[mb][1] = mfma<VAR>(ah[mb], bli, acc[mb][1]);
+                    acc[mb][0] = mfma<VAR>(al[mb], bhr, acc[mb][0]);
+                    acc[mb][1] = mfma<VAR>(al[mb], bhi, acc[mb][1]);
                 }
+            }
         }
         __syncthreads();
     }
 
-    // write FP32 partial sums S (interleaved re, im) of this split
-    float* out = part + (int64_t)split * g.pn_y * (int64_t)g.pn_x * 2;
+    // FP32 partial sums S of this split, interleaved (Re, Im) per pixel
+    const int col = x0 + ucol;
+    if (col < g.pn_x) {
+        float2* out = part + (int64_t)split * g.pn_y * (int64_t)g.pn_x;
 #pragma unroll
-    for (int mb = 0; mb < 4; ++mb)
-#pragma unroll
-        for (int nb = 0; nb < 2; ++nb) {
-            const int np = wave * 64 + nb * 32 + (lane & 31);
-            const int col = x0 + (np >> 1);
-            if (col >= g.pn_x) continue;
+        for (int mb = 0; mb < 4; ++mb)
 #pragma unroll
             for (int v = 0; v < 16; ++v) {
                 const int row = y0 + mb * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
-                if (row < g.pn_y) out[((int64_t)row * g.pn_x + col) * 2 + (np & 1)] = acc[mb][nb][v];
+                if (row < g.pn_y) out[(int64_t)row * g.pn_x + col] = make_float2(acc[mb][0][v], acc[mb][1][v]);
             }
-        }
+    }
 }
 
 // field += -j * amax * sum_split S   (FP64, fixed summation order)
@@ -316,25 +331,26 @@ int launch_fresnel_gemm(ohx_ctx* ctx, const Geometry& g, const PointSoA& soa, do
     const int var = (mode & OHX_MODE_MFMA_FP16X3) ? V_FP16X3 : (mode & OHX_MODE_MFMA_BF16X3) ? V_BF16X3 : V_BF16X1;
     const int tiles_x = (g.pn_x + BN - 1) / BN, tiles_y = (g.pn_y + BM - 1) / BM;
     const int tiles = tiles_x * tiles_y;
-    // split K until there are at least two workgroups per CU (one resides per CU)
-    int splits = (512 + tiles - 1) / tiles;
-    const int64_t max_by_k = (soa.n + 4 * KP - 1) / (4 * KP);   // >= 4 stages per split
-    if (splits > max_by_k) splits = (int)(max_by_k > 0 ? max_by_k : 1);
-    if (splits < 1) splits = 1;
+    // Split K so that (a) every CU gets several workgroups (two reside per CU)
+    // and (b) no FP32 accumulator sums more than MAX_SLICE points: FP32
+    // rounding of the running sums grows like sqrt(points per slice).
+    constexpr int64_t MAX_SLICE = 65536;
+    int64_t splits = std::max<int64_t>((1024 + tiles - 1) / tiles, (soa.n + MAX_SLICE - 1) / MAX_SLICE);
+    const int64_t max_by_k = std::max<int64_t>(1, (soa.n + 4 * KP - 1) / (4 * KP));  // >= 4 stages per split
+    splits = std::min(splits, max_by_k);
     int64_t kslice = (soa.n + splits - 1) / splits;
     kslice = (kslice + KP - 1) / KP * KP;
-    splits = (int)((soa.n + kslice - 1) / kslice);
-    if (splits < 1) splits = 1;
+    splits = std::max<int64_t>(1, (soa.n + kslice - 1) / kslice);
     const int64_t npx = (int64_t)g.pn_x * g.pn_y;
     const size_t part_bytes = (size_t)splits * npx * 2 * sizeof(float);
     int st = ensure_gemm_scratch(ctx, part_bytes + 256);
     if (st) return st;
     float* amax = (float*)ctx->gemm_scratch;
-    float* part = (float*)((char*)ctx->gemm_scratch + 256);
+    float2* part = (float2*)((char*)ctx->gemm_scratch + 256);
     hipLaunchKernelGGL(k_amp_init, dim3(1), dim3(64), 0, ctx->stream, amax);
     hipLaunchKernelGGL(k_amp_max, dim3(256), dim3(256), 0, ctx->stream, (const double*)soa.kamp, soa.n, amax);
     const size_t shmem = 0;
-    const dim3 grid((unsigned)(tiles * splits)), block(NT);
+    const dim3 grid((unsigned)((int64_t)tiles * splits)), block(NT);
     switch (var) {
     case V_BF16X1:
         hipLaunchKernelGGL(k_fresnel_gemm<V_BF16X1>, grid, block, shmem, ctx->stream, g, soa, part, tiles_x, tiles,
@@ -353,7 +369,7 @@ int launch_fresnel_gemm(ohx_ctx* ctx, const Geometry& g, const PointSoA& soa, do
     if (st) return st;
     int64_t rg = (npx + 255) / 256;
     if (rg > 65536) rg = 65536;
-    hipLaunchKernelGGL(k_gemm_reduce, dim3((unsigned)rg), dim3(256), 0, ctx->stream, (const float2*)part, splits,
+    hipLaunchKernelGGL(k_gemm_reduce, dim3((unsigned)rg), dim3(256), 0, ctx->stream, (const float2*)part, (int)splits,
                        npx, (double2*)d_field, (const float*)amax, var == V_FP16X3 ? 1 : 0);
     ctx->last.workgroups = (int64_t)tiles * splits;
     ctx->last.kernel_kind = 2;

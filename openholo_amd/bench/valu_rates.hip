@@ -1,0 +1,88 @@
+// Issue-rate microbenchmark for the VALU roofline of the direct kernels:
+// FP32 FMA, v_sin, v_fract, v_rsq and mixes, plus a 1 GiB copy kernel whose
+// FETCH_SIZE / WRITE_SIZE readings calibrate the HBM traffic counters.
+// Build: make -C openholo_amd bench   Run: openholo_amd/bin/valu_rates
+#include <hip/hip_runtime.h>
+#include <cstdio>
+#include <vector>
+#define CK(x) do { hipError_t e = (x); if (e != hipSuccess) { printf("err %s line %d\n", hipGetErrorString(e), __LINE__); return 1; } } while (0)
+constexpr int CH = 16;
+template <int KIND>
+__global__ __launch_bounds__(256) void k(float* out, int iters, float b, float c) {
+  float a[CH], r[CH];
+  for (int i = 0; i < CH; ++i) { a[i] = threadIdx.x * 1e-3f + i * 1e-2f; r[i] = 0.f; }
+  for (int it = 0; it < iters; ++it) {
+#pragma unroll
+    for (int i = 0; i < CH; ++i) {
+      if (KIND == 0) a[i] = fmaf(a[i], b, c);                                  // FMA
+      else if (KIND == 1) a[i] = __builtin_amdgcn_sinf(a[i]);                   // v_sin
+      else if (KIND == 2) a[i] = __builtin_amdgcn_fractf(a[i] + c);             // add+fract
+      else if (KIND == 3) { a[i] = __builtin_amdgcn_sinf(a[i]) * b + c; }       // sin + fma
+      else if (KIND == 4) { float s = __builtin_amdgcn_sinf(a[i]), t = __builtin_amdgcn_cosf(a[i]); a[i] = fmaf(s, b, t); } // sin+cos+fma
+      else if (KIND == 5) { a[i] = fmaf(a[i], b, c); a[i] = fmaf(a[i], b, c); a[i] = fmaf(a[i], b, c); a[i] = __builtin_amdgcn_sinf(a[i]); } // 3 fma + sin
+      else if (KIND == 6) a[i] = __builtin_amdgcn_rsqf(a[i] + c);               // add + rsq
+      else if (KIND == 7) {  // one Fresnel pair of the direct kernel: add, fract, sin, cos, 2 fma
+        const float f = __builtin_amdgcn_fractf(a[i] + c);
+        r[i] = fmaf(b, __builtin_amdgcn_sinf(f), r[i]);
+        a[i] = fmaf(b, __builtin_amdgcn_cosf(f), a[i]);
+      } else if (KIND == 8) {  // one RS pair: add, 2 fma, fract, sin, cos, fma, 2 fma
+        const float N = a[i] + c;
+        const float f = __builtin_amdgcn_fractf(fmaf(fmaf(-1e-3f, N, b), N, c));
+        const float amp = fmaf(-1e-4f, N, b);
+        r[i] = fmaf(amp, __builtin_amdgcn_sinf(f), r[i]);
+        a[i] = fmaf(amp, __builtin_amdgcn_cosf(f), a[i]);
+      }
+    }
+  }
+  float s = 0; for (int i = 0; i < CH; ++i) s += a[i] + r[i];
+  out[blockIdx.x * 256 + threadIdx.x] = s;
+}
+__global__ void kcopy(const float4* __restrict__ a, float4* __restrict__ b, size_t n) {
+  size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x, st = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += st) b[i] = a[i];
+}
+int main() {
+  const int blocks = 256 * 8, iters = 4096;
+  float* out; CK(hipMalloc(&out, blocks * 256 * sizeof(float)));
+  hipEvent_t e0, e1; CK(hipEventCreate(&e0)); CK(hipEventCreate(&e1));
+  const char* names[] = {"fma", "sin", "add+fract", "sin+fma", "sin+cos+fma", "3fma+sin", "add+rsq",
+                         "fresnel-pair", "rs-pair"};
+  const int instr[] = {1, 1, 2, 2, 3, 4, 2, 6, 9};
+  for (int kind = 0; kind < 9; ++kind) {
+    for (int rep = 0; rep < 2; ++rep) {
+      CK(hipEventRecord(e0));
+      switch (kind) {
+        case 0: hipLaunchKernelGGL(k<0>, dim3(blocks), dim3(256), 0, 0, out, iters, 0.999f, 1e-4f); break;
+        case 1: hipLaunchKernelGGL(k<1>, dim3(blocks), dim3(256), 0, 0, out, iters, 0.999f, 1e-4f); break;
+        case 2: hipLaunchKernelGGL(k<2>, dim3(blocks), dim3(256), 0, 0, out, iters, 0.999f, 1e-4f); break;
+        case 3: hipLaunchKernelGGL(k<3>, dim3(blocks), dim3(256), 0, 0, out, iters, 0.999f, 1e-4f); break;
+        case 4: hipLaunchKernelGGL(k<4>, dim3(blocks), dim3(256), 0, 0, out, iters, 0.999f, 1e-4f); break;
+        case 5: hipLaunchKernelGGL(k<5>, dim3(blocks), dim3(256), 0, 0, out, iters, 0.999f, 1e-4f); break;
+        case 6: hipLaunchKernelGGL(k<6>, dim3(blocks), dim3(256), 0, 0, out, iters, 0.999f, 1e-4f); break;
+        case 7: hipLaunchKernelGGL(k<7>, dim3(blocks), dim3(256), 0, 0, out, iters, 0.999f, 1e-4f); break;
+        case 8: hipLaunchKernelGGL(k<8>, dim3(blocks), dim3(256), 0, 0, out, iters, 0.999f, 1e-4f); break;
+      }
+      CK(hipGetLastError());
+      CK(hipEventRecord(e1)); CK(hipEventSynchronize(e1));
+      float ms; CK(hipEventElapsedTime(&ms, e0, e1));
+      double lanes_ops = (double)blocks * 256 * iters * CH;   // per "element op group"
+      if (rep) printf("%-12s %8.3f ms  %8.2f T elem-iter/s  %8.2f T VALU-instr/s (lane)\n", names[kind], ms,
+                      lanes_ops / ms / 1e9, lanes_ops * instr[kind] / ms / 1e9);
+    }
+  }
+  // copy kernel of known size for counter calibration: 1 GiB read + 1 GiB write
+  size_t n = (1ull << 30) / 16;
+  float4 *a, *b; CK(hipMalloc(&a, n * 16)); CK(hipMalloc(&b, n * 16));
+  CK(hipMemset(a, 0, n * 16));
+  for (int rep = 0; rep < 3; ++rep) {
+    CK(hipEventRecord(e0));
+    hipLaunchKernelGGL(kcopy, dim3(256 * 64), dim3(256), 0, 0, (const float4*)a, b, n);
+    CK(hipGetLastError());
+    CK(hipEventRecord(e1)); CK(hipEventSynchronize(e1));
+    float ms; CK(hipEventElapsedTime(&ms, e0, e1));
+    printf("copy 1 GiB -> 1 GiB: %.3f ms  %.1f GB/s (read+write)\n", ms, 2.0 * n * 16 / ms / 1e6);
+  }
+  hipDeviceProp_t p; CK(hipGetDeviceProperties(&p, 0));
+  printf("device %s CUs %d clock %d kHz\n", p.gcnArchName, p.multiProcessorCount, p.clockRate);
+  return 0;
+}

@@ -1,0 +1,154 @@
+// Command-line point-cloud generator with the reference CLI's flags
+// (Colab/OpenholoGeneration.cpp:119-195, point-cloud algorithm only):
+//   -a 0  -c config.xml  -i points.ply  -e encode  -m 0|1  -p 0|1  -f 0|1
+// -m 0: MODE_FLOAT|MODE_FASTMATH, -m 1: MODE_DOUBLE; -p 1: MODE_GPU (both -p
+// values run on the GPU here); -f 0: RS, 1: Fresnel.  Writes
+// <out>/PointCloud_<R-S|Fresnel>_<Phase|...>.bmp like ex_PointCloud.
+// Extensions used by the test suite:
+//   -o DIR            output directory (default Result)
+//   --vtx FILE        raw 56-byte Vertex records instead of -i
+//   --mfma bf16x1|bf16x3|fp16x3   matrix-core Fresnel variant
+//   --dump-field F    raw complex128 fields of all channels (row-major)
+//   --dump-enc F      raw float64 encoded buffers
+//   --dump-u8 F       raw uint8 normalised buffers
+//   --ohc F           also save the complex field as OHC
+//   --no-generate     stop after loading (with --dump-points / --dump-config)
+//   --dump-points F   raw 56-byte Vertex records as loaded
+//   --dump-config     print the parsed configuration as one JSON line
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <sys/stat.h>
+#include <vector>
+
+#include "ophPointCloud.h"
+
+static const char* encode_name(unsigned e)
+{
+    switch (e) {
+    case 0: return "Phase";
+    case 1: return "Amplitude";
+    case 2: return "Real";
+    case 3: return "Imaginary";
+    default: return "Unknown";
+    }
+}
+
+int main(int argc, char** argv)
+{
+    std::string config, input, vtx, outdir = "Result", dump_field, dump_enc, dump_u8, ohc, dump_points;
+    bool no_generate = false, dump_config = false;
+    int alg = 0, flag = 0;
+    unsigned encode = 0, mode = 0;
+    for (int i = 1; i < argc; ++i) {
+        const std::string a = argv[i];
+        const char* v = i + 1 < argc ? argv[i + 1] : "";
+        if (a == "-a") alg = std::atoi(v), ++i;
+        else if (a == "-c") config = v, ++i;
+        else if (a == "-i") input = v, ++i;
+        else if (a == "-e") encode = (unsigned)std::atoi(v), ++i;
+        else if (a == "-m") {
+            if (std::atoi(v) == 0) mode |= MODE_FLOAT | MODE_FASTMATH;
+            else mode |= MODE_DOUBLE;
+            ++i;
+        } else if (a == "-p") {
+            if (std::atoi(v) == 1) mode |= MODE_GPU;
+            ++i;
+        } else if (a == "-f") flag = std::atoi(v), ++i;
+        else if (a == "-o") outdir = v, ++i;
+        else if (a == "--vtx") vtx = v, ++i;
+        else if (a == "--mfma") {
+            const std::string m = v;
+            mode |= m == "bf16x1" ? MODE_MFMA_BF16X1 : m == "bf16x3" ? MODE_MFMA_BF16X3 : MODE_MFMA_FP16X3;
+            ++i;
+        } else if (a == "--dump-field") dump_field = v, ++i;
+        else if (a == "--dump-enc") dump_enc = v, ++i;
+        else if (a == "--dump-u8") dump_u8 = v, ++i;
+        else if (a == "--ohc") ohc = v, ++i;
+        else if (a == "--dump-points") dump_points = v, ++i;
+        else if (a == "--no-generate") no_generate = true;
+        else if (a == "--dump-config") dump_config = true;
+        else {
+            std::fprintf(stderr, "unknown argument %s\n", a.c_str());
+            return 2;
+        }
+    }
+    if (alg != 0) {
+        std::fprintf(stderr, "only the point-cloud algorithm (-a 0) is provided\n");
+        return 2;
+    }
+    ophPointCloud* h = new ophPointCloud();
+    h->SetMode(mode);
+    if (!h->readConfig(config.c_str())) return 1;
+    if (!vtx.empty()) {
+        FILE* fp = std::fopen(vtx.c_str(), "rb");
+        if (!fp) return 1;
+        std::fseek(fp, 0, SEEK_END);
+        const long bytes = std::ftell(fp);
+        std::fseek(fp, 0, SEEK_SET);
+        std::vector<Vertex> v(bytes / sizeof(Vertex));
+        if (std::fread(v.data(), sizeof(Vertex), v.size(), fp) != v.size()) return 1;
+        std::fclose(fp);
+        h->setNumberOfPoints(v.size());
+        h->setPointCloudModel(v.data());
+    } else if (h->loadPointCloud(input.c_str()) < 0) {
+        return 1;
+    }
+    if (dump_config) {
+        OphConfig& c = h->getContext();
+        vec3 sc;
+        h->getScale(sc);
+        std::printf("{\"pn\": [%d, %d], \"pp\": [%.17g, %.17g], \"lambdas\": [", c.pixel_number[_X],
+                    c.pixel_number[_Y], c.pixel_pitch[_X], c.pixel_pitch[_Y]);
+        for (uint w = 0; w < c.waveNum; ++w) std::printf("%s%.17g", w ? ", " : "", c.wave_length[w]);
+        std::printf("], \"scale\": [%.17g, %.17g, %.17g], \"distance\": %.17g, \"rotate\": %d, \"merge\": %d, "
+                    "\"flip\": %d, \"field_length\": %.17g, \"points\": %llu}\n",
+                    sc[0], sc[1], sc[2], h->getDistance(), (int)h->getImageConfig().rotate,
+                    (int)h->getImageConfig().merge, h->getImageConfig().flip, h->getFieldLength(),
+                    (unsigned long long)h->getNumberOfPoints());
+    }
+    if (!dump_points.empty()) {
+        FILE* fp = std::fopen(dump_points.c_str(), "wb");
+        if (!fp) return 1;
+        std::fwrite(h->getPointCloudModel(), sizeof(Vertex), h->getNumberOfPoints(), fp);
+        std::fclose(fp);
+    }
+    if (no_generate) {
+        h->release();
+        return 0;
+    }
+    const Real t = h->generateHologram((uint)flag);
+    if (t <= 0.0) {
+        std::fprintf(stderr, "generateHologram failed: %s\n", h->lastError().c_str());
+        return 1;
+    }
+    std::printf("generateHologram: %.6f s\n", t);
+    const uint nch = h->getContext().waveNum;
+    const long long npx = (long long)h->getContext().pixel_number[_X] * h->getContext().pixel_number[_Y];
+    if (!ohc.empty() && !h->saveAsOhc(ohc.c_str())) return 1;
+    if (!dump_field.empty()) {
+        FILE* fp = std::fopen(dump_field.c_str(), "wb");
+        for (uint c = 0; c < nch; ++c) std::fwrite(h->getComplexField()[c], 16, npx, fp);
+        std::fclose(fp);
+    }
+    h->encoding(encode);
+    h->normalize();
+    if (!dump_enc.empty()) {
+        FILE* fp = std::fopen(dump_enc.c_str(), "wb");
+        for (uint c = 0; c < nch; ++c) std::fwrite(h->getEncodedBuffer()[c], 8, npx, fp);
+        std::fclose(fp);
+    }
+    if (!dump_u8.empty()) {
+        FILE* fp = std::fopen(dump_u8.c_str(), "wb");
+        for (uint c = 0; c < nch; ++c) std::fwrite(h->getNormalizedBuffer()[c], 1, npx, fp);
+        std::fclose(fp);
+    }
+    ::mkdir(outdir.c_str(), 0755);
+    char path[1024];
+    std::snprintf(path, sizeof path, "%s/PointCloud_%s_%s.bmp", outdir.c_str(), flag == 0 ? "R-S" : "Fresnel",
+                  encode_name(encode));
+    const bool ok = h->save(path, (uint8_t)(nch * 8));
+    h->release();
+    return ok ? 0 : 1;
+}

@@ -1,0 +1,107 @@
+"""The C++ drop-in end to end on the GPU: the reference CLI's flags and the
+reference example's call sequence, on the reference's own input files
+(tests/golden), checked against the CPU oracle.
+
+Tolerances: field rel-L2 vs the FP64 oracle 1e-5 (FP32 direct, -m 0) and
+1e-9 (FP64 direct, -m 1) on sampled pixels; encoded buffers vs the oracle's
+encoder applied to the same field 1e-15; 8-bit output vs the oracle's
+normaliser applied to the same encoded buffers: exact.
+"""
+import os
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "openholo_amd", "bin")
+GOLD = os.path.join(ROOT, "tests", "golden")
+VERTEX = np.dtype([("pos", "<f8", (3,)), ("color", "<f8", (3,)), ("phase", "<f8")])
+CFG_RGB = os.path.join(GOLD, "Generation_PointCloud_RGB.xml")
+PLY = os.path.join(GOLD, "pointcloud_1470.ply")
+
+
+def run(*args):
+    r = subprocess.run(list(args), capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, f"{args[0]} failed:\n{r.stdout[-2000:]}\n{r.stderr[-2000:]}"
+    return r
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+@pytest.mark.parametrize("diff", [0, 1])
+@pytest.mark.parametrize("prec", [0, 1])
+def test_cli_rgb_end_to_end(tmp_path, orc, diff, prec):
+    d = tmp_path
+    run(os.path.join(BIN, "ophpc_cli"), "-a", "0", "-c", CFG_RGB, "-i", PLY, "-e", "0", "-m", str(prec),
+        "-p", "1", "-f", str(diff), "-o", str(d / "Result"), "--dump-points", str(d / "p.bin"),
+        "--dump-field", str(d / "f.bin"), "--dump-enc", str(d / "e.bin"), "--dump-u8", str(d / "u.bin"))
+    pn, pp, lams = (1920, 1080), (10e-6, 10e-6), [638e-9, 520e-9, 450e-9]
+    npx = pn[0] * pn[1]
+    v = np.fromfile(d / "p.bin", VERTEX)
+    f = np.fromfile(d / "f.bin", np.complex128).reshape(3, npx)
+    e = np.fromfile(d / "e.bin", np.float64).reshape(3, npx)
+    u = np.fromfile(d / "u.bin", np.uint8).reshape(3, npx)
+    g = orc.make_geom(pn, pp, scale=(0.01, 0.01, 0.01), distance=0.5)
+    pix = np.random.default_rng(8).choice(npx, 1500, replace=False)
+    tol = 1e-5 if prec == 0 else 1e-9
+    for ch, lam in enumerate(lams):
+        want = orc.field_exact(g, v, lam, ch, diff, pix)
+        assert rel_l2(f[ch][pix], want) < tol, ch
+        assert np.allclose(e[ch], orc.encode(f[ch], orc.ENCODE_PHASE), rtol=0, atol=4e-16)
+    want_u = orc.normalize(list(e))
+    for ch in range(3):
+        assert np.array_equal(u[ch], want_u[ch])
+    # merged 24-bit BMP, rotated 180 degrees (IMG_Merge=1, IMG_Rotation=1)
+    bmp = open(d / "Result" / f"PointCloud_{'R-S' if diff == 0 else 'Fresnel'}_Phase.bmp", "rb").read()
+    assert bmp[:2] == b"BM"
+    size, _, off = struct.unpack("<IIi", bmp[2:14])
+    w, h, planes, bpp = struct.unpack("<iiHH", bmp[18:30])
+    assert (size, w, h, planes, bpp) == (len(bmp), 1920, 1080, 1, 24)
+    pixels = np.frombuffer(bmp[off:], np.uint8).reshape(1080, 1920, 3)
+    bgr = np.stack([u[2], u[1], u[0]], axis=-1).reshape(1080, 1920, 3)
+    assert np.array_equal(pixels, bgr[::-1, ::-1])
+
+
+def test_example_program_ohc_and_bmp(tmp_path):
+    d = tmp_path
+    cfg = os.path.join(GOLD, "Dice_PointCloud.xml")
+    run(os.path.join(BIN, "examPointCloud"), cfg, PLY, str(d / "out"), "rs", "1")
+    run(os.path.join(BIN, "ophpc_cli"), "-c", cfg, "-i", PLY, "-p", "1", "-f", "0", "-o", str(d / "R"),
+        "--dump-field", str(d / "f.bin"), "--dump-u8", str(d / "u.bin"))
+    npx = 1920 * 1080
+    field = np.fromfile(d / "f.bin", np.complex128)
+    ohc = open(d / "out.ohc", "rb").read()
+    assert ohc[:2] == b"OH"
+    fsize, vmaj, vmin, _, _, data_off = struct.unpack("<QBBIII", ohc[2:24])
+    hsize, nx, ny, ppx, ppy, punit, nw = struct.unpack("<IIIddBI", ohc[24:57])
+    ctype, carr, wunit, dtype, store, code, pcode = struct.unpack("<7B", ohc[57:64])
+    assert (fsize, vmaj, vmin) == (len(ohc), 3, 0)
+    assert (nx, ny, ppx, ppy, punit, nw) == (1920, 1080, 4.3e-6, 4.3e-6, 1, 1)
+    assert (ctype, carr, wunit, dtype, store, code, pcode) == (2, 2, 1, 10, 1, 2, 0)
+    assert hsize == 65 + 8 and data_off == 24 + hsize
+    assert struct.unpack("<d", ohc[89:97])[0] == 532e-9
+    planes = np.frombuffer(ohc[data_off:], np.float64).reshape(2, npx)
+    assert np.array_equal(planes[0], field.real) and np.array_equal(planes[1], field.imag)
+    # 8-bit palette BMP of the single channel
+    bmp = open(d / "out.bmp", "rb").read()
+    off = struct.unpack("<i", bmp[10:14])[0]
+    assert struct.unpack("<H", bmp[28:30])[0] == 8 and off == 14 + 40 + 1024
+    assert np.array_equal(np.frombuffer(bmp[off:], np.uint8), np.fromfile(d / "u.bin", np.uint8))
+
+
+def test_cli_mfma_fresnel(tmp_path, orc):
+    d = tmp_path
+    run(os.path.join(BIN, "ophpc_cli"), "-c", CFG_RGB, "-i", PLY, "-m", "0", "-p", "1", "-f", "1",
+        "--mfma", "bf16x3", "-o", str(d / "R"), "--dump-points", str(d / "p.bin"), "--dump-field", str(d / "f.bin"))
+    v = np.fromfile(d / "p.bin", VERTEX)
+    f = np.fromfile(d / "f.bin", np.complex128).reshape(3, -1)
+    g = orc.make_geom((1920, 1080), (10e-6, 10e-6), scale=(0.01, 0.01, 0.01), distance=0.5)
+    pix = np.random.default_rng(9).choice(1920 * 1080, 1500, replace=False)
+    for ch, lam in enumerate([638e-9, 520e-9, 450e-9]):
+        assert rel_l2(f[ch][pix], orc.field_exact(g, v, lam, ch, 1, pix)) < 1e-4

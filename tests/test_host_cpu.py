@@ -1,0 +1,150 @@
+"""CPU checks of the C++ drop-in's host logic (no GPU needed): XML config
+tags (ophgen/src/ophGen.cpp:244-327, ophPointCloud.cpp:109-138) and PLY
+loading semantics (Openholo/src/PLYparser.cpp:142-449), through the
+ophpc_cli driver built from openholo_amd/host."""
+import json
+import os
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CLI = os.path.join(ROOT, "openholo_amd", "bin", "ophpc_cli")
+GOLD = os.path.join(ROOT, "tests", "golden")
+VERTEX = np.dtype([("pos", "<f8", (3,)), ("color", "<f8", (3,)), ("phase", "<f8")])
+
+
+def run_cli(*args, check=True):
+    r = subprocess.run([CLI, *args], capture_output=True, text=True, timeout=120)
+    if check and r.returncode != 0:
+        raise AssertionError(f"ophpc_cli failed: {r.stdout}\n{r.stderr}")
+    return r
+
+
+def parse_ascii_ply_reference_rules(path):
+    """x, y, z as decimal doubles; colour = float32(int)/255 (single precision);
+    phase rounded to float32; 0.5 colours when there is no 'color' element."""
+    lines = open(path).read().splitlines()
+    end = lines.index("end_header")
+    header = lines[:end]
+    has_color = any(l.startswith("element color") for l in header)
+    elems = []
+    for l in header:
+        t = l.split()
+        if t[0] == "element":
+            elems.append([t[1], int(t[2]), []])
+        elif t[0] == "property":
+            elems[-1][2].append(t[-1])
+    rows = lines[end + 1:]
+    out = []
+    pos = 0
+    for name, count, props in elems:
+        for k in range(count):
+            vals = rows[pos].split()
+            pos += 1
+            if name != "vertex":
+                continue
+            d = dict(zip(props, vals))
+            col = [np.float32(np.float32(int(d[c])) / np.float32(255.0)) for c in ("red", "green", "blue")]
+            out.append(((float(d["x"]), float(d["y"]), float(d["z"])),
+                        tuple(float(c) for c in col) if has_color else (0.5, 0.5, 0.5),
+                        float(np.float32(float(d["phase"]))) if "phase" in d else 0.0))
+    v = np.zeros(len(out), VERTEX)
+    for i, (p, c, ph) in enumerate(out):
+        v[i] = (p, c, ph)
+    return v
+
+
+def test_config_tags(tmp_path):
+    r = run_cli("-c", os.path.join(GOLD, "Generation_PointCloud_RGB.xml"), "-i",
+                os.path.join(GOLD, "pointcloud_1470.ply"), "--no-generate", "--dump-config")
+    cfg = json.loads(r.stdout.strip().splitlines()[-1])
+    assert cfg["pn"] == [1920, 1080]
+    assert cfg["pp"] == [10e-6, 10e-6]
+    assert cfg["lambdas"] == [638e-9, 520e-9, 450e-9]
+    assert cfg["scale"] == [0.01, 0.01, 0.01]
+    assert cfg["distance"] == 0.5
+    assert (cfg["rotate"], cfg["merge"], cfg["flip"]) == (1, 1, 0)
+    assert cfg["field_length"] == 1.0
+    assert cfg["points"] == 1470
+    r = run_cli("-c", os.path.join(GOLD, "Dice_PointCloud.xml"), "-i", os.path.join(GOLD, "pointcloud_1470.ply"),
+                "--no-generate", "--dump-config")
+    cfg = json.loads(r.stdout.strip().splitlines()[-1])
+    assert cfg["pn"] == [1920, 1080] and cfg["pp"] == [4.3e-6, 4.3e-6] and cfg["lambdas"] == [532e-9]
+    assert cfg["distance"] == 0.1 and cfg["rotate"] == 0
+
+
+def test_config_missing_tag_fails(tmp_path):
+    src = open(os.path.join(GOLD, "Dice_PointCloud.xml")).read()
+    bad = tmp_path / "bad.xml"
+    bad.write_text(src.replace("<SLM_PixelNumX>1920</SLM_PixelNumX>", ""))
+    r = run_cli("-c", str(bad), "-i", os.path.join(GOLD, "pointcloud_1470.ply"), "--no-generate", check=False)
+    assert r.returncode == 1
+    bad.write_text(src.replace("<Distance>0.1</Distance>", ""))
+    assert run_cli("-c", str(bad), "-i", os.path.join(GOLD, "pointcloud_1470.ply"), "--no-generate",
+                   check=False).returncode == 1
+    txt = tmp_path / "cfg.txt"
+    txt.write_text(src)
+    assert run_cli("-c", str(txt), "-i", os.path.join(GOLD, "pointcloud_1470.ply"), "--no-generate",
+                   check=False).returncode == 1
+
+
+def test_ascii_ply_matches_reference_rules(tmp_path):
+    out = tmp_path / "p.bin"
+    run_cli("-c", os.path.join(GOLD, "Dice_PointCloud.xml"), "-i", os.path.join(GOLD, "pointcloud_1470.ply"),
+            "--no-generate", "--dump-points", str(out))
+    got = np.fromfile(out, VERTEX)
+    want = parse_ascii_ply_reference_rules(os.path.join(GOLD, "pointcloud_1470.ply"))
+    assert len(got) == 1470
+    assert np.array_equal(got["pos"], want["pos"])
+    assert np.array_equal(got["color"], want["color"])
+    assert np.array_equal(got["phase"], want["phase"])
+
+
+def _write_binary_ply(path, verts, big=False, with_color_elem=True, extra=True):
+    e = ">" if big else "<"
+    hdr = ["ply", f"format binary_{'big' if big else 'little'}_endian 1.0"]
+    if with_color_elem:
+        hdr += ["element color 1", "property int channel"]
+    hdr += [f"element vertex {len(verts)}", "property float x", "property float y", "property float z"]
+    if extra:
+        hdr += ["property float nx"]
+    hdr += ["property uchar red", "property uchar green", "property uchar blue", "property float phase",
+            "element face 1", "property list uchar int vertex_indices", "end_header"]
+    body = b""
+    if with_color_elem:
+        body += struct.pack(e + "i", 3)
+    for (x, y, z, r, g, b, ph) in verts:
+        body += struct.pack(e + "fff", x, y, z)
+        if extra:
+            body += struct.pack(e + "f", 9.0)
+        body += struct.pack(e + "BBBf", r, g, b, ph)
+    body += struct.pack(e + "B", 3) + struct.pack(e + "iii", 0, 1, 2)
+    with open(path, "wb") as f:
+        f.write(("\n".join(hdr) + "\n").encode() + body)
+
+
+@pytest.mark.parametrize("big", [False, True])
+@pytest.mark.parametrize("color_elem", [True, False])
+def test_binary_ply(tmp_path, big, color_elem):
+    verts = [(0.25, -1.5, 3.0, 10, 200, 255, 1.25), (-2.0, 0.125, 7.5, 0, 1, 2, -0.5)]
+    ply = tmp_path / "b.ply"
+    _write_binary_ply(ply, verts, big=big, with_color_elem=color_elem)
+    out = tmp_path / "p.bin"
+    run_cli("-c", os.path.join(GOLD, "Dice_PointCloud.xml"), "-i", str(ply), "--no-generate",
+            "--dump-points", str(out))
+    got = np.fromfile(out, VERTEX)
+    assert len(got) == 2
+    for g, (x, y, z, r, gg, b, ph) in zip(got, verts):
+        assert tuple(g["pos"]) == (x, y, z)
+        want = [float(np.float32(c) / np.float32(255)) for c in (r, gg, b)] if color_elem else [0.5] * 3
+        assert list(g["color"]) == want
+        assert g["phase"] == ph
+
+
+def test_missing_ply_fails(tmp_path):
+    r = run_cli("-c", os.path.join(GOLD, "Dice_PointCloud.xml"), "-i", str(tmp_path / "none.ply"),
+                "--no-generate", check=False)
+    assert r.returncode == 1

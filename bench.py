@@ -51,6 +51,9 @@ def parse():
     ap.add_argument("--kernel", default="direct", choices=list(KERNELS))
     ap.add_argument("--variants", type=int, default=1,
                     help="also time the 1M-point Fresnel workload on every Fresnel kernel (0 = off)")
+    ap.add_argument("--serving", type=int, default=1,
+                    help="also run the mixed Poisson request stream on the serving engine (0 = off)")
+    ap.add_argument("--requests", type=int, default=120)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--probe-pixels", type=int, default=512)
@@ -133,6 +136,75 @@ def fresnel_variants(ctx, capi, synth, Event, launches=2):
     dv.free()
     fld.free()
     return out
+
+
+def serving_stream(capi, synth, n_requests: int):
+    """Mixed request stream on the continuous-batching server.  Pass 1 submits
+    every request at once to measure capacity (jobs/s); pass 2 replays the
+    same requests with Poisson arrivals at 70% of that rate and reports
+    per-request latency (completion - arrival) and throughput."""
+    import threading
+    reqs = synth.serving_requests(n_requests)
+    clouds = [synth.random_cloud(r["seed"], r["n"], 4e-3, (0.49, 0.51)) for r in reqs]
+    fresnel_mode = KERNELS["bf16x3"]
+    rs_mode = KERNELS["direct"]
+
+    def submit(srv, i):
+        r = reqs[i]
+        p = capi.params(r["pn"], r["pp"], r["lambda"])
+        return srv.submit(clouds[i], p, r["diff"], fresnel_mode if r["diff"] == 1 else rs_mode,
+                          want_field=False, want_u8=True)
+
+    pairs = sum(r["n"] * r["pn"][0] * r["pn"][1] for r in reqs)
+    with capi.Server() as srv:
+        # warm-up: one request of each kind
+        for i in range(min(4, len(reqs))):
+            srv.wait(submit(srv, i))
+        t0 = time.perf_counter()
+        ids = [submit(srv, i) for i in range(len(reqs))]
+        for j in ids:
+            st, *_ = srv.wait(j)
+            assert st == capi.OK
+        burst_s = time.perf_counter() - t0
+        capacity = len(reqs) / burst_s
+        rate = 0.7 * capacity
+        gaps = -np.log(1.0 - synth.uniform01(20261202 + 44, len(reqs))) / rate
+        ids = [None] * len(reqs)
+
+        def producer():
+            t = time.perf_counter()
+            for i in range(len(reqs)):
+                t += gaps[i]
+                d = t - time.perf_counter()
+                if d > 0:
+                    time.sleep(d)
+                ids[i] = submit(srv, i)
+
+        th = threading.Thread(target=producer)
+        t1 = time.perf_counter()
+        th.start()
+        lat, first_submit, last_end = [], None, None
+        for i in range(len(reqs)):
+            while ids[i] is None:
+                time.sleep(0.0005)
+            st, stt, _, _ = srv.wait(ids[i])
+            assert st == capi.OK
+            lat.append(stt.end_ms - stt.submit_ms)
+            first_submit = stt.submit_ms if first_submit is None else min(first_submit, stt.submit_ms)
+            last_end = stt.end_ms if last_end is None else max(last_end, stt.end_ms)
+        th.join()
+        c = srv.counters()
+    span_s = (last_end - first_submit) / 1e3
+    lat = np.array(lat)
+    return {
+        "requests": len(reqs), "pairs": float(pairs),
+        "burst_capacity_jobs_per_s": round(capacity, 2), "poisson_rate_jobs_per_s": round(rate, 2),
+        "throughput_Gpairs_per_s": round(pairs / span_s / 1e9, 1), "burst_Gpairs_per_s": round(pairs / burst_s / 1e9, 1),
+        "p50_ms": round(float(np.percentile(lat, 50)), 2), "p99_ms": round(float(np.percentile(lat, 99)), 2),
+        "mean_ms": round(float(lat.mean()), 2), "iterations": int(c.iterations), "graph_replays": int(c.graph_replays),
+        "peak_reserved_GB": round(c.peak_reserved_bytes / 1e9, 2),
+        "kernels": "RS: FP32 direct; Fresnel: BF16 3-term matrix cores",
+    }
 
 
 def main():
@@ -255,6 +327,11 @@ def main():
         b.free()
     var = fresnel_variants(ctx, capi, synth, Event) if args.variants else None
     ctx.close()
+    serving = None
+    if args.serving:
+        serving = serving_stream(capi, synth, args.requests)
+        print(f"# serving: p50 {serving['p50_ms']} ms, p99 {serving['p99_ms']} ms, "
+              f"{serving['throughput_Gpairs_per_s']} Gpairs/s", file=sys.stderr, flush=True)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -279,6 +356,8 @@ def main():
     }
     if var:
         line["variants"] = var
+    if serving:
+        line["serving"] = serving
     print(json.dumps(line), flush=True)
 
 

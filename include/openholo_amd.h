@@ -150,6 +150,61 @@ typedef struct ohx_launch_stats {
 } ohx_launch_stats;
 int ohx_last_launch_stats(ohx_ctx* ctx, ohx_launch_stats* out);
 
+/* ---- serving: continuous batching of hologram jobs ------------------------
+ * No reference counterpart (the reference generates one hologram per
+ * blocking call).  A server owns a device stream and a scheduler thread.
+ * Jobs (one channel each) are admitted in arrival order while they fit the
+ * HBM budget and a free slot; every admitted job keeps its point constants
+ * and field resident.  Each scheduler iteration advances every active job
+ * by one slice of its points: one persistent multi-job kernel (replayed
+ * from a captured hipGraph) covers all direct-sum jobs, matrix-core Fresnel
+ * jobs run their GEMM slice on the same stream.  Slice boundaries depend only
+ * on the job itself, so a job's result is bit-identical alone or in a mix.
+ * Finished jobs are phase-encoded, normalised to 8 bits and copied out. */
+typedef struct ohx_server ohx_server;
+
+typedef struct ohx_server_opts {
+    int32_t device;
+    int32_t max_active;        /* jobs in flight (0: 32)                       */
+    int32_t use_graph;         /* 1: replay a captured hipGraph per iteration  */
+    int32_t reserved;
+    uint64_t mem_budget;       /* HBM bytes for jobs (0: free HBM - 4 GiB)     */
+    double slice_pairs;        /* point-pixel pairs per job per iteration (0: 1.5e10) */
+    double slice_pairs_mfma;   /* same for matrix-core jobs (0: 1.5e11)        */
+} ohx_server_opts;
+
+typedef struct ohx_job {
+    const void* vertices;      /* host, n_points 56-byte Vertex records (copied at submit) */
+    int64_t n_points;
+    ohx_pc_params params;      /* geometry, wavelength, channel               */
+    uint32_t diff_flag;
+    uint32_t mode;
+    double* out_field;         /* host, 2*pn_x*pn_y doubles, or NULL           */
+    uint8_t* out_u8;           /* host, pn_x*pn_y phase bytes, or NULL         */
+} ohx_job;
+
+typedef struct ohx_job_stats {
+    int32_t status;            /* OHX_OK, OHX_ERR_CANCELLED, device errors     */
+    int32_t iterations;        /* scheduler iterations the job took part in    */
+    double submit_ms, admit_ms, start_ms, end_ms;   /* server clock            */
+    uint64_t device_bytes;     /* HBM reserved for the job                     */
+} ohx_job_stats;
+
+int ohx_server_create(const ohx_server_opts* opts, ohx_server** out);
+/* OHX_ERR_ADMISSION when the job can never fit the memory budget */
+int ohx_server_submit(ohx_server* srv, const ohx_job* job, uint64_t* job_id);
+/* blocks until the job finished (outputs written) or timeout_s elapsed
+ * (timeout_s < 0: no limit); a finished job is forgotten after its wait */
+int ohx_server_wait(ohx_server* srv, uint64_t job_id, double timeout_s, ohx_job_stats* stats);
+int ohx_server_cancel(ohx_server* srv, uint64_t job_id);
+/* server-wide counters: jobs done, iterations, HBM reserved now / at peak */
+typedef struct ohx_server_counters {
+    uint64_t jobs_done, iterations, graph_replays;
+    uint64_t reserved_bytes, peak_reserved_bytes, budget_bytes;
+} ohx_server_counters;
+int ohx_server_counters_get(ohx_server* srv, ohx_server_counters* out);
+int ohx_server_destroy(ohx_server* srv);
+
 #ifdef __cplusplus
 }
 #endif

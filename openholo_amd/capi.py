@@ -28,7 +28,10 @@ EXPORTS = [
     "ohx_last_error", "ohx_mem_info", "ohx_malloc", "ohx_free", "ohx_memcpy_h2d", "ohx_memcpy_d2h",
     "ohx_memcpy_d2d", "ohx_synchronize", "ohx_field_zero", "ohx_pc_diffract", "ohx_field_accumulate",
     "ohx_pc_probe_f64", "ohx_encode", "ohx_normalize_u8", "ohx_last_launch_stats",
+    "ohx_server_create", "ohx_server_submit", "ohx_server_wait", "ohx_server_cancel",
+    "ohx_server_counters_get", "ohx_server_destroy",
 ]
+ERR_ADMISSION, ERR_NOT_FOUND, ERR_TIMEOUT, ERR_CANCELLED = -4, -5, -6, -8
 
 
 class PcParams(ctypes.Structure):
@@ -45,6 +48,30 @@ class PcParams(ctypes.Structure):
 class LaunchStats(ctypes.Structure):
     _fields_ = [("pairs_nominal", ctypes.c_int64), ("workgroups", ctypes.c_int64),
                 ("kernel_kind", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+class ServerOpts(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int32), ("max_active", ctypes.c_int32), ("use_graph", ctypes.c_int32),
+                ("reserved", ctypes.c_int32), ("mem_budget", ctypes.c_uint64), ("slice_pairs", ctypes.c_double),
+                ("slice_pairs_mfma", ctypes.c_double)]
+
+
+class Job(ctypes.Structure):
+    _fields_ = [("vertices", ctypes.c_void_p), ("n_points", ctypes.c_int64), ("params", PcParams),
+                ("diff_flag", ctypes.c_uint32), ("mode", ctypes.c_uint32), ("out_field", ctypes.c_void_p),
+                ("out_u8", ctypes.c_void_p)]
+
+
+class JobStats(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int32), ("iterations", ctypes.c_int32), ("submit_ms", ctypes.c_double),
+                ("admit_ms", ctypes.c_double), ("start_ms", ctypes.c_double), ("end_ms", ctypes.c_double),
+                ("device_bytes", ctypes.c_uint64)]
+
+
+class ServerCounters(ctypes.Structure):
+    _fields_ = [("jobs_done", ctypes.c_uint64), ("iterations", ctypes.c_uint64), ("graph_replays", ctypes.c_uint64),
+                ("reserved_bytes", ctypes.c_uint64), ("peak_reserved_bytes", ctypes.c_uint64),
+                ("budget_bytes", ctypes.c_uint64)]
 
 
 def params(pn, pp, lam, channel=0, scale=(1.0, 1.0, 1.0), distance=0.0, offset=(0, 0),
@@ -101,6 +128,12 @@ def lib():
             "ohx_encode": ([vp, vp, i64, vp, u32], i32),
             "ohx_normalize_u8": ([vp, ctypes.POINTER(vp), i32, i64, ctypes.POINTER(vp)], i32),
             "ohx_last_launch_stats": ([vp, ctypes.POINTER(LaunchStats)], i32),
+            "ohx_server_create": ([ctypes.POINTER(ServerOpts), ctypes.POINTER(vp)], i32),
+            "ohx_server_submit": ([vp, ctypes.POINTER(Job), ctypes.POINTER(ctypes.c_uint64)], i32),
+            "ohx_server_wait": ([vp, ctypes.c_uint64, ctypes.c_double, ctypes.POINTER(JobStats)], i32),
+            "ohx_server_cancel": ([vp, ctypes.c_uint64], i32),
+            "ohx_server_counters_get": ([vp, ctypes.POINTER(ServerCounters)], i32),
+            "ohx_server_destroy": ([vp], i32),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -217,6 +250,63 @@ class Context:
         f, t = ctypes.c_size_t(), ctypes.c_size_t()
         self._check(lib().ohx_mem_info(self.h, ctypes.byref(f), ctypes.byref(t)), "ohx_mem_info")
         return f.value, t.value
+
+
+class Server:
+    """Continuous-batching hologram server (one channel per job)."""
+
+    def __init__(self, device=0, max_active=32, use_graph=True, mem_budget=0, slice_pairs=0.0,
+                 slice_pairs_mfma=0.0):
+        o = ServerOpts(int(device), int(max_active), int(bool(use_graph)), 0, int(mem_budget),
+                       float(slice_pairs), float(slice_pairs_mfma))
+        h = ctypes.c_void_p()
+        st = lib().ohx_server_create(ctypes.byref(o), ctypes.byref(h))
+        if st != OK:
+            raise OhxError(st, "ohx_server_create", lib().ohx_status_string(st).decode())
+        self.h = h.value
+        self._keep = {}
+
+    def submit(self, vertices: np.ndarray, p: PcParams, diff: int, mode: int, want_field=True, want_u8=False) -> int:
+        v = np.ascontiguousarray(vertices)
+        npx = p.pn_x * p.pn_y
+        f = np.empty(npx, np.complex128) if want_field else None
+        u = np.empty(npx, np.uint8) if want_u8 else None
+        j = Job(v.ctypes.data if len(v) else None, len(v), p, int(diff), int(mode),
+                f.ctypes.data if f is not None else None, u.ctypes.data if u is not None else None)
+        jid = ctypes.c_uint64()
+        st = lib().ohx_server_submit(self.h, ctypes.byref(j), ctypes.byref(jid))
+        if st != OK:
+            raise OhxError(st, "ohx_server_submit", lib().ohx_status_string(st).decode())
+        self._keep[jid.value] = (f, u)
+        return jid.value
+
+    def wait(self, jid: int, timeout_s: float = -1.0):
+        """-> (status, JobStats, field or None, u8 or None)"""
+        stt = JobStats()
+        st = lib().ohx_server_wait(self.h, ctypes.c_uint64(jid), float(timeout_s), ctypes.byref(stt))
+        if st == ERR_TIMEOUT or st == ERR_NOT_FOUND:
+            return st, stt, None, None
+        f, u = self._keep.pop(jid, (None, None))
+        return st, stt, f, u
+
+    def cancel(self, jid: int) -> int:
+        return lib().ohx_server_cancel(self.h, ctypes.c_uint64(jid))
+
+    def counters(self) -> ServerCounters:
+        c = ServerCounters()
+        lib().ohx_server_counters_get(self.h, ctypes.byref(c))
+        return c
+
+    def close(self):
+        if self.h:
+            lib().ohx_server_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
 
 
 def generate(ctx: Context, vertices: np.ndarray, pn, pp, lambdas, diff: int, mode: int,

@@ -66,24 +66,11 @@ int geometry_from_params(ohx_ctx* ctx, const ohx_pc_params* p, uint32_t diff, Ge
     return OHX_OK;
 }
 
-int ensure_points(ohx_ctx* ctx, int64_t n, PointSoA* soa)
+size_t points_bytes(int64_t n) { return (size_t)n * (6 * sizeof(double) + sizeof(int4)) + 8 * 256; }
+
+void carve_points(void* base, int64_t n, PointSoA* soa)
 {
-    const size_t per = 6 * sizeof(double) + sizeof(int4);
-    const size_t need = (size_t)n * per + 8 * 256;
-    if (need > ctx->pts_bytes) {
-        if (ctx->pts) {
-            int st = check_hip(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
-            if (st) return st;
-            (void)hipFree(ctx->pts);
-            ctx->pts = nullptr;
-            ctx->pts_bytes = 0;
-        }
-        size_t cap = need + need / 4;
-        int st = check_hip(ctx, hipMalloc(&ctx->pts, cap), "hipMalloc(points)");
-        if (st) return st;
-        ctx->pts_bytes = cap;
-    }
-    char* b = (char*)ctx->pts;
+    char* b = (char*)base;
     auto take = [&](size_t bytes) {
         char* r = b;
         b += (bytes + 255) & ~size_t(255);
@@ -97,6 +84,25 @@ int ensure_points(ohx_ctx* ctx, int64_t n, PointSoA* soa)
     soa->q = (double*)take(n * sizeof(double));
     soa->box = (int4*)take(n * sizeof(int4));
     soa->n = n;
+}
+
+int ensure_points(ohx_ctx* ctx, int64_t n, PointSoA* soa)
+{
+    const size_t need = points_bytes(n);
+    if (need > ctx->pts_bytes) {
+        if (ctx->pts) {
+            int st = check_hip(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+            if (st) return st;
+            (void)hipFree(ctx->pts);
+            ctx->pts = nullptr;
+            ctx->pts_bytes = 0;
+        }
+        size_t cap = need + need / 4;
+        int st = check_hip(ctx, hipMalloc(&ctx->pts, cap), "hipMalloc(points)");
+        if (st) return st;
+        ctx->pts_bytes = cap;
+    }
+    carve_points(ctx->pts, n, soa);
     return OHX_OK;
 }
 

@@ -53,16 +53,34 @@ struct Geometry {
     int32_t diff;
 };
 
+// One job of the persistent multi-job direct kernel (serving engine).
+struct DirectJob {
+    Geometry g;
+    PointSoA s;
+    double2* field;
+    int64_t p0, p1;       // point slice of this iteration
+    int tiles_x, tiles;   // 64 x 32 tiles of the hologram
+    int item_base;        // prefix sum of tiles over the table
+    int fp64;
+};
+
 int set_error(ohx_ctx* ctx, int status, const std::string& msg);
 int check_hip(ohx_ctx* ctx, hipError_t e, const char* what);
 int geometry_from_params(ohx_ctx* ctx, const ohx_pc_params* p, uint32_t diff, Geometry* g);
 int ensure_points(ohx_ctx* ctx, int64_t n, PointSoA* soa);
+size_t points_bytes(int64_t n);
+void carve_points(void* base, int64_t n, PointSoA* soa);
 int ensure_gemm_scratch(ohx_ctx* ctx, size_t bytes);
 
 // launchers (pc_direct.hip)
 int launch_prep(ohx_ctx* ctx, const void* d_vertices, const Geometry& g, const PointSoA& soa);
 int launch_direct(ohx_ctx* ctx, const Geometry& g, const PointSoA& soa, double* d_field, bool fp64);
 int launch_fresnel_gemm(ohx_ctx* ctx, const Geometry& g, const PointSoA& soa, double* d_field, uint32_t mode);
+int direct_tiles(const Geometry& g, int* tiles_x);
+int launch_direct_jobs(hipStream_t stream, const DirectJob* d_jobs, const int* d_njobs, int grid);
+int launch_fresnel_gemm_range(hipStream_t stream, void* scratch, size_t scratch_bytes, const Geometry& g,
+                              const PointSoA& soa, int64_t p0, int64_t p1, double* d_field, uint32_t mode);
+size_t fresnel_gemm_scratch_bytes(const Geometry& g, int64_t npoints);
 int launch_probe(ohx_ctx* ctx, const Geometry& g, const PointSoA& soa, const int64_t* d_pix,
                  int64_t n_pix, double* d_out);
 

@@ -162,20 +162,32 @@ __device__ inline void exact_term(double dx, double dy, double z, double cyc0, d
     im -= amp * cs;
 }
 
+// LDS of one workgroup (one tile): point constants of the current chunk
+struct DirectSmem {
+    float4 c0[PCH];
+    float4 c1[PCH];
+    float4 R[WAVES][PCH][2];
+    int4 box[PCH];
+    double d[6][PCH];   // px, py, z, cyc0, kamp, q (slow list)
+    int cls[PCH];
+    int cnt[2][WAVES];
+};
+
+// Accumulates points [p_begin, p_end) of s into one 64 x 32 tile of field.
 template <int DIFF, bool FP64>
-__global__ __launch_bounds__(NT) void k_pc_direct(Geometry g, PointSoA s, double2* __restrict__ field,
-                                                  int tiles_x)
+__device__ void direct_tile(DirectSmem& sm, const Geometry& g, const PointSoA& s, int64_t p_begin,
+                           int64_t p_end, double2* __restrict__ field, int tile, int tiles_x)
 {
-    __shared__ float4 s_c0[PCH];
-    __shared__ float4 s_c1[PCH];
-    __shared__ float4 s_R[WAVES][PCH][2];
-    __shared__ int4 s_box[PCH];
-    __shared__ double s_d[6][PCH];   // px, py, z, cyc0, kamp, q (slow list)
-    __shared__ int s_cls[PCH];
-    __shared__ int s_cnt[2][WAVES];
+    float4* s_c0 = sm.c0;
+    float4* s_c1 = sm.c1;
+    auto& s_R = sm.R;
+    int4* s_box = sm.box;
+    auto& s_d = sm.d;
+    int* s_cls = sm.cls;
+    auto& s_cnt = sm.cnt;
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int tx = blockIdx.x % tiles_x, ty = blockIdx.x / tiles_x;
+    const int tx = tile % tiles_x, ty = tile / tiles_x;
     const int x0 = tx * TW, y0 = ty * TH;
     const int cx1 = min(x0 + TW, g.pn_x), ry1 = min(y0 + TH, g.pn_y);
     const int col = x0 + lane;
@@ -192,7 +204,7 @@ __global__ __launch_bounds__(NT) void k_pc_direct(Geometry g, PointSoA s, double
 #pragma unroll
     for (int k = 0; k < RPT; ++k) accr[k] = acci[k] = 0.0;
 
-    for (int64_t base = 0; base < s.n; base += PCH) {
+    for (int64_t base = p_begin; base < p_end; base += PCH) {
         // ---- classify and prepare one point per thread --------------------
         int cls = CLS_OUT;
         float4 c0 = make_float4(0, 0, 0, 0), c1 = make_float4(0, 0, 0, 0);
@@ -200,7 +212,7 @@ __global__ __launch_bounds__(NT) void k_pc_direct(Geometry g, PointSoA s, double
         int4 b = make_int4(0, 0, 0, 0);
         double px = 0, py = 0, z = 0, cyc0 = 0, kamp = 0, q = 0;
         const int64_t p = base + tid;
-        if (tid < PCH && p < s.n) {
+        if (tid < PCH && p < p_end) {
             b = s.box[p];
             if (b.x < cx1 && b.y > x0 && b.z < ry1 && b.w > y0) {
                 const bool full = b.x <= x0 && b.y >= cx1 && b.z <= y0 && b.w >= ry1;
@@ -417,6 +429,39 @@ __global__ __launch_bounds__(NT) void k_pc_direct(Geometry g, PointSoA s, double
     }
 }
 
+template <int DIFF, bool FP64>
+__global__ __launch_bounds__(NT) void k_pc_direct(Geometry g, PointSoA s, double2* __restrict__ field,
+                                                  int tiles_x)
+{
+    __shared__ DirectSmem sm;
+    direct_tile<DIFF, FP64>(sm, g, s, 0, s.n, field, blockIdx.x, tiles_x);
+}
+
+// Persistent multi-job form used by the serving engine: workgroups stride
+// over the (job, tile) items of every job in the table; each item adds the
+// job's current point slice [p0, p1) into that job's resident field.
+__global__ __launch_bounds__(NT) void k_pc_direct_jobs(const DirectJob* __restrict__ jobs,
+                                                       const int* __restrict__ njobs_ptr)
+{
+    __shared__ DirectSmem sm;
+    const int njobs = *njobs_ptr;
+    const int total = njobs > 0 ? jobs[njobs - 1].item_base + jobs[njobs - 1].tiles : 0;
+    for (int item = blockIdx.x; item < total; item += gridDim.x) {
+        int j = 0;
+        while (j + 1 < njobs && jobs[j + 1].item_base <= item) ++j;
+        const DirectJob& jb = jobs[j];
+        const int tile = item - jb.item_base;
+        if (jb.g.diff == OHX_DIFF_RS) {
+            if (jb.fp64) direct_tile<OHX_DIFF_RS, true>(sm, jb.g, jb.s, jb.p0, jb.p1, jb.field, tile, jb.tiles_x);
+            else direct_tile<OHX_DIFF_RS, false>(sm, jb.g, jb.s, jb.p0, jb.p1, jb.field, tile, jb.tiles_x);
+        } else {
+            if (jb.fp64) direct_tile<OHX_DIFF_FRESNEL, true>(sm, jb.g, jb.s, jb.p0, jb.p1, jb.field, tile, jb.tiles_x);
+            else direct_tile<OHX_DIFF_FRESNEL, false>(sm, jb.g, jb.s, jb.p0, jb.p1, jb.field, tile, jb.tiles_x);
+        }
+        __syncthreads();   // LDS reuse across items
+    }
+}
+
 // ---------------------------------------------------------------------------
 // FP64 accuracy probe: one thread per listed pixel, Neumaier summation
 __device__ inline void neumaier(double& s, double& c, double x)
@@ -502,6 +547,18 @@ int launch_direct(ohx_ctx* ctx, const Geometry& g, const PointSoA& soa, double* 
     ctx->last.workgroups = nwg;
     ctx->last.kernel_kind = fp64 ? 1 : 0;
     return check_hip(ctx, hipGetLastError(), "k_pc_direct");
+}
+
+int direct_tiles(const Geometry& g, int* tiles_x)
+{
+    *tiles_x = (g.pn_x + TW - 1) / TW;
+    return *tiles_x * ((g.pn_y + TH - 1) / TH);
+}
+
+int launch_direct_jobs(hipStream_t stream, const DirectJob* d_jobs, const int* d_njobs, int grid)
+{
+    hipLaunchKernelGGL(k_pc_direct_jobs, dim3(grid), dim3(NT), 0, stream, d_jobs, d_njobs);
+    return hipGetLastError() == hipSuccess ? OHX_OK : OHX_ERR_DEVICE;
 }
 
 int launch_probe(ohx_ctx* ctx, const Geometry& g, const PointSoA& soa, const int64_t* d_pix,

@@ -206,8 +206,8 @@ __device__ inline u32x4 to_im(u32x4 w)
 
 template <int VAR>
 __global__ __launch_bounds__(NT, 2) void k_fresnel_gemm(Geometry g, PointSoA s, float2* __restrict__ part,
-                                                        int tiles_x, int tiles, int64_t kslice,
-                                                        const float* __restrict__ amax)
+                                                        int tiles_x, int tiles, int64_t kfirst, int64_t klast,
+                                                        int64_t kslice, const float* __restrict__ amax)
 {
     constexpr bool THREE = Traits<VAR>::three;
     __shared__ __align__(16) Smem<THREE> sm;
@@ -216,8 +216,8 @@ __global__ __launch_bounds__(NT, 2) void k_fresnel_gemm(Geometry g, PointSoA s, 
     const int split = blockIdx.x / tiles;
     const int tx = tile % tiles_x, ty = tile / tiles_x;
     const int x0 = tx * BN, y0 = ty * BM;
-    const int64_t kb = (int64_t)split * kslice;
-    const int64_t ke = min(s.n, kb + kslice);
+    const int64_t kb = kfirst + (int64_t)split * kslice;
+    const int64_t ke = min(klast, kb + kslice);
     const double xc = gpix_x(g, x0 + BN / 2), yc = gpix_y(g, y0 + BM / 2);
     const float am = amax[0];
     const float vscale = (Traits<VAR>::fp16 && am > 0.f) ? 1.0f / am : 1.0f;
@@ -309,10 +309,11 @@ __global__ __launch_bounds__(256) void k_gemm_reduce(const float2* __restrict__ 
     }
 }
 
-__global__ __launch_bounds__(256) void k_amp_max(const double* __restrict__ kamp, int64_t n, float* amax)
+__global__ __launch_bounds__(256) void k_amp_max(const double* __restrict__ kamp, int64_t p0, int64_t p1,
+                                                 float* amax)
 {
     float m = 0.f;
-    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+    for (int64_t i = p0 + (int64_t)blockIdx.x * 256 + threadIdx.x; i < p1; i += (int64_t)gridDim.x * 256)
         m = fmaxf(m, fabsf((float)kamp[i]));
     for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
     if ((threadIdx.x & 63) == 0 && m > 0.f) atomicMax((unsigned*)amax, __float_as_uint(m));
@@ -325,56 +326,87 @@ __global__ void k_amp_init(float* amax)
 
 }  // namespace gemm
 
-int launch_fresnel_gemm(ohx_ctx* ctx, const Geometry& g, const PointSoA& soa, double* d_field, uint32_t mode)
+namespace {
+struct GemmPlan {
+    int tiles_x, tiles;
+    int64_t splits, kslice, npx;
+};
+GemmPlan plan_gemm(const Geometry& g, int64_t npoints)
 {
     using namespace gemm;
-    const int var = (mode & OHX_MODE_MFMA_FP16X3) ? V_FP16X3 : (mode & OHX_MODE_MFMA_BF16X3) ? V_BF16X3 : V_BF16X1;
-    const int tiles_x = (g.pn_x + BN - 1) / BN, tiles_y = (g.pn_y + BM - 1) / BM;
-    const int tiles = tiles_x * tiles_y;
+    GemmPlan p;
+    p.tiles_x = (g.pn_x + BN - 1) / BN;
+    p.tiles = p.tiles_x * ((g.pn_y + BM - 1) / BM);
+    p.npx = (int64_t)g.pn_x * g.pn_y;
     // Split K so that (a) every CU gets several workgroups (two reside per CU)
     // and (b) no FP32 accumulator sums more than MAX_SLICE points: FP32
     // rounding of the running sums grows like sqrt(points per slice).
     constexpr int64_t MAX_SLICE = 65536;
-    int64_t splits = std::max<int64_t>((1024 + tiles - 1) / tiles, (soa.n + MAX_SLICE - 1) / MAX_SLICE);
-    const int64_t max_by_k = std::max<int64_t>(1, (soa.n + 4 * KP - 1) / (4 * KP));  // >= 4 stages per split
+    const int64_t n = std::max<int64_t>(npoints, 1);
+    int64_t splits = std::max<int64_t>((1024 + p.tiles - 1) / p.tiles, (n + MAX_SLICE - 1) / MAX_SLICE);
+    const int64_t max_by_k = std::max<int64_t>(1, (n + 4 * KP - 1) / (4 * KP));  // >= 4 stages per split
     splits = std::min(splits, max_by_k);
-    int64_t kslice = (soa.n + splits - 1) / splits;
+    int64_t kslice = (n + splits - 1) / splits;
     kslice = (kslice + KP - 1) / KP * KP;
-    splits = std::max<int64_t>(1, (soa.n + kslice - 1) / kslice);
-    const int64_t npx = (int64_t)g.pn_x * g.pn_y;
-    const size_t part_bytes = (size_t)splits * npx * 2 * sizeof(float);
-    int st = ensure_gemm_scratch(ctx, part_bytes + 256);
-    if (st) return st;
-    float* amax = (float*)ctx->gemm_scratch;
-    float2* part = (float2*)((char*)ctx->gemm_scratch + 256);
-    hipLaunchKernelGGL(k_amp_init, dim3(1), dim3(64), 0, ctx->stream, amax);
-    hipLaunchKernelGGL(k_amp_max, dim3(256), dim3(256), 0, ctx->stream, (const double*)soa.kamp, soa.n, amax);
-    const size_t shmem = 0;
-    const dim3 grid((unsigned)((int64_t)tiles * splits)), block(NT);
-    switch (var) {
-    case V_BF16X1:
-        hipLaunchKernelGGL(k_fresnel_gemm<V_BF16X1>, grid, block, shmem, ctx->stream, g, soa, part, tiles_x, tiles,
-                           kslice, (const float*)amax);
-        break;
-    case V_BF16X3:
-        hipLaunchKernelGGL(k_fresnel_gemm<V_BF16X3>, grid, block, shmem, ctx->stream, g, soa, part, tiles_x, tiles,
-                           kslice, (const float*)amax);
-        break;
-    default:
-        hipLaunchKernelGGL(k_fresnel_gemm<V_FP16X3>, grid, block, shmem, ctx->stream, g, soa, part, tiles_x, tiles,
-                           kslice, (const float*)amax);
-        break;
-    }
-    st = check_hip(ctx, hipGetLastError(), "k_fresnel_gemm");
-    if (st) return st;
-    int64_t rg = (npx + 255) / 256;
-    if (rg > 65536) rg = 65536;
-    hipLaunchKernelGGL(k_gemm_reduce, dim3((unsigned)rg), dim3(256), 0, ctx->stream, (const float2*)part, (int)splits,
-                       npx, (double2*)d_field, (const float*)amax, var == V_FP16X3 ? 1 : 0);
-    ctx->last.workgroups = (int64_t)tiles * splits;
-    ctx->last.kernel_kind = 2;
-    return check_hip(ctx, hipGetLastError(), "k_gemm_reduce");
+    p.splits = std::max<int64_t>(1, (n + kslice - 1) / kslice);
+    p.kslice = kslice;
+    return p;
+}
+}  // namespace
+
+size_t fresnel_gemm_scratch_bytes(const Geometry& g, int64_t npoints)
+{
+    const GemmPlan p = plan_gemm(g, npoints);
+    return 256 + (size_t)p.splits * p.npx * sizeof(float2);
 }
 
+int launch_fresnel_gemm_range(hipStream_t stream, void* scratch, size_t scratch_bytes, const Geometry& g,
+                              const PointSoA& soa, int64_t p0, int64_t p1, double* d_field, uint32_t mode)
+{
+    using namespace gemm;
+    if (p1 <= p0) return OHX_OK;
+    const int var = (mode & OHX_MODE_MFMA_FP16X3) ? V_FP16X3 : (mode & OHX_MODE_MFMA_BF16X3) ? V_BF16X3 : V_BF16X1;
+    const GemmPlan pl = plan_gemm(g, p1 - p0);
+    if (256 + (size_t)pl.splits * pl.npx * sizeof(float2) > scratch_bytes) return OHX_ERR_INVALID_ARG;
+    float* amax = (float*)scratch;
+    float2* part = (float2*)((char*)scratch + 256);
+    hipLaunchKernelGGL(k_amp_init, dim3(1), dim3(64), 0, stream, amax);
+    hipLaunchKernelGGL(k_amp_max, dim3(256), dim3(256), 0, stream, (const double*)soa.kamp, p0, p1, amax);
+    const dim3 grid((unsigned)((int64_t)pl.tiles * pl.splits)), block(NT);
+    switch (var) {
+    case V_BF16X1:
+        hipLaunchKernelGGL(k_fresnel_gemm<V_BF16X1>, grid, block, 0, stream, g, soa, part, pl.tiles_x, pl.tiles,
+                           p0, p1, pl.kslice, (const float*)amax);
+        break;
+    case V_BF16X3:
+        hipLaunchKernelGGL(k_fresnel_gemm<V_BF16X3>, grid, block, 0, stream, g, soa, part, pl.tiles_x, pl.tiles,
+                           p0, p1, pl.kslice, (const float*)amax);
+        break;
+    default:
+        hipLaunchKernelGGL(k_fresnel_gemm<V_FP16X3>, grid, block, 0, stream, g, soa, part, pl.tiles_x, pl.tiles,
+                           p0, p1, pl.kslice, (const float*)amax);
+        break;
+    }
+    if (hipGetLastError() != hipSuccess) return OHX_ERR_DEVICE;
+    int64_t rg = (pl.npx + 255) / 256;
+    if (rg > 65536) rg = 65536;
+    hipLaunchKernelGGL(k_gemm_reduce, dim3((unsigned)rg), dim3(256), 0, stream, (const float2*)part, (int)pl.splits,
+                       pl.npx, (double2*)d_field, (const float*)amax, var == V_FP16X3 ? 1 : 0);
+    return hipGetLastError() == hipSuccess ? OHX_OK : OHX_ERR_DEVICE;
+}
+
+int launch_fresnel_gemm(ohx_ctx* ctx, const Geometry& g, const PointSoA& soa, double* d_field, uint32_t mode)
+{
+    const size_t bytes = fresnel_gemm_scratch_bytes(g, soa.n);
+    int st = ensure_gemm_scratch(ctx, bytes);
+    if (st) return st;
+    st = launch_fresnel_gemm_range(ctx->stream, ctx->gemm_scratch, ctx->gemm_scratch_bytes, g, soa, 0, soa.n,
+                                   d_field, mode);
+    const GemmPlan pl = plan_gemm(g, soa.n);
+    ctx->last.workgroups = (int64_t)pl.tiles * pl.splits;
+    ctx->last.kernel_kind = 2;
+    if (st) return set_error(ctx, st, "matrix-core Fresnel launch failed");
+    return OHX_OK;
+}
 
 }  // namespace ohx

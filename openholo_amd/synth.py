@@ -94,3 +94,26 @@ def random_cloud(seed: int, n: int, xy_half: float, z_range, amp_range=(0.1, 1.0
     w = Workload("adhoc", n, xy_half, tuple(z_range), (1, 1), (1.0, 1.0), (1.0,), (0,), seed,
                  complex_amplitude=complex_amplitude, amp_range=tuple(amp_range))
     return points(w)
+
+
+def serving_requests(count: int, seed: int = 20261202 + 4):
+    """The mixed request stream: point count log-uniform in [1e3, 5e5],
+    resolution uniform over {1920x1080 at 8 um, 3840x2160 at 3.74 um},
+    wavelength uniform over {638, 532, 473} nm, Fresnel with probability 0.7
+    else RS; each request's points drawn as in the 10k-point RS workload
+    (x, y in +-4 mm, z in [0.49, 0.51] m, amplitude in [0.1, 1]) from its own
+    seed.  Draws per request, in order: count, resolution, wavelength, mode."""
+    u = uniform01(seed, 4 * count).reshape(count, 4)
+    reqs = []
+    for i in range(count):
+        n = int(round(10 ** (3 + u[i, 0] * (math.log10(5e5) - 3))))
+        hi = u[i, 1] >= 0.5
+        reqs.append({
+            "n": n,
+            "pn": (3840, 2160) if hi else (1920, 1080),
+            "pp": (3.74e-6, 3.74e-6) if hi else (8e-6, 8e-6),
+            "lambda": (638e-9, 532e-9, 473e-9)[min(2, int(u[i, 2] * 3))],
+            "diff": 1 if u[i, 3] < 0.7 else 0,
+            "seed": seed * 1000 + i,
+        })
+    return reqs

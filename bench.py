@@ -49,6 +49,8 @@ def parse():
     ap.add_argument("--workload", type=int, default=2, choices=[1, 2, 3, 5])
     ap.add_argument("--points", type=int, default=0, help="override point count (0 = workload's)")
     ap.add_argument("--kernel", default="direct", choices=list(KERNELS))
+    ap.add_argument("--fresnel-kernel", default=None, choices=list(KERNELS),
+                    help="kernel for the Fresnel part of the step (default: --kernel)")
     ap.add_argument("--variants", type=int, default=1,
                     help="also time the 1M-point Fresnel workload on every Fresnel kernel (0 = off)")
     ap.add_argument("--serving", type=int, default=1,
@@ -87,10 +89,13 @@ def cpu_baseline(w, target_s: float):
                       f"(oracle/ restatement of genCghPointCloudCPU, g++ -O3 -fopenmp)"}
 
 
-# HBM bytes per launch of the dominant direct kernels at the default workload
-# (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, scaled by the counters' reading of
-# a copy kernel of known size); see profiles/direct_100k_2160p_pmc.txt.
-TRAFFIC = {}
+# HBM bytes per launch of the direct kernels at the default workload
+# (rocprofv3 --pmc FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the scale factors read
+# off a 1 GiB copy kernel); profiles/k_pc_direct_rs_fresnel_100k_2160p.md.
+TRAFFIC = {0: 1.060e9, 1: 1.067e9}
+# Speed of light of the direct kernels' per-pair instruction mix on this
+# hardware (openholo_amd/bin/valu_rates); profiles/valu_mfma_rates_gfx950.md.
+MIX_SOL_PAIRS_PER_S = {0: 3.63e12, 1: 4.88e12}
 
 
 def fresnel_variants(ctx, capi, synth, Event, launches=2):
@@ -220,6 +225,8 @@ def main():
     w = synth.WORKLOADS[args.workload]
     n = args.points or w.n_points
     mode = KERNELS[args.kernel]
+    fmode = KERNELS[args.fresnel_kernel] if args.fresnel_kernel else mode
+    mode_of = {0: mode, 1: fmode}
     npx = w.n_pixels
     nch = len(w.lambdas)
     pts = synth.points(w, n)
@@ -242,14 +249,14 @@ def main():
         e0.record(stream)
         for ch in range(nch):
             ctx.field_zero(fields[ch], npx)
-            if timed and diff == dom:
+            if timed:
                 a, b = Event(), Event()
                 a.record(stream)
-                ctx.diffract(dv, n, fields[ch], params[ch], diff, mode)
+                ctx.diffract(dv, n, fields[ch], params[ch], diff, mode_of[diff])
                 b.record(stream)
                 kev.setdefault(diff, []).append((a, b))
             else:
-                ctx.diffract(dv, n, fields[ch], params[ch], diff, mode)
+                ctx.diffract(dv, n, fields[ch], params[ch], diff, mode_of[diff])
         for ch in range(nch):
             ctx.encode(fields[ch], npx, encs[ch], capi.ENCODE_PHASE)
         ctx.normalize_u8(encs, npx, u8s)
@@ -288,11 +295,16 @@ def main():
     if args.kernel == "direct":
         fl = FLOP_PER_PAIR[dom]
         achieved = pairs_launch / (k_ms / 1e3) * fl / 1e12
+        pps = pairs_launch / (k_ms / 1e3)
+        traffic = TRAFFIC.get(dom) if (args.workload == 2 and n == w.n_points) else None
         roofline = {"bound": "valu", "achieved": round(achieved, 2), "peak": PEAK_FP32_TFLOPS,
-                    "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": TRAFFIC.get(dom),
+                    "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": traffic,
                     "kernel": f"k_pc_direct<{'RS' if dom == 0 else 'Fresnel'},FP32>",
                     "avg_launch_ms": round(k_ms, 3),
-                    "model": f"{fl} FP32 FLOP/pair executed (FMA=2, sin/cos/fract/add=1)"}
+                    "model": f"{fl} FP32 FLOP/pair executed (FMA=2, sin/cos/fract/add=1); published peak "
+                             f"counts packed FP32, which is not emitted",
+                    "mix_sol_pairs_per_s": MIX_SOL_PAIRS_PER_S[dom],
+                    "frac_of_mix_sol": round(pps / MIX_SOL_PAIRS_PER_S[dom], 4)}
     elif args.kernel in MFMA_TERMS and dom == 1:
         achieved = pairs_launch / (k_ms / 1e3) * 8 * MFMA_TERMS[args.kernel] / 1e12
         roofline = {"bound": "mfma", "achieved": round(achieved, 2), "peak": PEAK_BF16_TFLOPS,
@@ -302,6 +314,9 @@ def main():
     else:
         roofline = {"bound": "valu", "achieved": None, "peak": 78.6, "unit": "TFLOP/s", "frac": None,
                     "traffic": None, "avg_launch_ms": round(k_ms, 3)}
+
+    per_mode_ms = {("RS" if d == 0 else "Fresnel"): round(float(np.mean([a.elapsed_ms(b) for a, b in ev])), 3)
+                   for d, ev in kev.items()}
 
     # p50 / p99 per hologram (one mode, all channels, encode + normalise)
     hol = sorted(a.elapsed_ms(b) for a, b in holo_ev)
@@ -316,7 +331,7 @@ def main():
         num = den = 0.0
         for ch in range(nch):
             ctx.field_zero(fields[ch], npx)
-            ctx.diffract(dv, n, fields[ch], params[ch], diff, mode)
+            ctx.diffract(dv, n, fields[ch], params[ch], diff, mode_of[diff])
             f = fields[ch].download(np.complex128, npx)[pix]
             ref = ctx.probe_f64(dv, n, params[ch], diff, pix)
             num += float(np.sum(np.abs(f - ref) ** 2))
@@ -346,8 +361,10 @@ def main():
         "data": "synthetic (SplitMix64 point cloud)",
         "config": {"workload": w.name, "points": n, "resolution": list(w.pn), "pitch_m": list(w.pp),
                    "lambdas_m": list(w.lambdas), "modes": ["RS" if m == 0 else "Fresnel" for m in w.modes],
-                   "kernel": args.kernel, "parallelism": "single GPU"},
+                   "kernel": args.kernel, "fresnel_kernel": args.fresnel_kernel or args.kernel,
+                   "parallelism": "single GPU"},
         "field_rel_l2_vs_fp64": {k: float(f"{v:.3e}") for k, v in rel.items()},
+        "kernel_ms_per_channel": per_mode_ms,
         "probe_pixels": int(len(pix)),
         "p50_ms_per_hologram": round(p50, 3) if p50 else None,
         "p99_ms_per_hologram": round(p99, 3) if p99 else None,

@@ -15,9 +15,13 @@
 // then does 8 points x 16 output columns' worth of complex MACs per instruction.
 //
 // Operands are never stored in HBM: per 16-point stage every thread generates
-// 8 V and 8 U elements (FP64 per-point constants recentred on the tile centre,
-// FP32 phase offsets < ~20 cycles, v_sin/v_cos) into double-buffered LDS while
-// the waves run the previous stage's MFMAs.  Precision variants:
+// 8 V and 8 U elements into double-buffered LDS.  Along a thread's rows (and
+// columns) the phase is a quadratic in the row index whose three coefficients
+// are computed in FP64 and reduced mod 1, so the FP32 evaluation (two FMAs,
+// then v_sin/v_cos) only sums terms below 16 cycles for any point depth.  Each
+// wave's generation of stage s+1 is software-pipelined with its MFMAs of
+// stage s, slot by slot (a few VALU instructions after every MFMA), so that
+// the VALU work issues while the matrix core runs.  Precision variants:
 //   BF16x1 : hi(V) hi(U)
 //   BF16x3 : hi hi + hi lo + lo hi   (lo = bf16(x - hi): ~2^-17 per factor)
 //   FP16x3 : same split in FP16 (~2^-22), V scaled by 1/max|A| to stay in range
@@ -45,9 +49,6 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 enum { V_BF16X1 = 0, V_BF16X3 = 1, V_FP16X3 = 2 };
-
-// largest phase excursion (cycles) over a tile generated with FP32 offsets
-constexpr double MAX_SPAN = 64.0;
 
 template <int VAR>
 struct Traits {
@@ -113,84 +114,141 @@ __device__ inline double gpix_y(const Geometry& g, int row)
     return g.y0 + (double)(g.pn_y - row + g.off_y) * g.pp_y;
 }
 
-// One stage: KP points from p0 (valid below p1) into buffer b.  Thread t
-// handles point t % 16, V rows and U columns t/16 + 16 i (i < 8).
+// Point data of one stage as read from HBM: thread t holds point t % 16.
+struct Raw {
+    double px, py, q, cyc0, kamp;
+    int4 box;
+    bool valid;
+};
+
+// Lanes past the slice end read the slice's last point and are marked invalid
+// (zero amplitude, full box), so the loads are unconditional and in bounds.
+__device__ inline void load_raw(Raw& r, const PointSoA& s, int64_t pi, int64_t p1)
+{
+    r.valid = pi < p1;
+    const int64_t j = r.valid ? pi : p1 - 1;
+    r.px = s.px[j];
+    r.py = s.py[j];
+    r.q = s.q[j];
+    r.cyc0 = s.cyc0[j];
+    r.kamp = s.kamp[j];
+    r.box = s.box[j];
+}
+
+// Per-(point, thread) constants of one stage.  Thread t generates V rows and
+// U columns grp + 16 i (grp = t / 16, i < 8); with k = i - 4 the phase (cycles)
+// of its k-th row is exactly  v0 + k v1 + k^2 v2  (mod 1), a quadratic in the
+// integer k, and likewise u0 + k u1 + k^2 u2 for its columns.  The three
+// coefficients are computed in FP64 and reduced mod 1 before rounding to FP32
+// (for integer k, k * frac(c) = k * c mod 1), so the FP32 evaluation only
+// ever sums terms below 1, 4 and 16 cycles, whatever the phase excursion.
+struct Pt {
+    float v0, v1, v2, u0, u1, u2, amp;
+    int rlo, rhi, clo, chi;   // box relative to the tile origin
+};
+
+// x - floor(x) in one v_fract_f64 (clamped below 1.0)
+__device__ inline double frac64(double x) { return __builtin_amdgcn_fract(x); }
+
+// ythr / xthr: y of row grp + 64 and x of column grp + 64 of the tile;
+// dy = -16 pp_y and dx = 16 pp_x (row index grows toward -y).
+__device__ inline void prep_y(Pt& P, const Raw& r, double ythr, double dy)
+{
+    const double ey = ythr - r.py, ty = r.q * ey;
+    P.v0 = (float)frac64(ty * ey + r.cyc0);
+    P.v1 = (float)frac64(ty * (2.0 * dy));
+    P.v2 = (float)frac64(r.q * (dy * dy));
+}
+__device__ inline void prep_x(Pt& P, const Raw& r, double xthr, double dx)
+{
+    const double ex = xthr - r.px, tx = r.q * ex;
+    P.u0 = (float)frac64(tx * ex);
+    P.u1 = (float)frac64(tx * (2.0 * dx));
+    P.u2 = (float)frac64(r.q * (dx * dx));
+}
+// amplitude and box; returns true when the box does not cover the whole tile
+__device__ inline bool prep_rest(Pt& P, const Raw& r, int x0, int y0, float vscale)
+{
+    P.amp = r.valid ? (float)r.kamp * vscale : 0.f;
+    P.rlo = r.valid ? r.box.z - y0 : 0;
+    P.rhi = r.valid ? r.box.w - y0 : BM;
+    P.clo = r.valid ? r.box.x - x0 : 0;
+    P.chi = r.valid ? r.box.y - x0 : BN;
+    return P.rlo > 0 || P.rhi < BM || P.clo > 0 || P.chi < BN;
+}
+__device__ inline bool prep(Pt& P, const Raw& r, double ythr, double xthr, double dy, double dx, int x0, int y0,
+                            float vscale)
+{
+    prep_y(P, r, ythr, dy);
+    prep_x(P, r, xthr, dx);
+    return prep_rest(P, r, x0, y0, vscale);
+}
+
+// phase (cycles, |phase| < 21) of the element k = i - 4 from its coefficients
+// (i is a compile-time constant wherever this is called).  No mod-1 reduction:
+// v_sin_f32 / v_cos_f32 reduce a revolutions argument themselves (on gfx950
+// sin(x) and sin(fract(x)) agree to 1 ulp for |x| <= 256), and the rounding
+// of the FMAs above is the same with or without it.
+__device__ inline float phase_at(int i, float c0, float c1, float c2)
+{
+    const float k = (float)(i - 4);
+    return i == 4 ? c0 : fmaf(fmaf(c2, k, c1), k, c0);
+}
+
+// Stores one element pair (hi and, 3-term, lo parts) of V or U.
 template <int VAR>
-__device__ inline void generate(Smem<Traits<VAR>::three>& sm, int b, const Geometry& g, const PointSoA& s,
-                                int64_t p0, int64_t p1, int x0, int y0, double xc, double yc, float vscale,
-                                float v0, float u0, float ppy16, float ppx16)
+__device__ inline void put(unsigned* hi_slot, unsigned* lo_slot, float re, float im)
 {
     constexpr bool FP16 = Traits<VAR>::fp16;
-    constexpr bool THREE = Traits<VAR>::three;
-    const int t = threadIdx.x;
-    const int p = t & (KP - 1), grp = t >> 4;
-    const int64_t pi = p0 + p;
-    float ac = 0.f, gx = 0.f, bc = 0.f, gy = 0.f, qf = 0.f, amp = 0.f;
-    double a64 = 0, b64 = 0, gx64 = 0, gy64 = 0, q64 = 0;
-    bool big = false;
-    int rlo = 0, rhi = 0, clo = 0, chi = 0;
-    if (pi < p1) {
-        const double px = s.px[pi], py = s.py[pi], q = s.q[pi];
-        const int4 box = s.box[pi];
-        rlo = box.z - y0;
-        rhi = box.w - y0;
-        clo = box.x - x0;
-        chi = box.y - x0;
-        const double dxc = xc - px, dyc = yc - py;
-        a64 = dxc * dxc * q;
-        a64 -= floor(a64);
-        b64 = dyc * dyc * q + s.cyc0[pi];
-        b64 -= floor(b64);
-        gx64 = 2.0 * dxc * q;
-        gy64 = 2.0 * dyc * q;
-        q64 = q;
-        ac = (float)a64;
-        bc = (float)b64;
-        gx = (float)gx64;
-        gy = (float)gy64;
-        qf = (float)q;
-        amp = (float)(s.kamp[pi]) * vscale;
-        // phase excursion over the tile; beyond MAX_SPAN cycles FP32 offsets
-        // lose accuracy, so this point's elements are phased in FP64
-        const double um = (BN / 2) * g.pp_x, vm = (BM / 2) * g.pp_y;
-        big = (fabs(gx64) * um + q * um * um + fabs(gy64) * vm + q * vm * vm) > MAX_SPAN;
+    const unsigned h = pack2<FP16>(re, im);
+    *hi_slot = h;
+    if constexpr (Traits<VAR>::three) *lo_slot = pack2<FP16>(re - lo16<FP16>(h), im - hi16<FP16>(h));
+}
+
+// LDS plane layout: plane h (hi / lo part) of one buffer's V or U block is
+// BM (= BN) rows of ROWW dwords; element (row, p) at h*PLANE + row*ROWW + p.
+constexpr int PLANE = BM * ROWW;
+
+template <int VAR, int I>
+__device__ inline void gen_elems(unsigned* wv, unsigned* wu, const Pt& P, int p, int grp)
+{
+    if constexpr (I < BM / 16) {
+        constexpr int H2 = Traits<VAR>::three ? PLANE : 0;
+        const int e = (grp + 16 * I) * ROWW + p;
+        const float fv = phase_at(I, P.v0, P.v1, P.v2);
+        put<VAR>(wv + e, wv + H2 + e, P.amp * __builtin_amdgcn_cosf(fv), P.amp * __builtin_amdgcn_sinf(fv));
+        const float fu = phase_at(I, P.u0, P.u1, P.u2);
+        put<VAR>(wu + e, wu + H2 + e, __builtin_amdgcn_cosf(fu), __builtin_amdgcn_sinf(fu));
+        gen_elems<VAR, I + 1>(wv, wu, P, p, grp);
     }
+}
+
+// Generation of one stage into buffer WB, ignoring the boxes: thread t
+// handles point p = t % 16, V rows and U columns grp + 16 i (i < 8).
+template <int VAR, int WB>
+__device__ inline void gen_stage(Smem<Traits<VAR>::three>& sm, const Pt& P, int p, int grp)
+{
+    gen_elems<VAR, 0>(&sm.v[WB][0][0][0], &sm.u[WB][0][0][0], P, p, grp);
+}
+
+// The boxes: zeroes the elements of rows / columns outside the point's box
+// (the same thread wrote them, so program order makes the zeros win).  Run
+// only by waves holding a point whose box does not cover the tile.
+template <int VAR, int WB>
+__device__ inline void mask_stage(Smem<Traits<VAR>::three>& sm, const Pt& P, int p, int grp)
+{
+    constexpr int H2 = Traits<VAR>::three ? 1 : 0;
 #pragma unroll
     for (int i = 0; i < BM / 16; ++i) {
         const int r = grp + 16 * i;
-        const float v = fmaf(-(float)i, ppy16, v0);          // y_r - y_c
-        float f;
-        if (big) {
-            const double vd = (double)(BM / 2 - r) * g.pp_y;
-            const double ph = b64 + vd * (gy64 + q64 * vd);
-            f = (float)(ph - floor(ph));
-        } else {
-            f = __builtin_amdgcn_fractf(fmaf(fmaf(qf, v, gy), v, bc));
+        if (r < P.rlo || r >= P.rhi) {
+            sm.v[WB][0][r][p] = 0u;
+            sm.v[WB][H2][r][p] = 0u;
         }
-        const float a = (r >= rlo && r < rhi) ? amp : 0.f;
-        const float vr = a * __builtin_amdgcn_cosf(f), vi = a * __builtin_amdgcn_sinf(f);
-        const unsigned h = pack2<FP16>(vr, vi);
-        sm.v[b][0][r][p] = h;
-        if constexpr (THREE) sm.v[b][1][r][p] = pack2<FP16>(vr - lo16<FP16>(h), vi - hi16<FP16>(h));
-    }
-#pragma unroll
-    for (int i = 0; i < BN / 16; ++i) {
-        const int c = grp + 16 * i;
-        const float u = fmaf((float)i, ppx16, u0);            // x_c - x_c(tile)
-        float f;
-        if (big) {
-            const double ud = (double)(c - BN / 2) * g.pp_x;
-            const double ph = a64 + ud * (gx64 + q64 * ud);
-            f = (float)(ph - floor(ph));
-        } else {
-            f = __builtin_amdgcn_fractf(fmaf(fmaf(qf, u, gx), u, ac));
+        if (r < P.clo || r >= P.chi) {     // column index grp + 16 i
+            sm.u[WB][0][r][p] = 0u;
+            sm.u[WB][H2][r][p] = 0u;
         }
-        const bool in = c >= clo && c < chi;
-        const float ur = in ? __builtin_amdgcn_cosf(f) : 0.f;
-        const float ui = in ? __builtin_amdgcn_sinf(f) : 0.f;
-        const unsigned h = pack2<FP16>(ur, ui);
-        sm.u[b][0][c][p] = h;
-        if constexpr (THREE) sm.u[b][1][c][p] = pack2<FP16>(ur - lo16<FP16>(h), ui - hi16<FP16>(h));
     }
 }
 
@@ -202,6 +260,161 @@ __device__ inline u32x4 to_im(u32x4 w)
 #pragma unroll
     for (int i = 0; i < 4; ++i) r[i] = __builtin_amdgcn_alignbit(w[i], w[i], 16);
     return r;
+}
+
+// State of one pipelined stage (see stage_body).
+template <int VAR>
+struct StageCtx {
+    f32x16 (&acc)[4][2];
+    Raw& R;
+    const PointSoA& s;
+    int64_t pnext, ke;
+    double ythr, xthr, dy, dx;
+    int x0, y0;
+    float vscale;
+    int p, grp, lrow, koff, ucol;
+    Pt P;
+    bool partial;
+    u32x4 A[8][2], Ub[2][2], B[2][4];
+    float2 cs[16];
+};
+
+// MFMA order per (ks, mb) group: 3-term (ah,Re) (ah,Im) (ah,lo Re) (ah,lo Im)
+// (al,Re) (al,Im); 1-term (ah,Re) (ah,Im).  Wave w owns output columns
+// 32w .. 32w+31 of the tile (Re and Im blocks) and all 128 rows (4 blocks).
+template <int VAR>
+__device__ inline void load_a(const unsigned* rv, StageCtx<VAR>& c, int g)
+{
+    const unsigned* a = rv + ((g & 3) * 32 + c.lrow) * ROWW + (g >> 2) * 8 + c.koff;
+    c.A[g][0] = *reinterpret_cast<const u32x4*>(a);
+    if constexpr (Traits<VAR>::three) c.A[g][1] = *reinterpret_cast<const u32x4*>(a + PLANE);
+}
+template <int VAR>
+__device__ inline void load_b(const unsigned* ru, StageCtx<VAR>& c, int ks)
+{
+    const unsigned* b = ru + c.ucol * ROWW + ks * 8 + c.koff;
+    c.Ub[ks][0] = *reinterpret_cast<const u32x4*>(b);
+    if constexpr (Traits<VAR>::three) c.Ub[ks][1] = *reinterpret_cast<const u32x4*>(b + PLANE);
+}
+template <int VAR>
+__device__ inline void xform_b(StageCtx<VAR>& c, int ks, int h)
+{
+    c.B[ks][2 * h] = to_re(c.Ub[ks][h]);
+    c.B[ks][2 * h + 1] = to_im(c.Ub[ks][h]);
+}
+// element E: V row (E even) or U column (E odd) grp + 16 (E / 2); first the
+// phase and sincos (V: times the amplitude), then packing and LDS stores
+template <int VAR, int E>
+__device__ inline void elem_a(StageCtx<VAR>& c)
+{
+    constexpr int i = E >> 1;
+    if constexpr (E & 1) {
+        const float f = phase_at(i, c.P.u0, c.P.u1, c.P.u2);
+        c.cs[E] = make_float2(__builtin_amdgcn_cosf(f), __builtin_amdgcn_sinf(f));
+    } else {
+        const float f = phase_at(i, c.P.v0, c.P.v1, c.P.v2);
+        c.cs[E] = make_float2(c.P.amp * __builtin_amdgcn_cosf(f), c.P.amp * __builtin_amdgcn_sinf(f));
+    }
+}
+template <int VAR, int E>
+__device__ inline void elem_b(unsigned* w, StageCtx<VAR>& c)
+{
+    constexpr int H2 = Traits<VAR>::three ? PLANE : 0;
+    const int e = (c.grp + 16 * (E >> 1)) * ROWW + c.p;
+    put<VAR>(w + e, w + H2 + e, c.cs[E].x, c.cs[E].y);
+}
+
+// Slot S of the pipelined stage: MFMA S, operand loads / transforms due at
+// this slot, and this slot's share of the next stage's generation.
+template <int VAR, int S>
+__device__ inline void slot(const unsigned* __restrict__ rv, const unsigned* __restrict__ ru,
+                            unsigned* __restrict__ wv, unsigned* __restrict__ wu, StageCtx<VAR>& c)
+{
+    constexpr bool THREE = Traits<VAR>::three;
+    constexpr int NTG = THREE ? 6 : 2;       // MFMAs per (ks, mb) group
+    constexpr int NM = 8 * NTG;              // MFMAs per stage and wave
+    if constexpr (S < NM) {
+        constexpr int g = S / NTG, t = S % NTG, ks = g >> 2, mb = g & 3;
+        c.acc[mb][t & 1] = mfma<VAR>(c.A[g][t < 4 ? 0 : 1], c.B[ks][t < 4 ? t : t - 4], c.acc[mb][t & 1]);
+        if constexpr (t == 0 && THREE && g + 1 < 8) load_a(rv, c, g + 1);
+        if constexpr (t == 0 && !THREE && g + 2 < 8) load_a(rv, c, g + 2);
+        if constexpr (S == NM / 4) load_b(ru, c, 1);
+        if constexpr (S == NM * 3 / 8) xform_b(c, 1, 0);
+        if constexpr (THREE && S == NM * 3 / 8 + 2) xform_b(c, 1, 1);
+        if constexpr (S == 0) prep_y(c.P, c.R, c.ythr, c.dy);
+        if constexpr (S == 1) prep_x(c.P, c.R, c.xthr, c.dx);
+        if constexpr (S == (THREE ? 2 : 1)) {
+            c.partial = prep_rest(c.P, c.R, c.x0, c.y0, c.vscale);
+            load_raw(c.R, c.s, c.pnext, c.ke);
+        }
+        if constexpr (THREE) {
+            // half-element k (k even: first half of element k/2) at slot 3 + (11 k) / 8
+            constexpr int k0 = ((S - 3) * 8 + 10) / 11;       // smallest k with 3 + 11k/8 >= S
+            if constexpr (S >= 3 && k0 < 32 && 3 + (11 * k0) / 8 == S) {
+                if constexpr (k0 & 1) elem_b<VAR, (k0 >> 1)>(((k0 >> 1) & 1) ? wu : wv, c);
+                else elem_a<VAR, (k0 >> 1)>(c);
+            }
+        } else {
+            // element E at slot 2 + (7 E) / 8 (two elements in some slots)
+            constexpr int e0 = ((S - 2) * 8 + 6) / 7;         // smallest E with 2 + 7E/8 >= S
+            if constexpr (S >= 2 && e0 < 16 && 2 + (7 * e0) / 8 == S) {
+                elem_a<VAR, e0>(c);
+                elem_b<VAR, e0>((e0 & 1) ? wu : wv, c);
+            }
+            if constexpr (S >= 2 && e0 + 1 < 16 && 2 + (7 * (e0 + 1)) / 8 == S) {
+                elem_a<VAR, e0 + 1>(c);
+                elem_b<VAR, e0 + 1>(((e0 + 1) & 1) ? wu : wv, c);
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        slot<VAR, S + 1>(rv, ru, wv, wu, c);
+    }
+}
+
+// One stage's MFMAs (from planes rv / ru) software-pipelined with the next
+// stage's operand generation (into planes wv / wu), slot by slot: slot S holds
+// MFMA S and a fixed share of the generation work (FP64 point constants, then
+// elements), fenced by sched_barrier so that the VALU work issues while the
+// matrix core runs.  The restrict-qualified planes tell the compiler that the
+// generation's LDS writes never alias the operand reads.  Also loads the
+// point data of the stage after next into R; returns true when this lane's
+// point has a box that does not cover the tile (P then holds the box).
+template <int VAR>
+__device__ inline bool stage_body(const unsigned* __restrict__ rv, const unsigned* __restrict__ ru,
+                                  unsigned* __restrict__ wv, unsigned* __restrict__ wu, f32x16 (&acc)[4][2],
+                                  Pt& P, Raw& R, const PointSoA& s, int64_t pnext, int64_t ke, double ythr,
+                                  double xthr, double dy, double dx, int x0, int y0, float vscale, int p, int grp,
+                                  int lrow, int koff, int ucol)
+{
+    constexpr bool THREE = Traits<VAR>::three;
+    StageCtx<VAR> c{acc, R, s, pnext, ke, ythr, xthr, dy, dx, x0, y0, vscale, p, grp, lrow, koff, ucol};
+    c.partial = false;
+    load_b(ru, c, 0);
+    load_a(rv, c, 0);
+    if constexpr (!THREE) load_a(rv, c, 1);
+    xform_b(c, 0, 0);
+    if constexpr (THREE) xform_b(c, 0, 1);
+    slot<VAR, 0>(rv, ru, wv, wu, c);
+    P = c.P;
+    return c.partial;
+}
+
+// One pipeline step: MFMAs of stage st (buffer RB) and generation of stage
+// st+1 (buffer RB^1) from R in one basic block, so that they interleave; R
+// then receives stage st+2's point data.  Past the last stage the generation
+// runs on clamped, zero-amplitude data into the unused buffer: the step stays
+// branch-free, which keeps the accumulators in place.
+template <int VAR, int RB>
+__device__ inline void step(Smem<Traits<VAR>::three>& sm, f32x16 (&acc)[4][2], Raw& R, const PointSoA& s, int st,
+                            int64_t kb, int64_t ke, double ythr, double xthr, double dy, double dx, int x0, int y0,
+                            float vscale, int p, int grp, int lrow, int koff, int ucolx)
+{
+    Pt P;
+    const bool partial = stage_body<VAR>(&sm.v[RB][0][0][0], &sm.u[RB][0][0][0], &sm.v[RB ^ 1][0][0][0],
+                                         &sm.u[RB ^ 1][0][0][0], acc, P, R, s, kb + (int64_t)(st + 2) * KP + p, ke,
+                                         ythr, xthr, dy, dx, x0, y0, vscale, p, grp, lrow, koff, ucolx);
+    if (__builtin_amdgcn_ballot_w64(partial)) mask_stage<VAR, RB ^ 1>(sm, P, p, grp);
+    __syncthreads();
 }
 
 template <int VAR>
@@ -218,63 +431,40 @@ __global__ __launch_bounds__(NT, 2) void k_fresnel_gemm(Geometry g, PointSoA s, 
     const int x0 = tx * BN, y0 = ty * BM;
     const int64_t kb = kfirst + (int64_t)split * kslice;
     const int64_t ke = min(klast, kb + kslice);
-    const double xc = gpix_x(g, x0 + BN / 2), yc = gpix_y(g, y0 + BM / 2);
+    const int nst = (int)((ke - kb + KP - 1) / KP);
     const float am = amax[0];
     const float vscale = (Traits<VAR>::fp16 && am > 0.f) ? 1.0f / am : 1.0f;
-    const int grp = threadIdx.x >> 4;
-    const float v0 = (float)((double)(BM / 2 - grp) * g.pp_y);
-    const float u0 = (float)((double)(grp - BN / 2) * g.pp_x);
-    const float ppy16 = (float)(16.0 * g.pp_y), ppx16 = (float)(16.0 * g.pp_x);
+    const int p = threadIdx.x & (KP - 1), grp = threadIdx.x >> 4;
+    const double ythr = gpix_y(g, y0 + grp + 64), xthr = gpix_x(g, x0 + grp + 64);
+    const double dy = -16.0 * g.pp_y, dx = 16.0 * g.pp_x;
 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lrow = lane & 31, koff = (lane >> 5) * 4;   // dword offset of this lane's 8 values
+    const int ucolx = wave * 32 + lrow;                     // this wave's 32 columns
+
+    // prologue: stage 0 into buffer 0, stage 1's point data into R (an empty
+    // split, which the planner never makes, would write zeros)
+    Raw R;
+    if (nst > 0) {
+        load_raw(R, s, kb + p, ke);
+        Pt P;
+        const bool partial = prep(P, R, ythr, xthr, dy, dx, x0, y0, vscale);
+        gen_stage<VAR, 0>(sm, P, p, grp);
+        if (partial) mask_stage<VAR, 0>(sm, P, p, grp);
+        load_raw(R, s, kb + KP + p, ke);
+    }
+    __syncthreads();
     f32x16 acc[4][2];
 #pragma unroll
     for (int i = 0; i < 4; ++i) acc[i][0] = acc[i][1] = f32x16{};
-
-    const int nst = (int)((ke - kb + KP - 1) / KP);
-    if (nst > 0) generate<VAR>(sm, 0, g, s, kb, ke, x0, y0, xc, yc, vscale, v0, u0, ppy16, ppx16);
-    __syncthreads();
-    const int lrow = lane & 31, koff = (lane >> 5) * 4;   // dword offset of this lane's 8 values
-    const int ucol = wave * 32 + lrow;                      // this wave's 32 columns
-    for (int st = 0; st < nst; ++st) {
-        const int b = st & 1;
-        if (st + 1 < nst)
-            generate<VAR>(sm, b ^ 1, g, s, kb + (int64_t)(st + 1) * KP, ke, x0, y0, xc, yc, vscale, v0, u0,
-                          ppy16, ppx16);
-#pragma unroll
-        for (int ks = 0; ks < 2; ++ks) {
-            const int kd = ks * 8 + koff;
-            u32x4 ah[4], al[4];
-#pragma unroll
-            for (int mb = 0; mb < 4; ++mb) {
-                ah[mb] = *reinterpret_cast<const u32x4*>(&sm.v[b][0][mb * 32 + lrow][kd]);
-                if constexpr (THREE) al[mb] = *reinterpret_cast<const u32x4*>(&sm.v[b][1][mb * 32 + lrow][kd]);
-            }
-            const u32x4 uh = *reinterpret_cast<const u32x4*>(&sm.u[b][0][ucol][kd]);
-            const u32x4 bhr = to_re(uh), bhi = to_im(uh);
-            u32x4 blr, bli;
-            if constexpr (THREE) {
-                const u32x4 ul = *reinterpret_cast<const u32x4*>(&sm.u[b][1][ucol][kd]);
-                blr = to_re(ul);
-                bli = to_im(ul);
-            }
-#pragma unroll
-            for (int mb = 0; mb < 4; ++mb) {
-                acc[mb][0] = mfma<VAR>(ah[mb], bhr, acc[mb][0]);
-                acc[mb][1] = mfma<VAR>(ah[mb], bhi, acc[mb][1]);
-                if constexpr (THREE) {
-                    acc[mb][0] = mfma<VAR>(ah[mb], blr, acc[mb][0]);
-                    acc[mb][1] = mfma<VAR>(ah[mb], bli, acc[mb][1]);
-                    acc[mb][0] = mfma<VAR>(al[mb], bhr, acc[mb][0]);
-                    acc[mb][1] = mfma<VAR>(al[mb], bhi, acc[mb][1]);
-                }
-            }
-        }
-        __syncthreads();
+    for (int st = 0; st < nst; st += 2) {
+        step<VAR, 0>(sm, acc, R, s, st, kb, ke, ythr, xthr, dy, dx, x0, y0, vscale, p, grp, lrow, koff, ucolx);
+        if (st + 1 >= nst) break;
+        step<VAR, 1>(sm, acc, R, s, st + 1, kb, ke, ythr, xthr, dy, dx, x0, y0, vscale, p, grp, lrow, koff, ucolx);
     }
 
     // FP32 partial sums S of this split, interleaved (Re, Im) per pixel
-    const int col = x0 + ucol;
+    const int col = x0 + ucolx;
     if (col < g.pn_x) {
         float2* out = part + (int64_t)split * g.pn_y * (int64_t)g.pn_x;
 #pragma unroll

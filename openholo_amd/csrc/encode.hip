@@ -11,7 +11,9 @@
 // Min/max are reduced per block (wave shuffles) and merged with 64-bit
 // atomics on an order-preserving integer image of the doubles; NaN is
 // skipped as the reference's < / > comparisons skip it.
+#include <algorithm>
 #include <cfloat>
+#include <cstdint>
 
 #include "ohx_internal.h"
 
@@ -62,20 +64,51 @@ __global__ void k_minmax_init(unsigned long long* mm)
     }
 }
 
+__device__ inline void minmax_upd(double v, unsigned long long& kmin, unsigned long long& kmax)
+{
+    if (v == v) {
+        const unsigned long long k = order_key(v);
+        kmin = k < kmin ? k : kmin;
+        kmax = k > kmax ? k : kmax;
+    }
+}
+
+// Few blocks (MINMAX_BLOCKS) so that only a few hundred 64-bit atomics meet
+// on the two result words; each thread keeps four 16-byte loads in flight.
+// Channel buffers that are not 16-byte aligned take the scalar loop.
+constexpr int MINMAX_BLOCKS = 512;
+
 __global__ __launch_bounds__(EB) void k_minmax(ChanPtrs cp, int nch, int64_t n, unsigned long long* mm)
 {
     unsigned long long kmin = ~0ull, kmax = 0ull;
+    const int64_t tid = (int64_t)blockIdx.x * EB + threadIdx.x;
     const int64_t stride = (int64_t)gridDim.x * EB;
     for (int c = 0; c < nch; ++c) {
         const double* __restrict__ e = cp.p[c];
-        for (int64_t i = (int64_t)blockIdx.x * EB + threadIdx.x; i < n; i += stride) {
-            const double v = e[i];
-            if (v == v) {
-                const unsigned long long k = order_key(v);
-                kmin = k < kmin ? k : kmin;
-                kmax = k > kmax ? k : kmax;
+        int64_t done = 0;
+        if ((reinterpret_cast<uintptr_t>(e) & 15) == 0) {
+            const double2* __restrict__ e2 = reinterpret_cast<const double2*>(e);
+            const int64_t n2 = n / 2;
+            int64_t i = tid;
+            for (; i + 3 * stride < n2; i += 4 * stride) {
+                const double2 a = e2[i], b = e2[i + stride], d = e2[i + 2 * stride], f = e2[i + 3 * stride];
+                minmax_upd(a.x, kmin, kmax);
+                minmax_upd(a.y, kmin, kmax);
+                minmax_upd(b.x, kmin, kmax);
+                minmax_upd(b.y, kmin, kmax);
+                minmax_upd(d.x, kmin, kmax);
+                minmax_upd(d.y, kmin, kmax);
+                minmax_upd(f.x, kmin, kmax);
+                minmax_upd(f.y, kmin, kmax);
             }
+            for (; i < n2; i += stride) {
+                const double2 a = e2[i];
+                minmax_upd(a.x, kmin, kmax);
+                minmax_upd(a.y, kmin, kmax);
+            }
+            done = 2 * n2;
         }
+        for (int64_t i = done + tid; i < n; i += stride) minmax_upd(e[i], kmin, kmax);
     }
     for (int off = 32; off > 0; off >>= 1) {
         const unsigned long long a = __shfl_xor(kmin, off);
@@ -100,18 +133,37 @@ __global__ __launch_bounds__(EB) void k_minmax(ChanPtrs cp, int nch, int64_t n, 
     }
 }
 
+__device__ inline uint8_t to_u8(double v, double mn, double gap)
+{
+    const double t = ((v - mn) / gap) * 255 + 0.5;
+    return (t >= 0.0 && t < 256.0) ? (uint8_t)(int)t : (uint8_t)0;
+}
+
+// Four pixels per thread (two 16-byte loads, one 4-byte store) where the
+// buffers are aligned for it; the remainder and unaligned buffers go pixel by
+// pixel.
 __global__ __launch_bounds__(EB) void k_to_u8(ChanPtrs cp, int nch, int64_t n, const unsigned long long* mm)
 {
     const double mn = key_value(mm[0]), mx = key_value(mm[1]);
     const double gap = mx - mn;
+    const int64_t tid = (int64_t)blockIdx.x * EB + threadIdx.x;
     const int64_t stride = (int64_t)gridDim.x * EB;
     for (int c = 0; c < nch; ++c) {
         const double* __restrict__ e = cp.p[c];
         uint8_t* __restrict__ o = cp.o[c];
-        for (int64_t i = (int64_t)blockIdx.x * EB + threadIdx.x; i < n; i += stride) {
-            const double t = ((e[i] - mn) / gap) * 255 + 0.5;
-            o[i] = (t >= 0.0 && t < 256.0) ? (uint8_t)(int)t : (uint8_t)0;
+        int64_t done = 0;
+        if ((reinterpret_cast<uintptr_t>(e) & 15) == 0 && (reinterpret_cast<uintptr_t>(o) & 3) == 0) {
+            const int64_t n4 = n / 4;
+            for (int64_t i = tid; i < n4; i += stride) {
+                const double2 a = reinterpret_cast<const double2*>(e)[2 * i];
+                const double2 b = reinterpret_cast<const double2*>(e)[2 * i + 1];
+                const unsigned w = (unsigned)to_u8(a.x, mn, gap) | ((unsigned)to_u8(a.y, mn, gap) << 8) |
+                                   ((unsigned)to_u8(b.x, mn, gap) << 16) | ((unsigned)to_u8(b.y, mn, gap) << 24);
+                reinterpret_cast<unsigned*>(o)[i] = w;
+            }
+            done = 4 * n4;
         }
+        for (int64_t i = done + tid; i < n; i += stride) o[i] = to_u8(e[i], mn, gap);
     }
 }
 
@@ -162,8 +214,9 @@ extern "C" int ohx_normalize_u8(ohx_ctx* c, const double* const* d_enc, int nch,
     std::lock_guard<std::mutex> lk(c->mu);
     (void)hipSetDevice(c->device);
     hipLaunchKernelGGL(k_minmax_init, dim3(1), dim3(64), 0, c->stream, c->minmax);
-    hipLaunchKernelGGL(k_minmax, dim3(egrid(n)), dim3(EB), 0, c->stream, cp, nch, n, c->minmax);
-    hipLaunchKernelGGL(k_to_u8, dim3(egrid(n)), dim3(EB), 0, c->stream, cp, nch, n,
+    const unsigned mg = std::min<unsigned>(egrid(n), MINMAX_BLOCKS);
+    hipLaunchKernelGGL(k_minmax, dim3(mg), dim3(EB), 0, c->stream, cp, nch, n, c->minmax);
+    hipLaunchKernelGGL(k_to_u8, dim3(egrid((n + 3) / 4)), dim3(EB), 0, c->stream, cp, nch, n,
                        (const unsigned long long*)c->minmax);
     return check_hip(c, hipGetLastError(), "k_normalize");
 }

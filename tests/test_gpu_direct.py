@@ -176,6 +176,38 @@ def test_encode_and_normalize_match_reference(ctx, orc):
             assert (gu == wu).mean() > 0.9999
 
 
+class _View:
+    """A device pointer offset into a DeviceBuffer (for alignment cases)."""
+    def __init__(self, buf, off):
+        self.ptr = buf.ptr + off
+
+
+@pytest.mark.parametrize("npx", [1, 3, 5, 1023, 70_001])
+def test_normalize_unaligned_and_odd_sizes(ctx, orc, npx):
+    """The vectorised min/max and u8 kernels against the oracle's normaliser on
+    identical encoded inputs (bit-exact), for buffers that are 16-byte aligned
+    and for encoded buffers 8 bytes / outputs 1 byte off alignment."""
+    rng = np.random.default_rng(npx)
+    encs = [rng.normal(size=npx) * 3 + 1 for _ in range(2)]
+    want = orc.normalize(encs)
+    for e_off, u_off in ((0, 0), (8, 1), (0, 3)):
+        bufs = [ctx.alloc(8 * npx + 16) for _ in encs] + [ctx.alloc(npx + 16) for _ in encs]
+        try:
+            de = [_View(b, e_off) for b in bufs[:2]]
+            du = [_View(b, u_off) for b in bufs[2:]]
+            for d, e in zip(de, encs):
+                capi.lib().ohx_memcpy_h2d(ctx.h, d.ptr, e.ctypes.data, 8 * npx)
+            ctx.normalize_u8(de, npx, du)
+            ctx.sync()
+            for d, w in zip(du, want):
+                got = np.empty(npx, np.uint8)
+                capi.lib().ohx_memcpy_d2h(ctx.h, got.ctypes.data, d.ptr, npx)
+                assert np.array_equal(got, w), (npx, e_off, u_off)
+        finally:
+            for b in bufs:
+                b.free()
+
+
 def test_end_to_end_u8_phase_image(ctx, orc):
     """generateHologram(RS) -> encoding(PHASE) -> normalize() against the
     reference restatement at the 10k-point RS workload's geometry (first 64

@@ -29,10 +29,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 # FP32 FLOPs per (point, pixel) pair executed by the direct kernels' fast
-# path (FMA = 2, every other VALU op incl. v_sin/v_cos/v_fract = 1):
-#   RS:      add, 2 fma (phase series), fract, sin, cos, fma (amplitude), 2 fma (accumulate) = 14
-#   Fresnel: add, fract, sin, cos, 2 fma (accumulate)                                      = 8
-FLOP_PER_PAIR = {0: 14, 1: 8}
+# path (FMA = 2, every other VALU op incl. v_sin/v_cos = 1):
+#   RS:      add, 2 fma (phase series), sin, cos, fma (amplitude), 2 fma (accumulate) = 13
+#   Fresnel: add, sin, cos, 2 fma (accumulate)                                      = 7
+FLOP_PER_PAIR = {0: 13, 1: 7}
 PEAK_FP32_TFLOPS = 157.3          # published MI355X FP32 vector peak
 PEAK_BF16_TFLOPS = 2500.0         # published dense BF16/FP16 matrix peak
 MODE_GPU, MODE_FLOAT = 1, 2
@@ -95,7 +95,7 @@ def cpu_baseline(w, target_s: float):
 TRAFFIC = {0: 1.060e9, 1: 1.067e9}
 # Speed of light of the direct kernels' per-pair instruction mix on this
 # hardware (openholo_amd/bin/valu_rates); profiles/valu_mfma_rates_gfx950.md.
-MIX_SOL_PAIRS_PER_S = {0: 3.63e12, 1: 4.88e12}
+MIX_SOL_PAIRS_PER_S = {0: 3.99e12, 1: 5.48e12}
 
 
 def fresnel_variants(ctx, capi, synth, Event, launches=2):

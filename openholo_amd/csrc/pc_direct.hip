@@ -135,7 +135,9 @@ __global__ __launch_bounds__(256) void k_pc_prep(const double* __restrict__ vtx,
 }
 
 // ---------------------------------------------------------------------------
-__device__ inline float fract32(float x) { return __builtin_amdgcn_fractf(x); }
+// v_sin_f32 / v_cos_f32 take revolutions and reduce them themselves: on
+// gfx950 sin(x) and sin(fract(x)) agree to 1 ulp for |x| <= 256, so the FP32
+// phases (below MAX_CYC + 1 cycles by the classification) go in unreduced.
 __device__ inline float sin_rev(float x) { return __builtin_amdgcn_sinf(x); }  // sin(2 pi x)
 __device__ inline float cos_rev(float x) { return __builtin_amdgcn_cosf(x); }  // cos(2 pi x)
 
@@ -342,7 +344,7 @@ __device__ void direct_tile(DirectSmem& sm, const Geometry& g, const PointSoA& s
                 const float bse = fmaf(fmaf(c.z, u, c.y), u, c.x);
 #pragma unroll
                 for (int k = 0; k < RPT; ++k) {
-                    const float f = fract32(bse + Rv[k]);
+                    const float f = bse + Rv[k];
                     pr[k] = fmaf(c.w, sin_rev(f), pr[k]);
                     pim[k] = fmaf(c.w, cos_rev(f), pim[k]);
                 }
@@ -352,7 +354,7 @@ __device__ void direct_tile(DirectSmem& sm, const Geometry& g, const PointSoA& s
 #pragma unroll
                 for (int k = 0; k < RPT; ++k) {
                     const float N = nx + Rv[k];
-                    const float f = fract32(fmaf(fmaf(c.z, N, c.w), N, c.x));
+                    const float f = fmaf(fmaf(c.z, N, c.w), N, c.x);
                     const float amp = fmaf(d.x, N, d.y);
                     pr[k] = fmaf(amp, sin_rev(f), pr[k]);
                     pim[k] = fmaf(amp, cos_rev(f), pim[k]);
@@ -379,11 +381,11 @@ __device__ void direct_tile(DirectSmem& sm, const Geometry& g, const PointSoA& s
                         in = in && rs_range_ok(qpx, qpy, zz, g.hx_coef, g.hy_coef, xxx, pix_y(g, row));
                     float f, amp;
                     if (DIFF == OHX_DIFF_FRESNEL) {
-                        f = fract32(fmaf(fmaf(c.z, u, c.y), u, c.x) + Rs[k]);
+                        f = fmaf(fmaf(c.z, u, c.y), u, c.x) + Rs[k];
                         amp = c.w;
                     } else {
                         const float N = fmaf(c.y, u, uu) + Rs[k];
-                        f = fract32(fmaf(fmaf(c.z, N, c.w), N, c.x));
+                        f = fmaf(fmaf(c.z, N, c.w), N, c.x);
                         amp = fmaf(d.x, N, d.y);
                     }
                     amp = in ? amp : 0.f;

@@ -37,11 +37,15 @@ namespace gemm {
 
 constexpr int BM = 128;           // tile rows (y)
 constexpr int BN = 128;           // tile columns (x), complex; wave w owns 32 of them
-constexpr int NT = 256;           // 4 waves
+constexpr int NT = 256;           // 4 waves; two workgroups per CU (80 KB LDS each)
 constexpr int KP = 16;            // points per stage
 constexpr int KS = 2 * KP;        // realified K per stage (32)
 constexpr int LDK = KS + 8;       // padded row length (bf16/f16 elements): 80 B rows
 constexpr int ROWW = LDK / 2;     // row length in dwords (20)
+constexpr int NGRP = NT / KP;     // thread groups, one point each (32)
+constexpr int NVT = BM / NGRP;    // V rows per thread and stage (4): grp + NGRP i
+constexpr int NUT = BN / NGRP;    // U columns per thread and stage (8): grp + NGRP j
+constexpr int NE = NVT + NUT;     // elements per thread and stage
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
@@ -135,10 +139,11 @@ __device__ inline void load_raw(Raw& r, const PointSoA& s, int64_t pi, int64_t p
     r.box = s.box[j];
 }
 
-// Per-(point, thread) constants of one stage.  Thread t generates V rows and
-// U columns grp + 16 i (grp = t / 16, i < 8); with k = i - 4 the phase (cycles)
-// of its k-th row is exactly  v0 + k v1 + k^2 v2  (mod 1), a quadratic in the
-// integer k, and likewise u0 + k u1 + k^2 u2 for its columns.  The three
+// Per-(point, thread) constants of one stage.  Thread t generates V rows
+// grp + NGRP i (i < NVT) and U columns grp + NGRP j (j < NUT), grp = t / 16;
+// with k = i - NVT/2 the phase (cycles) of its k-th row is exactly
+// v0 + k v1 + k^2 v2  (mod 1), a quadratic in the integer k, and likewise
+// u0 + k u1 + k^2 u2 (k = j - NUT/2) for its columns.  The three
 // coefficients are computed in FP64 and reduced mod 1 before rounding to FP32
 // (for integer k, k * frac(c) = k * c mod 1), so the FP32 evaluation only
 // ever sums terms below 1, 4 and 16 cycles, whatever the phase excursion.
@@ -150,8 +155,9 @@ struct Pt {
 // x - floor(x) in one v_fract_f64 (clamped below 1.0)
 __device__ inline double frac64(double x) { return __builtin_amdgcn_fract(x); }
 
-// ythr / xthr: y of row grp + 64 and x of column grp + 64 of the tile;
-// dy = -16 pp_y and dx = 16 pp_x (row index grows toward -y).
+// ythr / xthr: y of row grp + NGRP*(NVT/2) and x of column grp + NGRP*(NUT/2)
+// of the tile (the reference element k = 0); dy = -NGRP pp_y and
+// dx = NGRP pp_x (row index grows toward -y).
 __device__ inline void prep_y(Pt& P, const Raw& r, double ythr, double dy)
 {
     const double ey = ythr - r.py, ty = r.q * ey;
@@ -184,15 +190,15 @@ __device__ inline bool prep(Pt& P, const Raw& r, double ythr, double xthr, doubl
     return prep_rest(P, r, x0, y0, vscale);
 }
 
-// phase (cycles, |phase| < 21) of the element k = i - 4 from its coefficients
-// (i is a compile-time constant wherever this is called).  No mod-1 reduction:
+// phase (cycles, |phase| < 1 + |k| + k^2 <= 21) of element k from its
+// coefficients (k is a compile-time constant wherever this is called).  No mod-1 reduction:
 // v_sin_f32 / v_cos_f32 reduce a revolutions argument themselves (on gfx950
 // sin(x) and sin(fract(x)) agree to 1 ulp for |x| <= 256), and the rounding
 // of the FMAs above is the same with or without it.
-__device__ inline float phase_at(int i, float c0, float c1, float c2)
+__device__ inline float phase_at(int k, float c0, float c1, float c2)
 {
-    const float k = (float)(i - 4);
-    return i == 4 ? c0 : fmaf(fmaf(c2, k, c1), k, c0);
+    const float kf = (float)k;
+    return k == 0 ? c0 : fmaf(fmaf(c2, kf, c1), kf, c0);
 }
 
 // Stores one element pair (hi and, 3-term, lo parts) of V or U.
@@ -205,26 +211,47 @@ __device__ inline void put(unsigned* hi_slot, unsigned* lo_slot, float re, float
     if constexpr (Traits<VAR>::three) *lo_slot = pack2<FP16>(re - lo16<FP16>(h), im - hi16<FP16>(h));
 }
 
-// LDS plane layout: plane h (hi / lo part) of one buffer's V or U block is
-// BM (= BN) rows of ROWW dwords; element (row, p) at h*PLANE + row*ROWW + p.
-constexpr int PLANE = BM * ROWW;
+// LDS plane layout: plane h (hi / lo part) of one buffer's V block is BM rows
+// of ROWW dwords (U block: BN rows); element (row, p) at h*PLANE + row*ROWW + p.
+constexpr int PLANE_V = BM * ROWW;
+constexpr int PLANE_U = BN * ROWW;
 
-template <int VAR, int I>
+// A thread's NE elements per stage, V rows spread evenly among U columns:
+// element E is a V row when it raises the running V count.
+constexpr int v_before(int E) { return (E * NVT) / NE; }
+constexpr bool elem_is_v(int E) { return v_before(E + 1) > v_before(E); }
+constexpr int elem_index(int E) { return elem_is_v(E) ? v_before(E) : E - v_before(E); }
+
+// element E of the thread's stage: phase, sincos (V: times the amplitude)
+template <int E>
+__device__ inline float2 elem_value(const Pt& P)
+{
+    if constexpr (elem_is_v(E)) {
+        const float f = phase_at(elem_index(E) - NVT / 2, P.v0, P.v1, P.v2);
+        return make_float2(P.amp * __builtin_amdgcn_cosf(f), P.amp * __builtin_amdgcn_sinf(f));
+    } else {
+        const float f = phase_at(elem_index(E) - NUT / 2, P.u0, P.u1, P.u2);
+        return make_float2(__builtin_amdgcn_cosf(f), __builtin_amdgcn_sinf(f));
+    }
+}
+template <int VAR, int E>
+__device__ inline void elem_store(unsigned* wv, unsigned* wu, float2 cs, int p, int grp)
+{
+    const int e = (grp + NGRP * elem_index(E)) * ROWW + p;
+    if constexpr (elem_is_v(E)) put<VAR>(wv + e, wv + (Traits<VAR>::three ? PLANE_V : 0) + e, cs.x, cs.y);
+    else put<VAR>(wu + e, wu + (Traits<VAR>::three ? PLANE_U : 0) + e, cs.x, cs.y);
+}
+
+template <int VAR, int E>
 __device__ inline void gen_elems(unsigned* wv, unsigned* wu, const Pt& P, int p, int grp)
 {
-    if constexpr (I < BM / 16) {
-        constexpr int H2 = Traits<VAR>::three ? PLANE : 0;
-        const int e = (grp + 16 * I) * ROWW + p;
-        const float fv = phase_at(I, P.v0, P.v1, P.v2);
-        put<VAR>(wv + e, wv + H2 + e, P.amp * __builtin_amdgcn_cosf(fv), P.amp * __builtin_amdgcn_sinf(fv));
-        const float fu = phase_at(I, P.u0, P.u1, P.u2);
-        put<VAR>(wu + e, wu + H2 + e, __builtin_amdgcn_cosf(fu), __builtin_amdgcn_sinf(fu));
-        gen_elems<VAR, I + 1>(wv, wu, P, p, grp);
+    if constexpr (E < NE) {
+        elem_store<VAR, E>(wv, wu, elem_value<E>(P), p, grp);
+        gen_elems<VAR, E + 1>(wv, wu, P, p, grp);
     }
 }
 
-// Generation of one stage into buffer WB, ignoring the boxes: thread t
-// handles point p = t % 16, V rows and U columns grp + 16 i (i < 8).
+// Generation of one stage into buffer WB, ignoring the boxes.
 template <int VAR, int WB>
 __device__ inline void gen_stage(Smem<Traits<VAR>::three>& sm, const Pt& P, int p, int grp)
 {
@@ -239,15 +266,19 @@ __device__ inline void mask_stage(Smem<Traits<VAR>::three>& sm, const Pt& P, int
 {
     constexpr int H2 = Traits<VAR>::three ? 1 : 0;
 #pragma unroll
-    for (int i = 0; i < BM / 16; ++i) {
-        const int r = grp + 16 * i;
+    for (int i = 0; i < NVT; ++i) {
+        const int r = grp + NGRP * i;
         if (r < P.rlo || r >= P.rhi) {
             sm.v[WB][0][r][p] = 0u;
             sm.v[WB][H2][r][p] = 0u;
         }
-        if (r < P.clo || r >= P.chi) {     // column index grp + 16 i
-            sm.u[WB][0][r][p] = 0u;
-            sm.u[WB][H2][r][p] = 0u;
+    }
+#pragma unroll
+    for (int j = 0; j < NUT; ++j) {
+        const int c = grp + NGRP * j;
+        if (c < P.clo || c >= P.chi) {
+            sm.u[WB][0][c][p] = 0u;
+            sm.u[WB][H2][c][p] = 0u;
         }
     }
 }
@@ -276,7 +307,7 @@ struct StageCtx {
     Pt P;
     bool partial;
     u32x4 A[8][2], Ub[2][2], B[2][4];
-    float2 cs[16];
+    float2 cs[NE];
 };
 
 // MFMA order per (ks, mb) group: 3-term (ah,Re) (ah,Im) (ah,lo Re) (ah,lo Im)
@@ -287,14 +318,14 @@ __device__ inline void load_a(const unsigned* rv, StageCtx<VAR>& c, int g)
 {
     const unsigned* a = rv + ((g & 3) * 32 + c.lrow) * ROWW + (g >> 2) * 8 + c.koff;
     c.A[g][0] = *reinterpret_cast<const u32x4*>(a);
-    if constexpr (Traits<VAR>::three) c.A[g][1] = *reinterpret_cast<const u32x4*>(a + PLANE);
+    if constexpr (Traits<VAR>::three) c.A[g][1] = *reinterpret_cast<const u32x4*>(a + PLANE_V);
 }
 template <int VAR>
 __device__ inline void load_b(const unsigned* ru, StageCtx<VAR>& c, int ks)
 {
     const unsigned* b = ru + c.ucol * ROWW + ks * 8 + c.koff;
     c.Ub[ks][0] = *reinterpret_cast<const u32x4*>(b);
-    if constexpr (Traits<VAR>::three) c.Ub[ks][1] = *reinterpret_cast<const u32x4*>(b + PLANE);
+    if constexpr (Traits<VAR>::three) c.Ub[ks][1] = *reinterpret_cast<const u32x4*>(b + PLANE_U);
 }
 template <int VAR>
 __device__ inline void xform_b(StageCtx<VAR>& c, int ks, int h)
@@ -302,27 +333,40 @@ __device__ inline void xform_b(StageCtx<VAR>& c, int ks, int h)
     c.B[ks][2 * h] = to_re(c.Ub[ks][h]);
     c.B[ks][2 * h + 1] = to_im(c.Ub[ks][h]);
 }
-// element E: V row (E even) or U column (E odd) grp + 16 (E / 2); first the
-// phase and sincos (V: times the amplitude), then packing and LDS stores
-template <int VAR, int E>
-__device__ inline void elem_a(StageCtx<VAR>& c)
+// The generation schedule: 3-term, half-element h (even: value of element
+// h/2, odd: its store) at slot 3 + 44 h / (2 NE); 1-term, element E at slot
+// 2 + 14 E / NE (two elements share a slot when NE > 14).  Return the
+// half-element / element at slot S, or -1.
+constexpr int half_at(int S)
 {
-    constexpr int i = E >> 1;
-    if constexpr (E & 1) {
-        const float f = phase_at(i, c.P.u0, c.P.u1, c.P.u2);
-        c.cs[E] = make_float2(__builtin_amdgcn_cosf(f), __builtin_amdgcn_sinf(f));
-    } else {
-        const float f = phase_at(i, c.P.v0, c.P.v1, c.P.v2);
-        c.cs[E] = make_float2(c.P.amp * __builtin_amdgcn_cosf(f), c.P.amp * __builtin_amdgcn_sinf(f));
+    for (int h = 0; h < 2 * NE; ++h)
+        if (3 + (44 * h) / (2 * NE) == S) return h;
+    return -1;
+}
+constexpr int elem_at(int S, int n)   // n-th element at slot S (slots hold at most two)
+{
+    for (int e = 0; e < NE; ++e)
+        if (2 + (14 * e) / NE == S && n-- == 0) return e;
+    return -1;
+}
+// every element (1-term) and half-element (3-term) lands in exactly one slot
+constexpr bool schedule_complete()
+{
+    for (int e = 0; e < NE; ++e) {
+        int n = 0;
+        for (int S = 0; S < 16; ++S) n += (elem_at(S, 0) == e) + (elem_at(S, 1) == e) + (elem_at(S, 2) == e);
+        if (n != 1) return false;
     }
+    for (int S = 0; S < 16; ++S)
+        if (elem_at(S, 2) >= 0) return false;
+    for (int h = 0; h < 2 * NE; ++h) {
+        int n = 0;
+        for (int S = 0; S < 48; ++S) n += half_at(S) == h;
+        if (n != 1) return false;
+    }
+    return true;
 }
-template <int VAR, int E>
-__device__ inline void elem_b(unsigned* w, StageCtx<VAR>& c)
-{
-    constexpr int H2 = Traits<VAR>::three ? PLANE : 0;
-    const int e = (c.grp + 16 * (E >> 1)) * ROWW + c.p;
-    put<VAR>(w + e, w + H2 + e, c.cs[E].x, c.cs[E].y);
-}
+static_assert(schedule_complete(), "generation schedule must cover every element once");
 
 // Slot S of the pipelined stage: MFMA S, operand loads / transforms due at
 // this slot, and this slot's share of the next stage's generation.
@@ -348,23 +392,13 @@ __device__ inline void slot(const unsigned* __restrict__ rv, const unsigned* __r
             load_raw(c.R, c.s, c.pnext, c.ke);
         }
         if constexpr (THREE) {
-            // half-element k (k even: first half of element k/2) at slot 3 + (11 k) / 8
-            constexpr int k0 = ((S - 3) * 8 + 10) / 11;       // smallest k with 3 + 11k/8 >= S
-            if constexpr (S >= 3 && k0 < 32 && 3 + (11 * k0) / 8 == S) {
-                if constexpr (k0 & 1) elem_b<VAR, (k0 >> 1)>(((k0 >> 1) & 1) ? wu : wv, c);
-                else elem_a<VAR, (k0 >> 1)>(c);
-            }
+            constexpr int h = half_at(S);
+            if constexpr (h >= 0 && (h & 1) == 0) c.cs[h >> 1] = elem_value<(h >> 1)>(c.P);
+            if constexpr (h >= 0 && (h & 1) == 1) elem_store<VAR, (h >> 1)>(wv, wu, c.cs[h >> 1], c.p, c.grp);
         } else {
-            // element E at slot 2 + (7 E) / 8 (two elements in some slots)
-            constexpr int e0 = ((S - 2) * 8 + 6) / 7;         // smallest E with 2 + 7E/8 >= S
-            if constexpr (S >= 2 && e0 < 16 && 2 + (7 * e0) / 8 == S) {
-                elem_a<VAR, e0>(c);
-                elem_b<VAR, e0>((e0 & 1) ? wu : wv, c);
-            }
-            if constexpr (S >= 2 && e0 + 1 < 16 && 2 + (7 * (e0 + 1)) / 8 == S) {
-                elem_a<VAR, e0 + 1>(c);
-                elem_b<VAR, e0 + 1>(((e0 + 1) & 1) ? wu : wv, c);
-            }
+            constexpr int e0 = elem_at(S, 0), e1 = elem_at(S, 1);
+            if constexpr (e0 >= 0) elem_store<VAR, e0>(wv, wu, elem_value<e0>(c.P), c.p, c.grp);
+            if constexpr (e1 >= 0) elem_store<VAR, e1>(wv, wu, elem_value<e1>(c.P), c.p, c.grp);
         }
         __builtin_amdgcn_sched_barrier(0);
         slot<VAR, S + 1>(rv, ru, wv, wu, c);
@@ -435,8 +469,8 @@ __global__ __launch_bounds__(NT, 2) void k_fresnel_gemm(Geometry g, PointSoA s, 
     const float am = amax[0];
     const float vscale = (Traits<VAR>::fp16 && am > 0.f) ? 1.0f / am : 1.0f;
     const int p = threadIdx.x & (KP - 1), grp = threadIdx.x >> 4;
-    const double ythr = gpix_y(g, y0 + grp + 64), xthr = gpix_x(g, x0 + grp + 64);
-    const double dy = -16.0 * g.pp_y, dx = 16.0 * g.pp_x;
+    const double ythr = gpix_y(g, y0 + grp + NGRP * (NVT / 2)), xthr = gpix_x(g, x0 + grp + NGRP * (NUT / 2));
+    const double dy = -NGRP * g.pp_y, dx = NGRP * g.pp_x;
 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int lrow = lane & 31, koff = (lane >> 5) * 4;   // dword offset of this lane's 8 values

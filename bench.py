@@ -92,10 +92,10 @@ def cpu_baseline(w, target_s: float):
 # HBM bytes per launch of the direct kernels at the default workload
 # (rocprofv3 --pmc FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the scale factors read
 # off a 1 GiB copy kernel); profiles/k_pc_direct_rs_fresnel_100k_2160p.md.
-TRAFFIC = {0: 1.060e9, 1: 1.067e9}
+TRAFFIC = {0: 1.059e9, 1: 1.065e9}
 # Speed of light of the direct kernels' per-pair instruction mix on this
 # hardware (openholo_amd/bin/valu_rates); profiles/valu_mfma_rates_gfx950.md.
-MIX_SOL_PAIRS_PER_S = {0: 3.99e12, 1: 5.48e12}
+MIX_SOL_PAIRS_PER_S = {0: 3.97e12, 1: 5.50e12}
 
 
 def fresnel_variants(ctx, capi, synth, Event, launches=2):
@@ -301,7 +301,7 @@ def main():
                     "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": traffic,
                     "kernel": f"k_pc_direct<{'RS' if dom == 0 else 'Fresnel'},FP32>",
                     "avg_launch_ms": round(k_ms, 3),
-                    "model": f"{fl} FP32 FLOP/pair executed (FMA=2, sin/cos/fract/add=1); published peak "
+                    "model": f"{fl} FP32 FLOP/pair executed (FMA=2, sin/cos/add=1); published peak "
                              f"counts packed FP32, which is not emitted",
                     "mix_sol_pairs_per_s": MIX_SOL_PAIRS_PER_S[dom],
                     "frac_of_mix_sol": round(pps / MIX_SOL_PAIRS_PER_S[dom], 4)}
@@ -338,6 +338,33 @@ def main():
             den += float(np.sum(np.abs(ref) ** 2))
         rel["RS" if diff == 0 else "Fresnel"] = float(np.sqrt(num / den))
 
+    # PCIe-inclusive rate of the drop-in: ophPointCloud::generateHologram takes the
+    # vertices from the host and hands every channel's FP64 field back in pageable
+    # host buffers; one step timed end to end on the wall clock (never the value).
+    host_fields = [np.empty(npx, np.complex128) for _ in range(nch)]
+    lib = capi.lib()
+    ctx.sync()
+    t0 = time.perf_counter()
+    dv_h = ctx.to_device(pts)
+    for diff in w.modes:
+        for ch in range(nch):
+            ctx.field_zero(fields[ch], npx)
+            ctx.diffract(dv_h, n, fields[ch], params[ch], diff, mode_of[diff])
+            st = lib.ohx_memcpy_d2h(ctx.h, host_fields[ch].ctypes.data, fields[ch].ptr, 16 * npx)
+            assert st == capi.OK
+    ctx.sync()
+    host_ms = (time.perf_counter() - t0) * 1e3
+    dv_h.free()
+    t0 = time.perf_counter()
+    assert lib.ohx_memcpy_d2h(ctx.h, host_fields[0].ctypes.data, fields[0].ptr, 16 * npx) == capi.OK
+    d2h_s = time.perf_counter() - t0
+    handover = {"ms_per_step": round(host_ms, 2), "Gpairs_per_s": round(pairs_step / (host_ms / 1e3) / 1e9, 1),
+                "field_d2h_GB_per_s": round(16 * npx / d2h_s / 1e9, 2),
+                "what": "vertex upload + per channel and mode: zero, diffract, FP64 field to a pageable "
+                        "host buffer (the drop-in's generateHologram); wall clock"}
+    print(f"# host hand-over: {host_ms:.1f} ms/step, field D2H {handover['field_d2h_GB_per_s']} GB/s",
+          file=sys.stderr, flush=True)
+
     for b in [dv] + fields + encs + u8s:
         b.free()
     var = fresnel_variants(ctx, capi, synth, Event) if args.variants else None
@@ -370,6 +397,7 @@ def main():
         "p99_ms_per_hologram": round(p99, 3) if p99 else None,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "host_handover": handover,
     }
     if var:
         line["variants"] = var

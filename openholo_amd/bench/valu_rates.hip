@@ -4,6 +4,7 @@
 // Build: make -C openholo_amd bench   Run: openholo_amd/bin/valu_rates
 #include <hip/hip_runtime.h>
 #include <cstdio>
+#include <cstring>
 #include <vector>
 #define CK(x) do { hipError_t e = (x); if (e != hipSuccess) { printf("err %s line %d\n", hipGetErrorString(e), __LINE__); return 1; } } while (0)
 constexpr int CH = 16;
@@ -36,6 +37,20 @@ __global__ __launch_bounds__(256) void k(float* out, int iters, float b, float c
   }
   float s = 0; for (int i = 0; i < CH; ++i) s += a[i] + r[i];
   out[blockIdx.x * 256 + threadIdx.x] = s;
+}
+// v_sin / v_cos on an unreduced revolutions argument against the same after
+// v_fract: counts results that differ and the largest difference, for hashed
+// arguments uniform in [-lim, lim].
+__global__ void ksinrange(unsigned* cnt, unsigned* maxd, float lim) {
+  const unsigned i = blockIdx.x * 256 + threadIdx.x;
+  unsigned h = i * 2654435761u; h ^= h >> 15; h *= 2246822519u; h ^= h >> 13;
+  const float x = ((float)(h >> 8) * (1.0f / 16777216.0f) * 2.0f - 1.0f) * lim;
+  const float f = __builtin_amdgcn_fractf(x);
+  const float s1 = __builtin_amdgcn_sinf(x), s2 = __builtin_amdgcn_sinf(f);
+  const float c1 = __builtin_amdgcn_cosf(x), c2 = __builtin_amdgcn_cosf(f);
+  const float d = fmaxf(fabsf(s1 - s2), fabsf(c1 - c2));
+  if (s1 != s2 || c1 != c2) atomicAdd(cnt, 1u);
+  if (d > 0.f) atomicMax(maxd, __float_as_uint(d));
 }
 __global__ void kcopy(const float4* __restrict__ a, float4* __restrict__ b, size_t n) {
   size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x, st = (size_t)gridDim.x * blockDim.x;
@@ -85,5 +100,17 @@ int main() {
   }
   hipDeviceProp_t p; CK(hipGetDeviceProperties(&p, 0));
   printf("device %s CUs %d clock %d kHz\n", p.gcnArchName, p.multiProcessorCount, p.clockRate);
+  // unreduced sin/cos arguments (the direct and matrix-core kernels rely on |x| <= 256)
+  unsigned* sr; CK(hipMalloc(&sr, 8));
+  const float lims[] = {1.0f, 24.0f, 64.0f, 256.0f};
+  for (float lim : lims) {
+    CK(hipMemset(sr, 0, 8));
+    hipLaunchKernelGGL(ksinrange, dim3(65536), dim3(256), 0, 0, sr, sr + 1, lim);
+    CK(hipGetLastError()); CK(hipDeviceSynchronize());
+    unsigned h[2]; CK(hipMemcpy(h, sr, 8, hipMemcpyDeviceToHost));
+    float md; memcpy(&md, &h[1], 4);
+    printf("sin/cos(x) vs sin/cos(fract(x)), |x| <= %5.0f: %u of %u differ, max |diff| %.3e\n", lim, h[0],
+           65536u * 256u, md);
+  }
   return 0;
 }

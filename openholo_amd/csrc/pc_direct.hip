@@ -176,8 +176,11 @@ struct DirectSmem {
 };
 
 // Accumulates points [p_begin, p_end) of s into one 64 x 32 tile of field.
+// Always inlined: the multi-job kernel calls all four variants, and a real
+// call would pin that kernel at the callees' worst-case register count
+// (one wave per SIMD) and add call overhead per tile.
 template <int DIFF, bool FP64>
-__device__ void direct_tile(DirectSmem& sm, const Geometry& g, const PointSoA& s, int64_t p_begin,
+__device__ __attribute__((always_inline)) inline void direct_tile(DirectSmem& sm, const Geometry& g, const PointSoA& s, int64_t p_begin,
                            int64_t p_end, double2* __restrict__ field, int tile, int tiles_x)
 {
     float4* s_c0 = sm.c0;
@@ -431,9 +434,10 @@ __device__ void direct_tile(DirectSmem& sm, const Geometry& g, const PointSoA& s
     }
 }
 
+// Occupancy floor: the FP32 Fresnel loop fits three waves per SIMD, the others two.
 template <int DIFF, bool FP64>
-__global__ __launch_bounds__(NT) void k_pc_direct(Geometry g, PointSoA s, double2* __restrict__ field,
-                                                  int tiles_x)
+__global__ __launch_bounds__(NT, (DIFF == OHX_DIFF_FRESNEL && !FP64) ? 3 : 2) void k_pc_direct(
+    Geometry g, PointSoA s, double2* __restrict__ field, int tiles_x)
 {
     __shared__ DirectSmem sm;
     direct_tile<DIFF, FP64>(sm, g, s, 0, s.n, field, blockIdx.x, tiles_x);

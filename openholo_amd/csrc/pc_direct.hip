@@ -301,6 +301,9 @@ __device__ __attribute__((always_inline)) inline void direct_tile(DirectSmem& sm
             nF += s_cnt[0][w];
             nS += s_cnt[1][w];
         }
+        // the list lengths are workgroup-uniform: scalar loop control below
+        nF = __builtin_amdgcn_readfirstlane(nF);
+        nS = __builtin_amdgcn_readfirstlane(nS);
         if (cls != CLS_OUT) {
             const unsigned long long m = (cls == CLS_FAST) ? bf : bs;
             const int rank = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),

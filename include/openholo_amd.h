@@ -147,6 +147,12 @@ typedef struct ohx_launch_stats {
     int64_t workgroups;      /* workgroups of the diffraction kernel          */
     int32_t kernel_kind;     /* 0 FP32 direct, 1 FP64 direct, 2 MFMA GEMM     */
     int32_t reserved;
+    int64_t pairs_in_box;    /* point-pixel pairs inside the points' anti-
+                                aliasing boxes (ophGen.cpp:384-407,467-490):
+                                the pairs the kernels evaluate; the RS range
+                                test may drop a few more.  Read back from the
+                                device when this function is called, which
+                                synchronises ctx's stream.                   */
 } ohx_launch_stats;
 int ohx_last_launch_stats(ohx_ctx* ctx, ohx_launch_stats* out);
 

@@ -47,7 +47,8 @@ class PcParams(ctypes.Structure):
 
 class LaunchStats(ctypes.Structure):
     _fields_ = [("pairs_nominal", ctypes.c_int64), ("workgroups", ctypes.c_int64),
-                ("kernel_kind", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+                ("kernel_kind", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("pairs_in_box", ctypes.c_int64)]
 
 
 class ServerOpts(ctypes.Structure):

@@ -187,7 +187,7 @@ int ohx_ctx_create(int device, ohx_ctx** out)
     c->device = device;
     if (hipSetDevice(device) != hipSuccess ||
         hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipMalloc(&c->minmax, 4 * sizeof(unsigned long long)) != hipSuccess) {
+        hipMalloc(&c->minmax, 8 * sizeof(unsigned long long)) != hipSuccess) {
         if (c->stream) (void)hipStreamDestroy(c->stream);
         delete c;
         return OHX_ERR_DEVICE;
@@ -292,10 +292,12 @@ int ohx_pc_diffract(ohx_ctx* c, const void* d_vertices, int64_t n, double* d_fie
     (void)hipSetDevice(c->device);
     c->last = ohx_launch_stats{};
     c->last.pairs_nominal = n * (int64_t)g.pn_x * g.pn_y;
+    c->boxsum_pending = false;
     if (n == 0) return OHX_OK;  // setPointCloudModel with n < 1 is a no-op upstream
     PointSoA soa;
     if ((st = ensure_points(c, n, &soa))) return st;
-    if ((st = launch_prep(c, d_vertices, g, soa))) return st;
+    if ((st = launch_prep(c, d_vertices, g, soa, c->minmax + 4))) return st;
+    c->boxsum_pending = true;
     const uint32_t mfma = mode & (OHX_MODE_MFMA_BF16X1 | OHX_MODE_MFMA_BF16X3 | OHX_MODE_MFMA_FP16X3);
     if (mfma && diff == OHX_DIFF_FRESNEL) return launch_fresnel_gemm(c, g, soa, d_field, mode);
     const bool fp64 = (mode & (OHX_MODE_FLOAT | OHX_MODE_FASTMATH)) == 0 && mfma == 0;
@@ -324,6 +326,17 @@ int ohx_pc_probe_f64(ohx_ctx* c, const void* d_vertices, int64_t n, const ohx_pc
 int ohx_last_launch_stats(ohx_ctx* c, ohx_launch_stats* out)
 {
     if (!c || !out) return OHX_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (c->boxsum_pending) {   // the box-pixel count is read back on request only
+        (void)hipSetDevice(c->device);
+        unsigned long long v = 0;
+        int st = check_hip(c, hipMemcpyAsync(&v, c->minmax + 4, sizeof(v), hipMemcpyDeviceToHost, c->stream),
+                           "hipMemcpyAsync");
+        if (!st) st = check_hip(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+        if (st) return st;
+        c->last.pairs_in_box = (int64_t)v;
+        c->boxsum_pending = false;
+    }
     *out = c->last;
     return OHX_OK;
 }

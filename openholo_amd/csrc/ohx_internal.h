@@ -23,6 +23,7 @@ struct ohx_ctx {
     void* gemm_scratch = nullptr;
     size_t gemm_scratch_bytes = 0;
     ohx_launch_stats last{};
+    bool boxsum_pending = false;   // last.pairs_in_box still on the device (minmax[4])
 };
 
 namespace ohx {
@@ -73,7 +74,8 @@ void carve_points(void* base, int64_t n, PointSoA* soa);
 int ensure_gemm_scratch(ohx_ctx* ctx, size_t bytes);
 
 // launchers (pc_direct.hip)
-int launch_prep(ohx_ctx* ctx, const void* d_vertices, const Geometry& g, const PointSoA& soa);
+int launch_prep(ohx_ctx* ctx, const void* d_vertices, const Geometry& g, const PointSoA& soa,
+                unsigned long long* boxsum = nullptr);
 int launch_direct(ohx_ctx* ctx, const Geometry& g, const PointSoA& soa, double* d_field, bool fp64);
 int launch_fresnel_gemm(ohx_ctx* ctx, const Geometry& g, const PointSoA& soa, double* d_field, uint32_t mode);
 int direct_tiles(const Geometry& g, int* tiles_x);

@@ -81,57 +81,75 @@ __device__ inline bool rs_range_ok(double px, double py, double zz, double hxc, 
 
 // ---------------------------------------------------------------------------
 // per-point constants: Vertex (56 B) -> SoA
-__global__ __launch_bounds__(256) void k_pc_prep(const double* __restrict__ vtx, Geometry g, PointSoA s)
+// boxsum (optional): adds the number of pixels inside the boxes of this
+// launch's points (one atomic per workgroup), for throughput accounting.
+__global__ __launch_bounds__(256) void k_pc_prep(const double* __restrict__ vtx, Geometry g, PointSoA s,
+                                                 unsigned long long* __restrict__ boxsum)
 {
 #pragma clang fp contract(off)
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= s.n) return;
-    const double* v = vtx + 7 * i;
-    // genCghPointCloudCPU: scale, then distance (ophPointCloud.cpp:304-306, ophGen.cpp:379)
-    const double px = v[0] * g.scale[0];
-    const double py = v[1] * g.scale[1];
-    const double z = v[2] * g.scale[2] + g.distance;
-    const double a = v[3 + g.channel];
-    double hx, hy;
-    if (g.diff == OHX_DIFF_RS) {
-        hx = fabs(g.hx_coef * z);
-        hy = fabs(g.hy_coef * z);
-    } else {
-        const double operand = g.lambda * z;
-        hx = fabs(operand / (2 * g.pp_x));
-        hy = fabs(operand / (2 * g.pp_y));
-    }
-    // box bounds, ophGen.cpp:394-407 (x = -ss/2 = g.x0)
-    double X0 = floor(((px + hx) - g.x0) / g.pp_x) + 1;       // exclusive upper column
-    double X1 = floor(((px - hx) - g.x0) / g.pp_x) + 1;       // inclusive lower column
-    double Y0 = g.pn_y - floor(((py - hy) - g.y0) / g.pp_y);  // exclusive upper row
-    double Y1 = g.pn_y - floor(((py + hy) - g.y0) / g.pp_y);  // inclusive lower row
-    if (X0 > g.pn_x) X0 = g.pn_x;
-    if (X1 < 0) X1 = 0;
-    if (Y0 > g.pn_y) Y0 = g.pn_y;
-    if (Y1 < 0) Y1 = 0;
-    int4 b = make_int4(0, 0, 0, 0);
-    if (X1 < X0 && Y1 < Y0) b = make_int4((int)X1, (int)X0, (int)Y1, (int)Y0);
+    unsigned long long area = 0;
+    if (i < s.n) {
+        const double* v = vtx + 7 * i;
+        // genCghPointCloudCPU: scale, then distance (ophPointCloud.cpp:304-306, ophGen.cpp:379)
+        const double px = v[0] * g.scale[0];
+        const double py = v[1] * g.scale[1];
+        const double z = v[2] * g.scale[2] + g.distance;
+        const double a = v[3 + g.channel];
+        double hx, hy;
+        if (g.diff == OHX_DIFF_RS) {
+            hx = fabs(g.hx_coef * z);
+            hy = fabs(g.hy_coef * z);
+        } else {
+            const double operand = g.lambda * z;
+            hx = fabs(operand / (2 * g.pp_x));
+            hy = fabs(operand / (2 * g.pp_y));
+        }
+        // box bounds, ophGen.cpp:394-407 (x = -ss/2 = g.x0)
+        double X0 = floor(((px + hx) - g.x0) / g.pp_x) + 1;       // exclusive upper column
+        double X1 = floor(((px - hx) - g.x0) / g.pp_x) + 1;       // inclusive lower column
+        double Y0 = g.pn_y - floor(((py - hy) - g.y0) / g.pp_y);  // exclusive upper row
+        double Y1 = g.pn_y - floor(((py + hy) - g.y0) / g.pp_y);  // inclusive lower row
+        if (X0 > g.pn_x) X0 = g.pn_x;
+        if (X1 < 0) X1 = 0;
+        if (Y0 > g.pn_y) Y0 = g.pn_y;
+        if (Y1 < 0) Y1 = 0;
+        int4 b = make_int4(0, 0, 0, 0);
+        if (X1 < X0 && Y1 < Y0) b = make_int4((int)X1, (int)X0, (int)Y1, (int)Y0);
 
-    // z / lambda mod 1 with the exact remainder of the division
-    const double qz = z / g.lambda;
-    const double rem = fma(-qz, g.lambda, z);
-    double c = (qz - floor(qz)) + rem * g.inv_lambda;
-    if (g.use_phase) c += v[6] * INV_2PI;
-    c -= floor(c);
+        // z / lambda mod 1 with the exact remainder of the division
+        const double qz = z / g.lambda;
+        const double rem = fma(-qz, g.lambda, z);
+        double c = (qz - floor(qz)) + rem * g.inv_lambda;
+        if (g.use_phase) c += v[6] * INV_2PI;
+        c -= floor(c);
 
-    s.px[i] = px;
-    s.py[i] = py;
-    s.pz[i] = z;
-    s.cyc0[i] = c;
-    if (g.diff == OHX_DIFF_RS) {
-        s.kamp[i] = a * z / g.lambda;
-        s.q[i] = 0.0;
-    } else {
-        s.kamp[i] = a / (g.lambda * z);
-        s.q[i] = 0.5 / (g.lambda * z);
+        s.px[i] = px;
+        s.py[i] = py;
+        s.pz[i] = z;
+        s.cyc0[i] = c;
+        if (g.diff == OHX_DIFF_RS) {
+            s.kamp[i] = a * z / g.lambda;
+            s.q[i] = 0.0;
+        } else {
+            s.kamp[i] = a / (g.lambda * z);
+            s.q[i] = 0.5 / (g.lambda * z);
+        }
+        s.box[i] = b;
+        area = (unsigned long long)(b.y - b.x) * (unsigned long long)(b.w - b.z);
     }
-    s.box[i] = b;
+    if (boxsum) {   // uniform over the launch
+        for (int off = 32; off > 0; off >>= 1) area += __shfl_xor(area, off);
+        __shared__ unsigned long long part[4];
+        if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = area;
+        __syncthreads();
+        if (threadIdx.x == 0) atomicAdd(boxsum, part[0] + part[1] + part[2] + part[3]);
+    }
+}
+
+__global__ void k_zero_u64(unsigned long long* p)
+{
+    if (threadIdx.x == 0) *p = 0ull;
 }
 
 // ---------------------------------------------------------------------------
@@ -531,11 +549,13 @@ __global__ __launch_bounds__(256) void k_pc_probe(Geometry g, PointSoA s, const 
 }
 
 // ---------------------------------------------------------------------------
-int launch_prep(ohx_ctx* ctx, const void* d_vertices, const Geometry& g, const PointSoA& soa)
+int launch_prep(ohx_ctx* ctx, const void* d_vertices, const Geometry& g, const PointSoA& soa,
+                unsigned long long* boxsum)
 {
     const unsigned grid = (unsigned)((soa.n + 255) / 256);
+    if (boxsum) hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, ctx->stream, boxsum);
     hipLaunchKernelGGL(k_pc_prep, dim3(grid), dim3(256), 0, ctx->stream,
-                       (const double*)d_vertices, g, soa);
+                       (const double*)d_vertices, g, soa, boxsum);
     return check_hip(ctx, hipGetLastError(), "k_pc_prep");
 }
 

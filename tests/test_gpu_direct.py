@@ -127,6 +127,30 @@ def test_complex_amplitude_extension(ctx, orc):
     assert rel_l2(got[pix], want) < 1e-5
 
 
+@pytest.mark.parametrize("diff", [0, 1])
+def test_box_pixel_count(ctx, orc, diff):
+    """ohx_last_launch_stats().pairs_in_box equals the sum over points of the
+    reference box areas (ophGen.cpp:384-407,467-490), on partial masks; and
+    for Fresnel, whose mask is the box, the count of written pixels of a
+    one-point field."""
+    pn, pp, off, dist = (100, 37), (10e-6, 8e-6), (3, -2), 0.5e-3
+    g = orc.make_geom(pn, pp, offset=off, scale=(1.0, 1.0, 1.0), distance=dist)
+    v = synth.random_cloud(301 + diff, 45, 350e-6, (1.0e-3, 3.5e-3))
+    lam = 633e-9
+    want = 0
+    for q in v:
+        b = orc.box(g, q["pos"], lam, diff)     # the oracle adds the geometry distance
+        want += (b[1] - b[0]) * (b[3] - b[2])
+    assert 0 < want < len(v) * pn[0] * pn[1]          # partial masks
+    capi.generate(ctx, v, pn, pp, [lam], diff, FP32, distance=dist, offset=off)
+    st = ctx.last_stats()
+    assert st.pairs_nominal == len(v) * pn[0] * pn[1]
+    assert st.pairs_in_box == want
+    if diff == 1:
+        f1 = capi.generate(ctx, v[:1], pn, pp, [lam], diff, FP32, distance=dist, offset=off)[0]
+        assert ctx.last_stats().pairs_in_box == int((f1 != 0).sum())
+
+
 def test_partial_masks_large_geometry(ctx, orc):
     """7680x4320 at 3.74 um, lambda 473 nm, z down to 0.3 m: single-point
     masks are partial, so tiles straddling a mask edge take the per-pixel

@@ -95,8 +95,8 @@ def cpu_baseline(w, target_s: float):
         return time.perf_counter() - t0
 
     per_pt = lambda n: w.n_pixels * len(w.lambdas) * len(w.modes) * n
-    # the reference parallelises over points: give every thread several
-    n = min(w.n_points, 4 * threads)
+    # the reference parallelises over points: start at one point per thread
+    n = min(w.n_points, threads)
     while True:
         t = run(n, threads)
         if t >= target_s / 5 or n >= w.n_points:

@@ -619,15 +619,29 @@ int launch_fresnel_gemm_range(hipStream_t stream, void* scratch, size_t scratch_
     return hipGetLastError() == hipSuccess ? OHX_OK : OHX_ERR_DEVICE;
 }
 
+// The split-K partial planes grow with points x pixels (16 x 66 MB for 1M points
+// at 3840x2160); above MAX_GEMM_SCRATCH the points are processed in chunks, each
+// reduced into the field before the next (chunk boundaries depend only on the
+// point count and geometry, so results stay deterministic).
+constexpr size_t MAX_GEMM_SCRATCH = size_t(4) << 30;
+
 int launch_fresnel_gemm(ohx_ctx* ctx, const Geometry& g, const PointSoA& soa, double* d_field, uint32_t mode)
 {
-    const size_t bytes = fresnel_gemm_scratch_bytes(g, soa.n);
+    int64_t chunk = soa.n;
+    while (chunk > gemm::KP && fresnel_gemm_scratch_bytes(g, chunk) > MAX_GEMM_SCRATCH)
+        chunk = (chunk + 1) / 2;
+    const size_t bytes = fresnel_gemm_scratch_bytes(g, chunk);
     int st = ensure_gemm_scratch(ctx, bytes);
     if (st) return st;
-    st = launch_fresnel_gemm_range(ctx->stream, ctx->gemm_scratch, ctx->gemm_scratch_bytes, g, soa, 0, soa.n,
-                                   d_field, mode);
-    const GemmPlan pl = plan_gemm(g, soa.n);
-    ctx->last.workgroups = (int64_t)pl.tiles * pl.splits;
+    int64_t wgs = 0;
+    for (int64_t p0 = 0; p0 < soa.n && !st; p0 += chunk) {
+        const int64_t p1 = std::min<int64_t>(soa.n, p0 + chunk);
+        st = launch_fresnel_gemm_range(ctx->stream, ctx->gemm_scratch, ctx->gemm_scratch_bytes, g, soa, p0, p1,
+                                       d_field, mode);
+        const GemmPlan pl = plan_gemm(g, p1 - p0);
+        wgs += (int64_t)pl.tiles * pl.splits;
+    }
+    ctx->last.workgroups = wgs;
     ctx->last.kernel_kind = 2;
     if (st) return set_error(ctx, st, "matrix-core Fresnel launch failed");
     return OHX_OK;

@@ -97,3 +97,31 @@ def test_gemm_rs_falls_back_to_direct(ctx):
     a = capi.generate(ctx, v, pn, pp, [532e-9], 0, VARIANTS["bf16x1"][0])[0]
     b = capi.generate(ctx, v, pn, pp, [532e-9], 0, FP32)[0]
     assert np.array_equal(a, b)
+
+
+def test_gemm_chunked_point_range(ctx):
+    """5M points at 3840x2160: the split-K scratch would exceed its 4 GiB cap,
+    so the launch runs in two point chunks, each reduced into the field.  The
+    result against the library's FP64 probe stays within the BF16x3 bound."""
+    mode, tol = VARIANTS["bf16x3"]
+    w = synth.WORKLOADS[3]
+    n = 5_000_000
+    v = synth.random_cloud(77, n, 8e-3, (0.4, 0.6))
+    p = capi.params(w.pn, w.pp, w.lambdas[0])
+    dv = ctx.to_device(v)
+    npx = w.n_pixels
+    fld = ctx.alloc(16 * npx)
+    try:
+        ctx.field_zero(fld, npx)
+        ctx.diffract(dv, n, fld, p, 1, mode)
+        st = ctx.last_stats()
+        pix = np.random.default_rng(5).choice(npx, 128, replace=False)
+        got = fld.download(np.complex128, npx)[pix]
+        want = ctx.probe_f64(dv, n, p, 1, pix)
+    finally:
+        dv.free()
+        fld.free()
+    assert st.kernel_kind == 2 and st.workgroups == 2 * 510 * 39   # two chunks of 2.5M points
+    err = rel_l2(got, want)
+    print(f"chunked bf16x3: rel-L2 {err:.3e}")
+    assert err < tol

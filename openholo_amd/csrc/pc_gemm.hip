@@ -42,8 +42,8 @@ constexpr int KP = 16;            // points per stage
 constexpr int KS = 2 * KP;        // realified K per stage (32)
 constexpr int LDK = KS + 8;       // padded row length (bf16/f16 elements): 80 B rows
 constexpr int ROWW = LDK / 2;     // row length in dwords (20)
-constexpr int NGRP = NT / KP;     // thread groups, one point each (32)
-constexpr int NVT = BM / NGRP;    // V rows per thread and stage (4): grp + NGRP i
+constexpr int NGRP = NT / KP;     // thread groups of 16 lanes, one point per lane (16)
+constexpr int NVT = BM / NGRP;    // V rows per thread and stage (8): grp + NGRP i
 constexpr int NUT = BN / NGRP;    // U columns per thread and stage (8): grp + NGRP j
 constexpr int NE = NVT + NUT;     // elements per thread and stage
 

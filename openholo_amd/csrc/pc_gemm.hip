@@ -190,6 +190,35 @@ __device__ inline bool prep(Pt& P, const Raw& r, double ythr, double xthr, doubl
     return prep_rest(P, r, x0, y0, vscale);
 }
 
+// The same work in NPIECE pieces of two or three FP64 operations, so that the
+// pipelined stage can spread it over several MFMA slots (FP64 issues at half
+// the FP32 rate; a burst of it leaves the matrix core idle).
+constexpr int NPIECE = 8;
+template <int K>
+__device__ inline void prep_piece(Pt& P, double (&t)[4], const Raw& r, double ythr, double xthr, double dy,
+                                  double dx)
+{
+    if constexpr (K == 0) {
+        t[0] = ythr - r.py;
+        t[1] = r.q * t[0];
+    } else if constexpr (K == 1) {
+        P.v0 = (float)frac64(t[1] * t[0] + r.cyc0);
+    } else if constexpr (K == 2) {
+        P.v1 = (float)frac64(t[1] * (2.0 * dy));
+    } else if constexpr (K == 3) {
+        P.v2 = (float)frac64(r.q * (dy * dy));
+    } else if constexpr (K == 4) {
+        t[2] = xthr - r.px;
+        t[3] = r.q * t[2];
+    } else if constexpr (K == 5) {
+        P.u0 = (float)frac64(t[3] * t[2]);
+    } else if constexpr (K == 6) {
+        P.u1 = (float)frac64(t[3] * (2.0 * dx));
+    } else if constexpr (K == 7) {
+        P.u2 = (float)frac64(r.q * (dx * dx));
+    }
+}
+
 // phase (cycles, |phase| < 1 + |k| + k^2 <= 21) of element k from its
 // coefficients (k is a compile-time constant wherever this is called).  No mod-1 reduction:
 // v_sin_f32 / v_cos_f32 reduce a revolutions argument themselves (on gfx950
@@ -305,6 +334,7 @@ struct StageCtx {
     float vscale;
     int p, grp, lrow, koff, ucol;
     Pt P;
+    double t[4];   // FP64 temporaries shared by the prep pieces
     bool partial;
     u32x4 A[8][2], Ub[2][2], B[2][4];
     float2 cs[NE];
@@ -334,19 +364,24 @@ __device__ inline void xform_b(StageCtx<VAR>& c, int ks, int h)
     c.B[ks][2 * h + 1] = to_im(c.Ub[ks][h]);
 }
 // The generation schedule: 3-term, half-element h (even: value of element
-// h/2, odd: its store) at slot 3 + 44 h / (2 NE); 1-term, element E at slot
-// 2 + 14 E / NE (two elements share a slot when NE > 14).  Return the
-// half-element / element at slot S, or -1.
+// h/2, odd: its store) at slot H0 + (46 - H0) h / (2 NE); 1-term, element E
+// at slot E0 + (16 - E0) E / NE (two elements share a slot when needed).
+// Return the half-element / element at slot S, or -1.
+// 3-term: prep piece K at slot K, amplitude/box and the next loads at slot
+// NPIECE, half-elements from slot NPIECE + 1 to 46.  1-term: prep pieces two
+// per slot (slots 0 .. NPIECE/2 - 1), amplitude/box at NPIECE/2, elements from
+// NPIECE/2 + 1 to 15.
+constexpr int H0 = NPIECE + 1, E0 = NPIECE / 2 + 1;
 constexpr int half_at(int S)
 {
     for (int h = 0; h < 2 * NE; ++h)
-        if (3 + (44 * h) / (2 * NE) == S) return h;
+        if (H0 + ((46 - H0) * h) / (2 * NE) == S) return h;
     return -1;
 }
 constexpr int elem_at(int S, int n)   // n-th element at slot S (slots hold at most two)
 {
     for (int e = 0; e < NE; ++e)
-        if (2 + (14 * e) / NE == S && n-- == 0) return e;
+        if (E0 + ((16 - E0) * e) / NE == S && n-- == 0) return e;
     return -1;
 }
 // every element (1-term) and half-element (3-term) lands in exactly one slot
@@ -385,9 +420,12 @@ __device__ inline void slot(const unsigned* __restrict__ rv, const unsigned* __r
         if constexpr (S == NM / 4) load_b(ru, c, 1);
         if constexpr (S == NM * 3 / 8) xform_b(c, 1, 0);
         if constexpr (THREE && S == NM * 3 / 8 + 2) xform_b(c, 1, 1);
-        if constexpr (S == 0) prep_y(c.P, c.R, c.ythr, c.dy);
-        if constexpr (S == 1) prep_x(c.P, c.R, c.xthr, c.dx);
-        if constexpr (S == (THREE ? 2 : 1)) {
+        if constexpr (THREE && S < NPIECE) prep_piece<S>(c.P, c.t, c.R, c.ythr, c.xthr, c.dy, c.dx);
+        if constexpr (!THREE && S < NPIECE / 2) {
+            prep_piece<2 * S>(c.P, c.t, c.R, c.ythr, c.xthr, c.dy, c.dx);
+            prep_piece<2 * S + 1>(c.P, c.t, c.R, c.ythr, c.xthr, c.dy, c.dx);
+        }
+        if constexpr (S == (THREE ? NPIECE : NPIECE / 2)) {
             c.partial = prep_rest(c.P, c.R, c.x0, c.y0, c.vscale);
             load_raw(c.R, c.s, c.pnext, c.ke);
         }

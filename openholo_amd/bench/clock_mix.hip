@@ -14,8 +14,9 @@
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
-// NM MFMAs (0/1), NV FMAs and NS sin+cos pairs per slot; 16 slots per iteration
-template <int NM, int NV, int NS>
+// NM MFMAs (0/1), NV FMAs (FP64 when D64) and NS sin+cos pairs per slot; 16
+// slots per iteration
+template <int NM, int NV, int NS, bool D64 = false>
 __global__ __launch_bounds__(256) void mixk(float* out, long long* tm, int iters)
 {
     __shared__ float pad[20 * 1024];
@@ -27,7 +28,11 @@ __global__ __launch_bounds__(256) void mixk(float* out, long long* tm, int iters
     }
     f32x16 d[4] = {};
     float x[8];
-    for (int i = 0; i < 8; ++i) x[i] = 0.001f * (threadIdx.x + i);
+    double y[8];
+    for (int i = 0; i < 8; ++i) {
+        x[i] = 0.001f * (threadIdx.x + i);
+        y[i] = 0.001 * (threadIdx.x + i);
+    }
     const float c1 = out[0] + 0.9999f, c2 = out[1] + 1e-4f;
     pad[threadIdx.x] = c1;
     __syncthreads();
@@ -36,7 +41,10 @@ __global__ __launch_bounds__(256) void mixk(float* out, long long* tm, int iters
         for (int i = 0; i < 16; ++i) {
             if constexpr (NM > 0) d[i & 3] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, d[i & 3], 0, 0, 0);
 #pragma unroll
-            for (int j = 0; j < NV; ++j) x[j & 7] = fmaf(x[j & 7], c1, c2);
+            for (int j = 0; j < NV; ++j) {
+                if constexpr (D64) y[j & 7] = fma(y[j & 7], (double)c1, (double)c2);
+                else x[j & 7] = fmaf(x[j & 7], c1, c2);
+            }
 #pragma unroll
             for (int j = 0; j < NS; ++j)
                 x[j & 7] = __builtin_amdgcn_sinf(x[j & 7]) + __builtin_amdgcn_cosf(x[(j + 1) & 7]);
@@ -45,7 +53,7 @@ __global__ __launch_bounds__(256) void mixk(float* out, long long* tm, int iters
     }
     float s = pad[(threadIdx.x + 1) & 255];
     for (int i = 0; i < 4; ++i) s += d[i][0] + d[i][15];
-    for (int i = 0; i < 8; ++i) s += x[i];
+    for (int i = 0; i < 8; ++i) s += x[i] + (float)y[i];
     out[2 + blockIdx.x * 256 + threadIdx.x] = s;
     if (threadIdx.x == 0) {
         tm[4 * blockIdx.x] = c0;
@@ -55,7 +63,7 @@ __global__ __launch_bounds__(256) void mixk(float* out, long long* tm, int iters
     }
 }
 
-template <int NM, int NV, int NS>
+template <int NM, int NV, int NS, bool D64 = false>
 int run(const char* name, float* out, long long* tm, int wall_khz)
 {
     const int blocks = 256 * 8, iters = 1024;
@@ -65,7 +73,7 @@ int run(const char* name, float* out, long long* tm, int wall_khz)
     float ms = 0;
     for (int rep = 0; rep < 2; ++rep) {
         CK(hipEventRecord(e0));
-        hipLaunchKernelGGL((mixk<NM, NV, NS>), dim3(blocks), dim3(256), 0, 0, out, tm, iters);
+        hipLaunchKernelGGL((mixk<NM, NV, NS, D64>), dim3(blocks), dim3(256), 0, 0, out, tm, iters);
         CK(hipGetLastError());
         CK(hipEventRecord(e1));
         CK(hipEventSynchronize(e1));
@@ -100,5 +108,8 @@ int main()
     run<1, 6, 0>("mfma + 6 fma", out, tm, wall_khz);
     run<1, 4, 1>("mfma + 4 fma + sin + cos", out, tm, wall_khz);
     run<1, 8, 1>("mfma + 8 fma + sin + cos", out, tm, wall_khz);
+    run<0, 8, 0, true>("8 fma_f64", out, tm, wall_khz);
+    run<1, 4, 0>("mfma + 4 fma", out, tm, wall_khz);
+    run<1, 4, 0, true>("mfma + 4 fma_f64", out, tm, wall_khz);
     return 0;
 }

@@ -52,6 +52,23 @@ __global__ void ksinrange(unsigned* cnt, unsigned* maxd, float lim) {
   if (s1 != s2 || c1 != c2) atomicAdd(cnt, 1u);
   if (d > 0.f) atomicMax(maxd, __float_as_uint(d));
 }
+// FP64 issue rate: KIND 0 fma_f64, 1 add_f64 + fract_f64, 2 cvt_f32_f64 + add_f64
+template <int KIND>
+__global__ __launch_bounds__(256) void k64(float* out, int iters, double b, double c) {
+  double a[CH / 2];
+  float f[CH / 2];
+  for (int i = 0; i < CH / 2; ++i) { a[i] = threadIdx.x * 1e-3 + i * 1e-2; f[i] = 0.f; }
+  for (int it = 0; it < iters; ++it) {
+#pragma unroll
+    for (int i = 0; i < CH / 2; ++i) {
+      if (KIND == 0) a[i] = fma(a[i], b, c);
+      else if (KIND == 1) a[i] = __builtin_amdgcn_fract(a[i] + c);
+      else { f[i] += (float)a[i]; a[i] += c; }
+    }
+  }
+  float s = 0; for (int i = 0; i < CH / 2; ++i) s += (float)a[i] + f[i];
+  out[blockIdx.x * 256 + threadIdx.x] = s;
+}
 __global__ void kcopy(const float4* __restrict__ a, float4* __restrict__ b, size_t n) {
   size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x, st = (size_t)gridDim.x * blockDim.x;
   for (; i < n; i += st) b[i] = a[i];
@@ -84,6 +101,22 @@ int main() {
       double lanes_ops = (double)blocks * 256 * iters * CH;   // per "element op group"
       if (rep) printf("%-12s %8.3f ms  %8.2f T elem-iter/s  %8.2f T VALU-instr/s (lane)\n", names[kind], ms,
                       lanes_ops / ms / 1e9, lanes_ops * instr[kind] / ms / 1e9);
+    }
+  }
+  const char* n64[] = {"fma_f64", "add_f64+fract_f64", "cvt_f32_f64+add_f32+add_f64"};
+  const int i64[] = {1, 2, 3};
+  for (int kind = 0; kind < 3; ++kind) {
+    for (int rep = 0; rep < 2; ++rep) {
+      CK(hipEventRecord(e0));
+      if (kind == 0) hipLaunchKernelGGL(k64<0>, dim3(blocks), dim3(256), 0, 0, out, iters, 0.999, 1e-4);
+      if (kind == 1) hipLaunchKernelGGL(k64<1>, dim3(blocks), dim3(256), 0, 0, out, iters, 0.999, 1e-4);
+      if (kind == 2) hipLaunchKernelGGL(k64<2>, dim3(blocks), dim3(256), 0, 0, out, iters, 0.999, 1e-4);
+      CK(hipGetLastError());
+      CK(hipEventRecord(e1)); CK(hipEventSynchronize(e1));
+      float ms; CK(hipEventElapsedTime(&ms, e0, e1));
+      const double el = (double)blocks * 256 * iters * (CH / 2);
+      if (rep) printf("%-28s %8.3f ms  %8.2f T elem-iter/s  %8.2f T VALU-instr/s (lane)\n", n64[kind], ms, el / ms / 1e9,
+                      el * i64[kind] / ms / 1e9);
     }
   }
   // copy kernel of known size for counter calibration: 1 GiB read + 1 GiB write

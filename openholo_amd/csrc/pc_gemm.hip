@@ -119,15 +119,18 @@ __device__ inline double gpix_y(const Geometry& g, int row)
 }
 
 // Point data of one stage as read from HBM: thread t holds point t % 16.
+// pc = (v2, u2, amplitude, 0) from k_gemm_consts.
 struct Raw {
-    double px, py, q, cyc0, kamp;
+    double px, py, q, cyc0;
+    float4 pc;
     int4 box;
     bool valid;
 };
 
 // Lanes past the slice end read the slice's last point and are marked invalid
 // (zero amplitude, full box), so the loads are unconditional and in bounds.
-__device__ inline void load_raw(Raw& r, const PointSoA& s, int64_t pi, int64_t p1)
+__device__ inline void load_raw(Raw& r, const PointSoA& s, const float4* __restrict__ pconst, int64_t kfirst,
+                                int64_t pi, int64_t p1)
 {
     r.valid = pi < p1;
     const int64_t j = r.valid ? pi : p1 - 1;
@@ -135,7 +138,7 @@ __device__ inline void load_raw(Raw& r, const PointSoA& s, int64_t pi, int64_t p
     r.py = s.py[j];
     r.q = s.q[j];
     r.cyc0 = s.cyc0[j];
-    r.kamp = s.kamp[j];
+    r.pc = pconst[j - kfirst];
     r.box = s.box[j];
 }
 
@@ -163,37 +166,35 @@ __device__ inline void prep_y(Pt& P, const Raw& r, double ythr, double dy)
     const double ey = ythr - r.py, ty = r.q * ey;
     P.v0 = (float)frac64(ty * ey + r.cyc0);
     P.v1 = (float)frac64(ty * (2.0 * dy));
-    P.v2 = (float)frac64(r.q * (dy * dy));
+    P.v2 = r.pc.x;
 }
 __device__ inline void prep_x(Pt& P, const Raw& r, double xthr, double dx)
 {
     const double ex = xthr - r.px, tx = r.q * ex;
     P.u0 = (float)frac64(tx * ex);
     P.u1 = (float)frac64(tx * (2.0 * dx));
-    P.u2 = (float)frac64(r.q * (dx * dx));
+    P.u2 = r.pc.y;
 }
 // amplitude and box; returns true when the box does not cover the whole tile
-__device__ inline bool prep_rest(Pt& P, const Raw& r, int x0, int y0, float vscale)
+__device__ inline bool prep_rest(Pt& P, const Raw& r, int x0, int y0)
 {
-    P.amp = r.valid ? (float)r.kamp * vscale : 0.f;
+    P.amp = r.valid ? r.pc.z : 0.f;
     P.rlo = r.valid ? r.box.z - y0 : 0;
     P.rhi = r.valid ? r.box.w - y0 : BM;
     P.clo = r.valid ? r.box.x - x0 : 0;
     P.chi = r.valid ? r.box.y - x0 : BN;
     return P.rlo > 0 || P.rhi < BM || P.clo > 0 || P.chi < BN;
 }
-__device__ inline bool prep(Pt& P, const Raw& r, double ythr, double xthr, double dy, double dx, int x0, int y0,
-                            float vscale)
+__device__ inline bool prep(Pt& P, const Raw& r, double ythr, double xthr, double dy, double dx, int x0, int y0)
 {
     prep_y(P, r, ythr, dy);
     prep_x(P, r, xthr, dx);
-    return prep_rest(P, r, x0, y0, vscale);
+    return prep_rest(P, r, x0, y0);
 }
 
-// The same work in NPIECE pieces of two or three FP64 operations, so that the
-// pipelined stage can spread it over several MFMA slots (FP64 issues at half
-// the FP32 rate; a burst of it leaves the matrix core idle).
-constexpr int NPIECE = 8;
+// The same work in NPIECE pieces of two or three FP64 operations, spread by the
+// pipelined stage over its first MFMA slots.
+constexpr int NPIECE = 6;
 template <int K>
 __device__ inline void prep_piece(Pt& P, double (&t)[4], const Raw& r, double ythr, double xthr, double dy,
                                   double dx)
@@ -205,17 +206,15 @@ __device__ inline void prep_piece(Pt& P, double (&t)[4], const Raw& r, double yt
         P.v0 = (float)frac64(t[1] * t[0] + r.cyc0);
     } else if constexpr (K == 2) {
         P.v1 = (float)frac64(t[1] * (2.0 * dy));
+        P.v2 = r.pc.x;
     } else if constexpr (K == 3) {
-        P.v2 = (float)frac64(r.q * (dy * dy));
-    } else if constexpr (K == 4) {
         t[2] = xthr - r.px;
         t[3] = r.q * t[2];
-    } else if constexpr (K == 5) {
+    } else if constexpr (K == 4) {
         P.u0 = (float)frac64(t[3] * t[2]);
-    } else if constexpr (K == 6) {
+    } else if constexpr (K == 5) {
         P.u1 = (float)frac64(t[3] * (2.0 * dx));
-    } else if constexpr (K == 7) {
-        P.u2 = (float)frac64(r.q * (dx * dx));
+        P.u2 = r.pc.y;
     }
 }
 
@@ -328,10 +327,10 @@ struct StageCtx {
     f32x16 (&acc)[4][2];
     Raw& R;
     const PointSoA& s;
-    int64_t pnext, ke;
+    const float4* pconst;
+    int64_t kfirst, pnext, ke;
     double ythr, xthr, dy, dx;
     int x0, y0;
-    float vscale;
     int p, grp, lrow, koff, ucol;
     Pt P;
     double t[4];   // FP64 temporaries shared by the prep pieces
@@ -426,8 +425,8 @@ __device__ inline void slot(const unsigned* __restrict__ rv, const unsigned* __r
             prep_piece<2 * S + 1>(c.P, c.t, c.R, c.ythr, c.xthr, c.dy, c.dx);
         }
         if constexpr (S == (THREE ? NPIECE : NPIECE / 2)) {
-            c.partial = prep_rest(c.P, c.R, c.x0, c.y0, c.vscale);
-            load_raw(c.R, c.s, c.pnext, c.ke);
+            c.partial = prep_rest(c.P, c.R, c.x0, c.y0);
+            load_raw(c.R, c.s, c.pconst, c.kfirst, c.pnext, c.ke);
         }
         if constexpr (THREE) {
             constexpr int h = half_at(S);
@@ -454,12 +453,12 @@ __device__ inline void slot(const unsigned* __restrict__ rv, const unsigned* __r
 template <int VAR>
 __device__ inline bool stage_body(const unsigned* __restrict__ rv, const unsigned* __restrict__ ru,
                                   unsigned* __restrict__ wv, unsigned* __restrict__ wu, f32x16 (&acc)[4][2],
-                                  Pt& P, Raw& R, const PointSoA& s, int64_t pnext, int64_t ke, double ythr,
-                                  double xthr, double dy, double dx, int x0, int y0, float vscale, int p, int grp,
-                                  int lrow, int koff, int ucol)
+                                  Pt& P, Raw& R, const PointSoA& s, const float4* pconst, int64_t kfirst,
+                                  int64_t pnext, int64_t ke, double ythr, double xthr, double dy, double dx, int x0,
+                                  int y0, int p, int grp, int lrow, int koff, int ucol)
 {
     constexpr bool THREE = Traits<VAR>::three;
-    StageCtx<VAR> c{acc, R, s, pnext, ke, ythr, xthr, dy, dx, x0, y0, vscale, p, grp, lrow, koff, ucol};
+    StageCtx<VAR> c{acc, R, s, pconst, kfirst, pnext, ke, ythr, xthr, dy, dx, x0, y0, p, grp, lrow, koff, ucol};
     c.partial = false;
     load_b(ru, c, 0);
     load_a(rv, c, 0);
@@ -477,22 +476,25 @@ __device__ inline bool stage_body(const unsigned* __restrict__ rv, const unsigne
 // runs on clamped, zero-amplitude data into the unused buffer: the step stays
 // branch-free, which keeps the accumulators in place.
 template <int VAR, int RB>
-__device__ inline void step(Smem<Traits<VAR>::three>& sm, f32x16 (&acc)[4][2], Raw& R, const PointSoA& s, int st,
-                            int64_t kb, int64_t ke, double ythr, double xthr, double dy, double dx, int x0, int y0,
-                            float vscale, int p, int grp, int lrow, int koff, int ucolx)
+__device__ inline void step(Smem<Traits<VAR>::three>& sm, f32x16 (&acc)[4][2], Raw& R, const PointSoA& s,
+                            const float4* pconst, int64_t kfirst, int st, int64_t kb, int64_t ke, double ythr,
+                            double xthr, double dy, double dx, int x0, int y0, int p, int grp, int lrow, int koff,
+                            int ucolx)
 {
     Pt P;
     const bool partial = stage_body<VAR>(&sm.v[RB][0][0][0], &sm.u[RB][0][0][0], &sm.v[RB ^ 1][0][0][0],
-                                         &sm.u[RB ^ 1][0][0][0], acc, P, R, s, kb + (int64_t)(st + 2) * KP + p, ke,
-                                         ythr, xthr, dy, dx, x0, y0, vscale, p, grp, lrow, koff, ucolx);
+                                         &sm.u[RB ^ 1][0][0][0], acc, P, R, s, pconst, kfirst,
+                                         kb + (int64_t)(st + 2) * KP + p, ke, ythr, xthr, dy, dx, x0, y0, p, grp,
+                                         lrow, koff, ucolx);
     if (__builtin_amdgcn_ballot_w64(partial)) mask_stage<VAR, RB ^ 1>(sm, P, p, grp);
     __syncthreads();
 }
 
+// pconst: per-point constants of points kfirst .. klast-1 (k_gemm_consts)
 template <int VAR>
-__global__ __launch_bounds__(NT, 2) void k_fresnel_gemm(Geometry g, PointSoA s, float2* __restrict__ part,
-                                                        int tiles_x, int tiles, int64_t kfirst, int64_t klast,
-                                                        int64_t kslice, const float* __restrict__ amax)
+__global__ __launch_bounds__(NT, 2) void k_fresnel_gemm(Geometry g, PointSoA s, const float4* __restrict__ pconst,
+                                                        float2* __restrict__ part, int tiles_x, int tiles,
+                                                        int64_t kfirst, int64_t klast, int64_t kslice)
 {
     constexpr bool THREE = Traits<VAR>::three;
     __shared__ __align__(16) Smem<THREE> sm;
@@ -504,8 +506,6 @@ __global__ __launch_bounds__(NT, 2) void k_fresnel_gemm(Geometry g, PointSoA s, 
     const int64_t kb = kfirst + (int64_t)split * kslice;
     const int64_t ke = min(klast, kb + kslice);
     const int nst = (int)((ke - kb + KP - 1) / KP);
-    const float am = amax[0];
-    const float vscale = (Traits<VAR>::fp16 && am > 0.f) ? 1.0f / am : 1.0f;
     const int p = threadIdx.x & (KP - 1), grp = threadIdx.x >> 4;
     const double ythr = gpix_y(g, y0 + grp + NGRP * (NVT / 2)), xthr = gpix_x(g, x0 + grp + NGRP * (NUT / 2));
     const double dy = -NGRP * g.pp_y, dx = NGRP * g.pp_x;
@@ -518,21 +518,23 @@ __global__ __launch_bounds__(NT, 2) void k_fresnel_gemm(Geometry g, PointSoA s, 
     // split, which the planner never makes, would write zeros)
     Raw R;
     if (nst > 0) {
-        load_raw(R, s, kb + p, ke);
+        load_raw(R, s, pconst, kfirst, kb + p, ke);
         Pt P;
-        const bool partial = prep(P, R, ythr, xthr, dy, dx, x0, y0, vscale);
+        const bool partial = prep(P, R, ythr, xthr, dy, dx, x0, y0);
         gen_stage<VAR, 0>(sm, P, p, grp);
         if (partial) mask_stage<VAR, 0>(sm, P, p, grp);
-        load_raw(R, s, kb + KP + p, ke);
+        load_raw(R, s, pconst, kfirst, kb + KP + p, ke);
     }
     __syncthreads();
     f32x16 acc[4][2];
 #pragma unroll
     for (int i = 0; i < 4; ++i) acc[i][0] = acc[i][1] = f32x16{};
     for (int st = 0; st < nst; st += 2) {
-        step<VAR, 0>(sm, acc, R, s, st, kb, ke, ythr, xthr, dy, dx, x0, y0, vscale, p, grp, lrow, koff, ucolx);
+        step<VAR, 0>(sm, acc, R, s, pconst, kfirst, st, kb, ke, ythr, xthr, dy, dx, x0, y0, p, grp, lrow, koff,
+                     ucolx);
         if (st + 1 >= nst) break;
-        step<VAR, 1>(sm, acc, R, s, st + 1, kb, ke, ythr, xthr, dy, dx, x0, y0, vscale, p, grp, lrow, koff, ucolx);
+        step<VAR, 1>(sm, acc, R, s, pconst, kfirst, st + 1, kb, ke, ythr, xthr, dy, dx, x0, y0, p, grp, lrow, koff,
+                     ucolx);
     }
 
     // FP32 partial sums S of this split, interleaved (Re, Im) per pixel
@@ -586,6 +588,24 @@ __global__ void k_amp_init(float* amax)
     if (threadIdx.x == 0) amax[0] = 0.f;
 }
 
+// Per-point constants of the stage generation, once per launch instead of once
+// per (point, tile, thread): the k^2 coefficients v2 = frac(q dy^2) and
+// u2 = frac(q dx^2) (dy = -NGRP pp_y, dx = NGRP pp_x; FP64 then reduced mod 1
+// and rounded) and the amplitude a/(lambda z), times 1/max|A| for FP16.
+__global__ __launch_bounds__(256) void k_gemm_consts(Geometry g, PointSoA s, int64_t p0, int64_t p1,
+                                                     const float* __restrict__ amax, int scaled,
+                                                     float4* __restrict__ pconst)
+{
+    const int64_t i = p0 + (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= p1) return;
+    const double dy = -NGRP * g.pp_y, dx = NGRP * g.pp_x;
+    const double q = s.q[i];
+    const float am = amax[0];
+    const float vscale = (scaled && am > 0.f) ? 1.0f / am : 1.0f;
+    pconst[i - p0] = make_float4((float)frac64(q * (dy * dy)), (float)frac64(q * (dx * dx)),
+                                 (float)s.kamp[i] * vscale, 0.f);
+}
+
 }  // namespace gemm
 
 namespace {
@@ -616,10 +636,14 @@ GemmPlan plan_gemm(const Geometry& g, int64_t npoints)
 }
 }  // namespace
 
+// scratch: [amax, 256 B] [per-point float4 constants, padded to 256 B]
+// [split-K partial planes]
+static size_t consts_bytes(int64_t npoints) { return ((size_t)npoints * sizeof(float4) + 255) / 256 * 256; }
+
 size_t fresnel_gemm_scratch_bytes(const Geometry& g, int64_t npoints)
 {
     const GemmPlan p = plan_gemm(g, npoints);
-    return 256 + (size_t)p.splits * p.npx * sizeof(float2);
+    return 256 + consts_bytes(npoints) + (size_t)p.splits * p.npx * sizeof(float2);
 }
 
 int launch_fresnel_gemm_range(hipStream_t stream, void* scratch, size_t scratch_bytes, const Geometry& g,
@@ -629,24 +653,27 @@ int launch_fresnel_gemm_range(hipStream_t stream, void* scratch, size_t scratch_
     if (p1 <= p0) return OHX_OK;
     const int var = (mode & OHX_MODE_MFMA_FP16X3) ? V_FP16X3 : (mode & OHX_MODE_MFMA_BF16X3) ? V_BF16X3 : V_BF16X1;
     const GemmPlan pl = plan_gemm(g, p1 - p0);
-    if (256 + (size_t)pl.splits * pl.npx * sizeof(float2) > scratch_bytes) return OHX_ERR_INVALID_ARG;
+    if (fresnel_gemm_scratch_bytes(g, p1 - p0) > scratch_bytes) return OHX_ERR_INVALID_ARG;
     float* amax = (float*)scratch;
-    float2* part = (float2*)((char*)scratch + 256);
+    float4* pconst = (float4*)((char*)scratch + 256);
+    float2* part = (float2*)((char*)scratch + 256 + consts_bytes(p1 - p0));
     hipLaunchKernelGGL(k_amp_init, dim3(1), dim3(64), 0, stream, amax);
     hipLaunchKernelGGL(k_amp_max, dim3(256), dim3(256), 0, stream, (const double*)soa.kamp, p0, p1, amax);
+    hipLaunchKernelGGL(k_gemm_consts, dim3((unsigned)((p1 - p0 + 255) / 256)), dim3(256), 0, stream, g, soa, p0,
+                       p1, (const float*)amax, var == V_FP16X3 ? 1 : 0, pconst);
     const dim3 grid((unsigned)((int64_t)pl.tiles * pl.splits)), block(NT);
     switch (var) {
     case V_BF16X1:
-        hipLaunchKernelGGL(k_fresnel_gemm<V_BF16X1>, grid, block, 0, stream, g, soa, part, pl.tiles_x, pl.tiles,
-                           p0, p1, pl.kslice, (const float*)amax);
+        hipLaunchKernelGGL(k_fresnel_gemm<V_BF16X1>, grid, block, 0, stream, g, soa, (const float4*)pconst, part,
+                           pl.tiles_x, pl.tiles, p0, p1, pl.kslice);
         break;
     case V_BF16X3:
-        hipLaunchKernelGGL(k_fresnel_gemm<V_BF16X3>, grid, block, 0, stream, g, soa, part, pl.tiles_x, pl.tiles,
-                           p0, p1, pl.kslice, (const float*)amax);
+        hipLaunchKernelGGL(k_fresnel_gemm<V_BF16X3>, grid, block, 0, stream, g, soa, (const float4*)pconst, part,
+                           pl.tiles_x, pl.tiles, p0, p1, pl.kslice);
         break;
     default:
-        hipLaunchKernelGGL(k_fresnel_gemm<V_FP16X3>, grid, block, 0, stream, g, soa, part, pl.tiles_x, pl.tiles,
-                           p0, p1, pl.kslice, (const float*)amax);
+        hipLaunchKernelGGL(k_fresnel_gemm<V_FP16X3>, grid, block, 0, stream, g, soa, (const float4*)pconst, part,
+                           pl.tiles_x, pl.tiles, p0, p1, pl.kslice);
         break;
     }
     if (hipGetLastError() != hipSuccess) return OHX_ERR_DEVICE;

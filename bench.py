@@ -34,6 +34,10 @@ sys.path.insert(0, ROOT)
 #   Fresnel: add, sin, cos, 2 fma (accumulate)                                      = 7
 FLOP_PER_PAIR = {0: 13, 1: 7}
 PEAK_FP32_TFLOPS = 157.3          # published MI355X FP32 vector peak
+# FP64 mode, pair loop of the separable form (FMA = 2): RS series correction
+# + complex products, Fresnel one complex multiply-add
+FLOP_PER_PAIR_FP64 = {0: 37, 1: 8}
+PEAK_FP64_TFLOPS = 78.6           # published MI355X FP64 vector peak
 PEAK_BF16_TFLOPS = 2500.0         # published dense BF16/FP16 matrix peak
 MODE_GPU, MODE_FLOAT = 1, 2
 KERNELS = {"direct": MODE_GPU | MODE_FLOAT, "fp64": MODE_GPU, "bf16x1": MODE_GPU | MODE_FLOAT | 0x10,
@@ -342,9 +346,18 @@ def main():
                     "unit": "TFLOP/s", "frac": round(achieved / PEAK_BF16_TFLOPS, 4), "traffic": None,
                     "kernel": f"k_fresnel_gemm<{args.kernel}>", "avg_launch_ms": round(k_ms, 3),
                     "model": f"8 FLOP per complex MAC x {MFMA_TERMS[args.kernel]} term(s)"}
+    elif args.kernel == "fp64":
+        fl = FLOP_PER_PAIR_FP64[dom]
+        achieved = pairs_launch / (k_ms / 1e3) * fl / 1e12
+        roofline = {"bound": "valu", "achieved": round(achieved, 2), "peak": PEAK_FP64_TFLOPS,
+                    "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": None,
+                    "kernel": f"k_pc_direct64<{'RS' if dom == 0 else 'Fresnel'}>",
+                    "avg_launch_ms": round(k_ms, 3),
+                    "model": f"{fl} FP64 FLOP/pair in the pair loop (FMA=2); the per-lane column "
+                             f"factor (one FP64 sincospi per point, shared by 8 rows) is not counted"}
     else:
-        roofline = {"bound": "valu", "achieved": None, "peak": 78.6, "unit": "TFLOP/s", "frac": None,
-                    "traffic": None, "avg_launch_ms": round(k_ms, 3)}
+        roofline = {"bound": "valu", "achieved": None, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                    "frac": None, "traffic": None, "avg_launch_ms": round(k_ms, 3)}
 
     per_mode_ms = {("RS" if d == 0 else "Fresnel"): round(float(np.mean([a.elapsed_ms(b) for a, b in ev])), 3)
                    for d, ev in kev.items()}

@@ -34,6 +34,10 @@
 //  * FP32 partial sums per chunk are folded into FP64 accumulators; the
 //    field (interleaved FP64 complex, the reference's Complex<Real> layout)
 //    is read-modify-written once per pixel at the end, 1 KB coalesced per row.
+//  * MODE_DOUBLE (every term in FP64) uses the same tiles but separates the
+//    phase into a column factor (one FP64 sincos per lane and point) times
+//    a row factor (tabulated in LDS per chunk), times a short FP64 series
+//    correction for RS; a Fresnel pair is one complex multiply-add.
 #include "ohx_internal.h"
 
 namespace ohx {
@@ -197,7 +201,7 @@ struct DirectSmem {
 // Always inlined: the multi-job kernel calls all four variants, and a real
 // call would pin that kernel at the callees' worst-case register count
 // (one wave per SIMD) and add call overhead per tile.
-template <int DIFF, bool FP64>
+template <int DIFF>
 __device__ __attribute__((always_inline)) inline void direct_tile(DirectSmem& sm, const Geometry& g, const PointSoA& s, int64_t p_begin,
                            int64_t p_end, double2* __restrict__ field, int tile, int tiles_x)
 {
@@ -246,9 +250,7 @@ __device__ __attribute__((always_inline)) inline void direct_tile(DirectSmem& sm
                 kamp = s.kamp[p];
                 q = s.q[p];
                 const double dxc = xc - px, dyc = yc - py;
-                if (FP64) {
-                    cls = CLS_EXACT;
-                } else if (DIFF == OHX_DIFF_FRESNEL) {
+                if (DIFF == OHX_DIFF_FRESNEL) {
                     // FP32 phase terms must stay below MAX_CYC cycles on the tile
                     const double span = (2.0 * fabs(dxc) * umax + (double)umax * umax +
                                          2.0 * fabs(dyc) * vmax + (double)vmax * vmax) * q;
@@ -455,13 +457,302 @@ __device__ __attribute__((always_inline)) inline void direct_tile(DirectSmem& sm
     }
 }
 
+// ---------------------------------------------------------------------------
+// FP64 path (MODE_DOUBLE): every term in FP64, with the per-pixel sin/cos
+// taken out of the loop by separating the phase over the tile.
+//  * Fresnel: c = cyc0 + q dx^2 + q dy^2 exactly, so the term
+//      (1/j) a e^{j 2pi c} = A(col) B(row),
+//      A = a (sin, -cos)(2pi (cyc0 + q dx^2)),  B = (cos, sin)(2pi q dy^2),
+//    and the mask box is separable too (A = 0 outside its columns, B = 0
+//    outside its rows): a pair is one complex multiply-add (4 FMAs).
+//  * RS: with N = r^2 - r_c^2 = N_x(col) + N_y(row) about the tile centre
+//    and e = N / r_c^2,
+//      r/lambda = r_c/lambda + h N + phi(N),  h = 1/(2 r_c lambda),
+//      phi = (r_c/lambda)(-e^2/8 + e^3/16 - 5 e^4/128),
+//      z/r^2 = (z/r_c^2)(1 - e + e^2 - e^3),
+//    so the term is A(col) B(row) T(N), T = (z/r^2)/(z/r_c^2) e^{j 2pi phi},
+//    with |2pi phi| <= 4e-3 rad evaluated by cos/sin polynomials (remainders
+//    below 1e-14).  A (point, tile) pair takes this form only when the whole
+//    tile is inside the mask with margin and the truncated series terms are
+//    below 1e-13 cycles and 1e-14 relative amplitude; every other pair goes
+//    to the per-pixel list (exact_term under the reference's mask).
+constexpr int PCD = 32;   // points per chunk on the FP64 path
+constexpr int NCS = 12;   // constants per point of the separable list
+constexpr double TWO_PI = 6.283185307179586476925286766559;
+// separable-list constant slots
+enum { F_PX = 0, F_PY, F_CYC, F_Q, F_AMP };
+enum { R_C0 = 0, R_H, R_G2X, R_AMP, R_K2, R_K3, R_K4, R_M1, R_M2, R_M3, R_G2Y };
+
+struct DirectSmem64 {
+    double2 B[TH][PCD];    // row factors B(row) of the separable list
+    double Ny[TH][PCD];    // RS: N_y(row)
+    double c[PCD][NCS];    // separable list: per-point constants
+    int4 sbox[PCD];        // separable list: mask box (Fresnel)
+    double d[6][PCD];      // per-pixel list: px, py, z, cyc0, kamp, q
+    int4 ebox[PCD];        // per-pixel list: mask box
+    int cnt[2];
+};
+
+template <int DIFF>
+__device__ __attribute__((always_inline)) inline void direct_tile64(DirectSmem64& sm, const Geometry& g,
+                                                                    const PointSoA& s, int64_t p_begin, int64_t p_end,
+                                                                    double2* __restrict__ field, int tile, int tiles_x)
+{
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tx = tile % tiles_x, ty = tile / tiles_x;
+    const int x0 = tx * TW, y0 = ty * TH;
+    const int cx1 = min(x0 + TW, g.pn_x), ry1 = min(y0 + TH, g.pn_y);
+    const int col = x0 + lane;
+    const int rowb = y0 + wave * RPT;
+    const double xc = pix_x(g, x0 + TW / 2), yc = pix_y(g, y0 + TH / 2);
+    const double xxx = pix_x(g, col);
+    const double u = xxx - xc;
+    // |u| <= 32 pitches and |v| <= 16 pitches over the tile
+    const double umax = (double)(TW / 2) * g.pp_x, vmax = (double)(TH / 2) * g.pp_y;
+
+    double accr[RPT], acci[RPT];
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) accr[k] = acci[k] = 0.0;
+
+    for (int64_t base = p_begin; base < p_end; base += PCD) {
+        // ---- classify one point per lane of wave 0 --------------------------
+        if (wave == 0) {
+            bool sep = false, ex = false;
+            int4 b = make_int4(0, 0, 0, 0);
+            double px = 0, py = 0, z = 0, cyc0 = 0, kamp = 0, q = 0;
+            double cc[NCS];
+#pragma unroll
+            for (int i = 0; i < NCS; ++i) cc[i] = 0.0;
+            const int64_t p = base + lane;
+            if (lane < PCD && p < p_end) {
+                b = s.box[p];
+                if (b.x < cx1 && b.y > x0 && b.z < ry1 && b.w > y0) {
+                    px = s.px[p];
+                    py = s.py[p];
+                    z = s.pz[p];
+                    cyc0 = s.cyc0[p];
+                    kamp = s.kamp[p];
+                    q = s.q[p];
+                    if (DIFF == OHX_DIFF_FRESNEL) {
+                        sep = true;
+                        cc[F_PX] = px;
+                        cc[F_PY] = py;
+                        cc[F_CYC] = cyc0;
+                        cc[F_Q] = q;
+                        cc[F_AMP] = kamp;
+                    } else {
+                        const bool full = b.x <= x0 && b.y >= cx1 && b.z <= y0 && b.w >= ry1;
+                        const double dxc = xc - px, dyc = yc - py;
+                        const double zz = z * z;
+                        const double rho2 = dxc * dxc + dyc * dyc;
+                        const double rc2 = rho2 + zz;
+                        const double rc = sqrt(rc2);
+                        const double nmax = 2.0 * umax * fabs(dxc) + umax * umax + 2.0 * vmax * fabs(dyc) + vmax * vmax;
+                        const double e = nmax / rc2;
+                        const double e2 = e * e;
+                        const double rl = rc * g.inv_lambda;   // r_c / lambda
+                        const double phimax = TWO_PI * rl * e2 * (0.125 + e * (0.0625 + e * 0.0390625));
+                        bool ok = full && rl * e2 * e2 * e <= 1e-12 && e2 * e2 <= 1e-14 && phimax <= 4e-3;
+                        if (ok) {
+                            // reference range test holds on the whole tile with margin
+                            const double xl = pix_x(g, x0), xh = pix_x(g, cx1 - 1);
+                            const double yh = pix_y(g, y0), yl = pix_y(g, ry1 - 1);
+                            const double dxmax = fmax(fabs(xl - px), fabs(xh - px));
+                            const double dymax = fmax(fabs(yl - py), fabs(yh - py));
+                            const double dxmin = (px >= xl && px <= xh) ? 0.0 : fmin(fabs(xl - px), fabs(xh - px));
+                            const double dymin = (py >= yl && py <= yh) ? 0.0 : fmin(fabs(yl - py), fabs(yh - py));
+                            const double tol = 1e-12;
+                            ok = g.hx_coef * sqrt(dymin * dymin + zz) - dxmax > tol &&
+                                 g.hy_coef * sqrt(dxmin * dxmin + zz) - dymax > tol;
+                        }
+                        sep = ok;
+                        ex = !ok;
+                        if (ok) {
+                            double c0 = cyc0 + (rho2 / (rc + z)) * g.inv_lambda;
+                            c0 -= floor(c0);
+                            const double h = 0.5 * g.inv_lambda / rc;
+                            const double ir = 1.0 / rc2;
+                            cc[R_C0] = c0;
+                            cc[R_H] = h;
+                            cc[R_G2X] = 2.0 * dxc;
+                            cc[R_AMP] = kamp * ir;
+                            cc[R_K2] = -TWO_PI * 0.25 * h * ir;
+                            cc[R_K3] = TWO_PI * 0.125 * h * ir * ir;
+                            cc[R_K4] = -TWO_PI * 0.078125 * h * ir * ir * ir;
+                            cc[R_M1] = -ir;
+                            cc[R_M2] = ir * ir;
+                            cc[R_M3] = -ir * ir * ir;
+                            cc[R_G2Y] = 2.0 * dyc;
+                        }
+                    }
+                }
+            }
+            // deterministic compaction in point order
+            const unsigned long long bsep = __ballot(sep), bex = __ballot(ex);
+            const unsigned long long m = sep ? bsep : bex;
+            const int rank = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+            if (sep) {
+#pragma unroll
+                for (int i = 0; i < NCS; ++i) sm.c[rank][i] = cc[i];
+                sm.sbox[rank] = b;
+            }
+            if (ex) {
+                sm.d[0][rank] = px;
+                sm.d[1][rank] = py;
+                sm.d[2][rank] = z;
+                sm.d[3][rank] = cyc0;
+                sm.d[4][rank] = kamp;
+                sm.d[5][rank] = q;
+                sm.ebox[rank] = b;
+            }
+            if (lane == 0) {
+                sm.cnt[0] = __popcll(bsep);
+                sm.cnt[1] = __popcll(bex);
+            }
+        }
+        __syncthreads();
+        const int nS = __builtin_amdgcn_readfirstlane(sm.cnt[0]);
+        const int nE = __builtin_amdgcn_readfirstlane(sm.cnt[1]);
+
+        // ---- row factors: lanes j and j + 32 take half of this wave's rows of point j
+        {
+            const int j = lane & (PCD - 1);
+            const int r0 = wave * RPT + (lane >> 5) * (RPT / 2);
+            if (j < nS) {
+#pragma unroll
+                for (int k = 0; k < RPT / 2; ++k) {
+                    const int rr = r0 + k;
+                    const int row = y0 + rr;
+                    const double yyy = pix_y(g, row);
+                    double ph, ny = 0.0;
+                    bool rin = true;
+                    if (DIFF == OHX_DIFF_FRESNEL) {
+                        const double dy = yyy - sm.c[j][F_PY];
+                        ph = sm.c[j][F_Q] * dy * dy;
+                        const int4 bb = sm.sbox[j];
+                        rin = row >= bb.z && row < bb.w;
+                    } else {
+                        const double v = yyy - yc;
+                        ny = v * (sm.c[j][R_G2Y] + v);
+                        ph = sm.c[j][R_H] * ny;
+                    }
+                    ph -= floor(ph);
+                    double sn, cs;
+                    sincospi(2.0 * ph, &sn, &cs);
+                    sm.B[rr][j] = rin ? make_double2(cs, sn) : make_double2(0.0, 0.0);
+                    if (DIFF == OHX_DIFF_RS) sm.Ny[rr][j] = ny;
+                }
+            }
+        }
+        __syncthreads();
+
+        // ---- separable list --------------------------------------------------
+        for (int j = 0; j < nS; ++j) {
+            double Ar, Ai;
+            if (DIFF == OHX_DIFF_FRESNEL) {
+                const double px = sm.c[j][F_PX], cyc0 = sm.c[j][F_CYC], q = sm.c[j][F_Q];
+                const int4 bb = sm.sbox[j];
+                const double dx = xxx - px;
+                double ph = fma(q * dx, dx, cyc0);
+                ph -= floor(ph);
+                double sn, cs;
+                sincospi(2.0 * ph, &sn, &cs);
+                const double a = (col >= bb.x && col < bb.y) ? sm.c[j][F_AMP] : 0.0;
+                Ar = a * sn;
+                Ai = -a * cs;
+#pragma unroll
+                for (int k = 0; k < RPT; ++k) {
+                    const double2 bk = sm.B[wave * RPT + k][j];
+                    accr[k] = fma(Ar, bk.x, fma(-Ai, bk.y, accr[k]));
+                    acci[k] = fma(Ar, bk.y, fma(Ai, bk.x, acci[k]));
+                }
+            } else {
+                const double nx = u * (sm.c[j][R_G2X] + u);
+                double ph = fma(sm.c[j][R_H], nx, sm.c[j][R_C0]);
+                ph -= floor(ph);
+                double sn, cs;
+                sincospi(2.0 * ph, &sn, &cs);
+                const double a0 = sm.c[j][R_AMP];
+                Ar = a0 * sn;
+                Ai = -a0 * cs;
+                const double k2 = sm.c[j][R_K2], k3 = sm.c[j][R_K3], k4 = sm.c[j][R_K4];
+                const double m1 = sm.c[j][R_M1], m2 = sm.c[j][R_M2], m3 = sm.c[j][R_M3];
+#pragma unroll
+                for (int k = 0; k < RPT; ++k) {
+                    const double2 bk = sm.B[wave * RPT + k][j];
+                    const double N = nx + sm.Ny[wave * RPT + k][j];
+                    const double t = fma(fma(k4, N, k3), N, k2);
+                    const double ph2 = (N * N) * t;                 // 2 pi phi(N), radians
+                    const double p2 = ph2 * ph2;
+                    const double cr = fma(p2, fma(p2, 1.0 / 24.0, -0.5), 1.0);
+                    const double ci = ph2 * fma(p2, -1.0 / 6.0, 1.0);
+                    const double am = fma(fma(fma(m3, N, m2), N, m1), N, 1.0);
+                    const double tr = am * cr, ti = am * ci;
+                    const double sr = fma(Ar, bk.x, -Ai * bk.y), si = fma(Ar, bk.y, Ai * bk.x);
+                    accr[k] = fma(sr, tr, fma(-si, ti, accr[k]));
+                    acci[k] = fma(sr, ti, fma(si, tr, acci[k]));
+                }
+            }
+        }
+
+        // ---- per-pixel list: exact FP64 terms under the reference's mask ---
+        if (DIFF == OHX_DIFF_RS) {
+            for (int j = 0; j < nE; ++j) {
+                const int4 bb = sm.ebox[j];
+                const double qpx = sm.d[0][j], qpy = sm.d[1][j], qz = sm.d[2][j];
+                const double qc = sm.d[3][j], qk = sm.d[4][j], qq = sm.d[5][j];
+                const double zz = qz * qz;
+                const bool colin = col >= bb.x && col < bb.y;
+#pragma unroll
+                for (int k = 0; k < RPT; ++k) {
+                    const int row = rowb + k;
+                    const double yyy = pix_y(g, row);
+                    bool in = colin && row >= bb.z && row < bb.w;
+                    in = in && rs_range_ok(qpx, qpy, zz, g.hx_coef, g.hy_coef, xxx, yyy);
+                    if (in) exact_term<DIFF>(xxx - qpx, yyy - qpy, qz, qc, qk, qq, g.inv_lambda, accr[k], acci[k]);
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+            }
+        }
+        __syncthreads();
+    }
+
+    if (col < g.pn_x) {
+#pragma unroll
+        for (int k = 0; k < RPT; ++k) {
+            const int row = rowb + k;
+            if (row < g.pn_y) {
+                const int64_t idx = (int64_t)row * g.pn_x + col;
+                double2 f = field[idx];
+                f.x += accr[k];
+                f.y += acci[k];
+                field[idx] = f;
+            }
+        }
+    }
+}
+
+// LDS of the multi-job kernel, which runs both paths
+union TileSmem {
+    DirectSmem f32;
+    DirectSmem64 f64;
+};
+
 // Occupancy floor: the FP32 Fresnel loop fits three waves per SIMD, the others two.
-template <int DIFF, bool FP64>
-__global__ __launch_bounds__(NT, (DIFF == OHX_DIFF_FRESNEL && !FP64) ? 3 : 2) void k_pc_direct(
+template <int DIFF>
+__global__ __launch_bounds__(NT, (DIFF == OHX_DIFF_FRESNEL) ? 3 : 2) void k_pc_direct(
     Geometry g, PointSoA s, double2* __restrict__ field, int tiles_x)
 {
     __shared__ DirectSmem sm;
-    direct_tile<DIFF, FP64>(sm, g, s, 0, s.n, field, blockIdx.x, tiles_x);
+    direct_tile<DIFF>(sm, g, s, 0, s.n, field, blockIdx.x, tiles_x);
+}
+
+template <int DIFF>
+__global__ __launch_bounds__(NT, 2) void k_pc_direct64(Geometry g, PointSoA s, double2* __restrict__ field,
+                                                        int tiles_x)
+{
+    __shared__ DirectSmem64 sm;
+    direct_tile64<DIFF>(sm, g, s, 0, s.n, field, blockIdx.x, tiles_x);
 }
 
 // Persistent multi-job form used by the serving engine: workgroups stride
@@ -470,7 +761,7 @@ __global__ __launch_bounds__(NT, (DIFF == OHX_DIFF_FRESNEL && !FP64) ? 3 : 2) vo
 __global__ __launch_bounds__(NT) void k_pc_direct_jobs(const DirectJob* __restrict__ jobs,
                                                        const int* __restrict__ njobs_ptr)
 {
-    __shared__ DirectSmem sm;
+    __shared__ TileSmem sm;
     const int njobs = *njobs_ptr;
     const int total = njobs > 0 ? jobs[njobs - 1].item_base + jobs[njobs - 1].tiles : 0;
     for (int item = blockIdx.x; item < total; item += gridDim.x) {
@@ -479,11 +770,11 @@ __global__ __launch_bounds__(NT) void k_pc_direct_jobs(const DirectJob* __restri
         const DirectJob& jb = jobs[j];
         const int tile = item - jb.item_base;
         if (jb.g.diff == OHX_DIFF_RS) {
-            if (jb.fp64) direct_tile<OHX_DIFF_RS, true>(sm, jb.g, jb.s, jb.p0, jb.p1, jb.field, tile, jb.tiles_x);
-            else direct_tile<OHX_DIFF_RS, false>(sm, jb.g, jb.s, jb.p0, jb.p1, jb.field, tile, jb.tiles_x);
+            if (jb.fp64) direct_tile64<OHX_DIFF_RS>(sm.f64, jb.g, jb.s, jb.p0, jb.p1, jb.field, tile, jb.tiles_x);
+            else direct_tile<OHX_DIFF_RS>(sm.f32, jb.g, jb.s, jb.p0, jb.p1, jb.field, tile, jb.tiles_x);
         } else {
-            if (jb.fp64) direct_tile<OHX_DIFF_FRESNEL, true>(sm, jb.g, jb.s, jb.p0, jb.p1, jb.field, tile, jb.tiles_x);
-            else direct_tile<OHX_DIFF_FRESNEL, false>(sm, jb.g, jb.s, jb.p0, jb.p1, jb.field, tile, jb.tiles_x);
+            if (jb.fp64) direct_tile64<OHX_DIFF_FRESNEL>(sm.f64, jb.g, jb.s, jb.p0, jb.p1, jb.field, tile, jb.tiles_x);
+            else direct_tile<OHX_DIFF_FRESNEL>(sm.f32, jb.g, jb.s, jb.p0, jb.p1, jb.field, tile, jb.tiles_x);
         }
         __syncthreads();   // LDS reuse across items
     }
@@ -567,11 +858,11 @@ int launch_direct(ohx_ctx* ctx, const Geometry& g, const PointSoA& soa, double* 
     double2* f = (double2*)d_field;
     const dim3 grid((unsigned)nwg), block(NT);
     if (g.diff == OHX_DIFF_RS) {
-        if (fp64) hipLaunchKernelGGL((k_pc_direct<OHX_DIFF_RS, true>), grid, block, 0, ctx->stream, g, soa, f, tiles_x);
-        else hipLaunchKernelGGL((k_pc_direct<OHX_DIFF_RS, false>), grid, block, 0, ctx->stream, g, soa, f, tiles_x);
+        if (fp64) hipLaunchKernelGGL((k_pc_direct64<OHX_DIFF_RS>), grid, block, 0, ctx->stream, g, soa, f, tiles_x);
+        else hipLaunchKernelGGL((k_pc_direct<OHX_DIFF_RS>), grid, block, 0, ctx->stream, g, soa, f, tiles_x);
     } else {
-        if (fp64) hipLaunchKernelGGL((k_pc_direct<OHX_DIFF_FRESNEL, true>), grid, block, 0, ctx->stream, g, soa, f, tiles_x);
-        else hipLaunchKernelGGL((k_pc_direct<OHX_DIFF_FRESNEL, false>), grid, block, 0, ctx->stream, g, soa, f, tiles_x);
+        if (fp64) hipLaunchKernelGGL((k_pc_direct64<OHX_DIFF_FRESNEL>), grid, block, 0, ctx->stream, g, soa, f, tiles_x);
+        else hipLaunchKernelGGL((k_pc_direct<OHX_DIFF_FRESNEL>), grid, block, 0, ctx->stream, g, soa, f, tiles_x);
     }
     ctx->last.workgroups = nwg;
     ctx->last.kernel_kind = fp64 ? 1 : 0;

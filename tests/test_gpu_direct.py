@@ -101,6 +101,42 @@ def test_fp32_realistic_geometry_vs_exact(ctx, orc, diff):
 
 
 @pytest.mark.parametrize("diff", [0, 1])
+def test_fp64_realistic_geometry_vs_exact(ctx, orc, diff):
+    """FP64 mode at the 10k-point RS workload's geometry (1920x1080 at 8 um,
+    532 nm, z ~ 0.5 m), 256 points: every (point, tile) pair takes the
+    separable form (column factor x row factor, and for RS the short series
+    correction).  Field vs the extended-precision oracle on 3000 pixels."""
+    w = synth.WORKLOADS[1]
+    v = synth.points(w, 256)
+    g = orc.make_geom(w.pn, w.pp)
+    got = capi.generate(ctx, v, w.pn, w.pp, [w.lambdas[0]], diff, FP64)[0].ravel()
+    pix = np.random.default_rng(13).choice(w.n_pixels, 3000, replace=False)
+    want = orc.field_exact(g, v, w.lambdas[0], 0, diff, pix)
+    assert rel_l2(got[pix], want) < 1e-11
+
+
+@pytest.mark.parametrize("diff", [0, 1])
+def test_fp64_near_and_far_points_in_one_launch(ctx, orc, diff):
+    """Near points (z 2-6 mm: partial masks; for RS the series is not accurate
+    there, so their pairs take the per-pixel exact list) and far points
+    (z ~ 0.3 m: separable form) interleaved in one cloud, so both lists share
+    tiles.  Supports equal the reference restatement's; values vs the
+    extended-precision oracle."""
+    pn, pp, lam = (300, 170), (6e-6, 6e-6), 532e-9
+    near = synth.random_cloud(41 + diff, 40, 400e-6, (2e-3, 6e-3))
+    far = synth.random_cloud(43 + diff, 40, 400e-6, (0.30, 0.31))
+    v = np.concatenate([near[:20], far[:20], near[20:], far[20:]])
+    g = orc.make_geom(pn, pp)
+    want = orc.gen_restate(g, v, [lam], diff)[0]
+    got = capi.generate(ctx, v, pn, pp, [lam], diff, FP64)[0]
+    assert np.array_equal(got != 0, want != 0)
+    assert rel_l2(got, want) < 2e-9          # the restatement's own k r rounding
+    pix = np.random.default_rng(17).choice(pn[0] * pn[1], 3000, replace=False)
+    exact = orc.field_exact(g, v, lam, 0, diff, pix)
+    assert rel_l2(got.ravel()[pix], exact) < 1e-11
+
+
+@pytest.mark.parametrize("diff", [0, 1])
 def test_probe_matches_exact_oracle(ctx, orc, diff):
     w = synth.WORKLOADS[2]
     v = synth.points(w, 500)

@@ -32,7 +32,8 @@ sys.path.insert(0, ROOT)
 # path (FMA = 2, every other VALU op incl. v_sin/v_cos = 1):
 #   RS:      add, 2 fma (phase series), sin, cos, fma (amplitude), 2 fma (accumulate) = 13
 #   Fresnel: add, sin, cos, 2 fma (accumulate)                                      = 7
-FLOP_PER_PAIR = {0: 13, 1: 7}
+# FP32 separable pair loops (FMA = 2): RS mul + 6 FMA, Fresnel 4 FMA
+FLOP_PER_PAIR = {0: 13, 1: 8}
 PEAK_FP32_TFLOPS = 157.3          # published MI355X FP32 vector peak
 # FP64 mode, pair loop of the separable form (FMA = 2): RS series correction
 # + complex products, Fresnel one complex multiply-add
@@ -124,9 +125,10 @@ def cpu_baseline(w, target_s: float):
 # (rocprofv3 --pmc FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the scale factors read
 # off a 1 GiB copy kernel); profiles/k_pc_direct_rs_fresnel_100k_2160p.md.
 TRAFFIC = {0: 1.059e9, 1: 1.065e9}
-# Speed of light of the direct kernels' per-pair instruction mix on this
+# Speed of light of the direct kernels' separable pair loops (per lane and point,
+# 8 rows: column factor + 8 pairs) on this
 # hardware (openholo_amd/bin/valu_rates); profiles/valu_mfma_rates_gfx950.md.
-MIX_SOL_PAIRS_PER_S = {0: 3.97e12, 1: 5.50e12}
+MIX_SOL_PAIRS_PER_S = {0: 6.67e12, 1: 11.43e12}
 
 
 def fresnel_variants(ctx, capi, synth, Event, launches=2):
@@ -336,8 +338,9 @@ def main():
                     "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": traffic,
                     "kernel": f"k_pc_direct<{'RS' if dom == 0 else 'Fresnel'},FP32>",
                     "avg_launch_ms": round(k_ms, 3),
-                    "model": f"{fl} FP32 FLOP/pair executed (FMA=2, sin/cos/add=1); published peak "
-                             f"counts packed FP32, which is not emitted",
+                    "model": f"{fl} FP32 FLOP/pair in the separable pair loop (FMA=2); the per-lane "
+                             f"column factor (v_sin/v_cos once per point, shared by 8 rows) is not "
+                             f"counted; published peak counts packed FP32, which is not emitted",
                     "mix_sol_pairs_per_s": MIX_SOL_PAIRS_PER_S[dom],
                     "frac_of_mix_sol": round(pps / MIX_SOL_PAIRS_PER_S[dom], 4)}
     elif args.kernel in MFMA_TERMS and dom == 1:

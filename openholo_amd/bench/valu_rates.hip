@@ -38,6 +38,41 @@ __global__ __launch_bounds__(256) void k(float* out, int iters, float b, float c
   float s = 0; for (int i = 0; i < CH; ++i) s += a[i] + r[i];
   out[blockIdx.x * 256 + threadIdx.x] = s;
 }
+// The separable pair loops of the direct kernels, per lane and point over 8
+// rows: KIND 0 Fresnel (2 FMA + v_sin + v_cos per point, then 4 FMA per
+// pair), KIND 1 RS (5 VALU + v_sin + v_cos + 2 MUL per point, then MUL +
+// 6 FMA per pair).  The row factors sit in registers here (LDS in the kernel).
+template <int KIND>
+__global__ __launch_bounds__(256) void ksep(float* out, int iters, float b, float c) {
+  float pr[8], pim[8], br[8], bi[8], ny[8];
+  for (int k = 0; k < 8; ++k) {
+    pr[k] = pim[k] = 0.f;
+    br[k] = 0.5f + 0.01f * k; bi[k] = 0.25f - 0.01f * k; ny[k] = 1e-3f * k;
+  }
+  float x = threadIdx.x * 1e-4f;
+  for (int it = 0; it < iters; ++it) {
+    float sa, ca, kx = 0.f;
+    if (KIND == 0) {
+      const float f = fmaf(fmaf(x, b, c), x, c);
+      sa = __builtin_amdgcn_sinf(f); ca = __builtin_amdgcn_cosf(f);
+    } else {
+      const float fa = fmaf(fmaf(c, x, b), x, c);
+      const float am = fmaf(c, x, b);
+      kx = b * x;
+      sa = am * __builtin_amdgcn_sinf(fa); ca = am * __builtin_amdgcn_cosf(fa);
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      float r = br[k], i = bi[k];
+      if (KIND == 1) { const float e = kx * ny[k]; r = fmaf(-e, bi[k], br[k]); i = fmaf(e, br[k], bi[k]); }
+      pr[k] = fmaf(sa, r, fmaf(ca, i, pr[k]));
+      pim[k] = fmaf(ca, r, fmaf(-sa, i, pim[k]));
+    }
+    x = fmaf(x, b, c);
+  }
+  float s = 0; for (int k = 0; k < 8; ++k) s += pr[k] + pim[k];
+  out[blockIdx.x * 256 + threadIdx.x] = s;
+}
 // v_sin / v_cos on an unreduced revolutions argument against the same after
 // v_fract: counts results that differ and the largest difference, for hashed
 // arguments uniform in [-lim, lim].
@@ -101,6 +136,19 @@ int main() {
       double lanes_ops = (double)blocks * 256 * iters * CH;   // per "element op group"
       if (rep) printf("%-12s %8.3f ms  %8.2f T elem-iter/s  %8.2f T VALU-instr/s (lane)\n", names[kind], ms,
                       lanes_ops / ms / 1e9, lanes_ops * instr[kind] / ms / 1e9);
+    }
+  }
+  const char* nsep[] = {"fresnel-sep (8 rows)", "rs-sep (8 rows)"};
+  for (int kind = 0; kind < 2; ++kind) {
+    for (int rep = 0; rep < 2; ++rep) {
+      CK(hipEventRecord(e0));
+      if (kind == 0) hipLaunchKernelGGL(ksep<0>, dim3(blocks), dim3(256), 0, 0, out, iters, 0.999f, 1e-4f);
+      else hipLaunchKernelGGL(ksep<1>, dim3(blocks), dim3(256), 0, 0, out, iters, 0.999f, 1e-4f);
+      CK(hipGetLastError());
+      CK(hipEventRecord(e1)); CK(hipEventSynchronize(e1));
+      float ms; CK(hipEventElapsedTime(&ms, e0, e1));
+      const double pairs = (double)blocks * 256 * iters * 8;
+      if (rep) printf("%-28s %8.3f ms  %8.2f T pairs/s\n", nsep[kind], ms, pairs / ms / 1e9);
     }
   }
   const char* n64[] = {"fma_f64", "add_f64+fract_f64", "cvt_f32_f64+add_f32+add_f64"};

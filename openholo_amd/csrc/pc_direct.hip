@@ -23,14 +23,20 @@
 //    (ballot + mbcnt, deterministic).  Recentring keeps every FP32 phase
 //    below ~10 cycles, so FP32 rounding costs ~1e-6 rad instead of the ~1 rad
 //    a direct FP32 evaluation of k r ~ 6e6 rad would.
-//  * Fresnel phase is separable: c = phi_c + u (g_x + q u) + v (g_y + q v);
-//    the row term is tabulated in LDS per chunk, the column term is one FMA
-//    pair per lane per point, so a pair costs add, fract, v_sin, v_cos and
-//    two FMAs (12 VALU issue slots).
-//  * RS uses N = r^2 - r_c^2 = (2 dx_c u + u^2) + (2 dy_c v + v^2) and the
-//    series r - r_c = N/(2 r_c) - N^2/(8 r_c^3) (valid bound checked per pair
-//    in FP64), amplitude z/r^2 = (z/r_c^2)(1 - N/r_c^2): 15 slots per pair,
-//    no sqrt or reciprocal in the loop.
+//  * Fresnel phase is separable: c = phi_c + u (g_x + q u) + v (g_y + q v),
+//    so a term is A(col) B(row) with A = (sin, cos)(2 pi (phi_c + u (g_x + q u)))
+//    (one v_sin/v_cos per lane and point) and B = amp e^{j 2pi v (g_y + q v)}
+//    tabulated in LDS per chunk (zero outside the box rows; A is zeroed
+//    outside the box columns): a pair is one complex multiply-add, 4 FMAs.
+//  * RS uses N = r^2 - r_c^2 = (2 dx_c u + u^2) + (2 dy_c v + v^2) = n_x + N_y
+//    and the series r - r_c = N/(2 r_c) - N^2/(8 r_c^3) (valid bound checked
+//    per pair in FP64), amplitude z/r^2 = (z/r_c^2)(1 - N/r_c^2): no sqrt or
+//    reciprocal in the loop.  The phase splits into a column part, a row
+//    part and the cross term -2 c_2 n_x N_y cycles (c_2 = 1/(8 r_c^3 lambda)),
+//    at most MAX_EPS rad, taken as a first-order factor 1 + j eps (the
+//    amplitude's cross term is below 1e-7): 7 FMAs per pair, no v_sin or
+//    v_cos.  Pairs with a larger cross term (FASTNS) evaluate the phase per
+//    pixel: add, two FMAs, v_sin, v_cos, three FMAs.
 //  * FP32 partial sums per chunk are folded into FP64 accumulators; the
 //    field (interleaved FP64 complex, the reference's Complex<Real> layout)
 //    is read-modify-written once per pixel at the end, 1 KB coalesced per row.
@@ -49,9 +55,18 @@ constexpr int TH = WAVES * RPT;   // tile height
 constexpr int NT = TW * WAVES;    // threads per workgroup
 constexpr int PCH = 128;          // points per LDS chunk
 
-enum { CLS_OUT = 0, CLS_FAST = 1, CLS_MASKED = 2, CLS_EXACT = 3 };
+// FAST: whole tile inside the mask, separable form; MASKED: per-pixel mask;
+// EXACT: FP64 terms; FASTNS (RS): whole tile inside the mask, but the
+// cross term of the separable form is too large, so the phase is evaluated
+// per pixel.
+enum { CLS_OUT = 0, CLS_FAST = 1, CLS_MASKED = 2, CLS_EXACT = 3, CLS_FASTNS = 4 };
+
+// Largest cross-term phase (rad) of the separable RS form: e^{j eps} is taken
+// as 1 + j eps, which is within eps^2/2 <= 4.5e-8.
+constexpr double MAX_EPS = 3e-4;
 
 constexpr double INV_2PI = 0.15915494309189533576888376337251436;
+constexpr double TWO_PI = 6.283185307179586476925286766559;
 // Largest phase excursion (cycles) over a tile that the FP32 path accepts:
 // FP32 rounding then stays below ~4e-6 cycles per term; larger spans (points
 // a few mm from the hologram) take the FP64 path.
@@ -187,10 +202,13 @@ __device__ inline void exact_term(double dx, double dy, double z, double cyc0, d
 }
 
 // LDS of one workgroup (one tile): point constants of the current chunk
+template <int DIFF>
 struct DirectSmem {
     float4 c0[PCH];
     float4 c1[PCH];
-    float4 R[WAVES][PCH][2];
+    float2 R[WAVES][PCH][RPT];   // row factor B(row) of the separable form
+    float Ny[WAVES][PCH][DIFF == OHX_DIFF_RS ? RPT : 1];   // RS: N_y(row)
+    float mu[PCH];               // RS: -1 / r_c^2
     int4 box[PCH];
     double d[6][PCH];   // px, py, z, cyc0, kamp, q (slow list)
     int cls[PCH];
@@ -202,7 +220,7 @@ struct DirectSmem {
 // call would pin that kernel at the callees' worst-case register count
 // (one wave per SIMD) and add call overhead per tile.
 template <int DIFF>
-__device__ __attribute__((always_inline)) inline void direct_tile(DirectSmem& sm, const Geometry& g, const PointSoA& s, int64_t p_begin,
+__device__ __attribute__((always_inline)) inline void direct_tile(DirectSmem<DIFF>& sm, const Geometry& g, const PointSoA& s, int64_t p_begin,
                            int64_t p_end, double2* __restrict__ field, int tile, int tiles_x)
 {
     float4* s_c0 = sm.c0;
@@ -226,6 +244,10 @@ __device__ __attribute__((always_inline)) inline void direct_tile(DirectSmem& sm
     const float uu = u * u;
     const double xxx = pix_x(g, col);
     const float umax = (float)(TW / 2) * (float)g.pp_x, vmax = (float)(TH / 2) * (float)g.pp_y;
+    // v = y_row - y_c of this wave's rows
+    float vr[RPT];
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) vr[k] = (float)(-(double)(wave * RPT + k - TH / 2) * g.pp_y);
 
     double accr[RPT], acci[RPT];
 #pragma unroll
@@ -235,7 +257,7 @@ __device__ __attribute__((always_inline)) inline void direct_tile(DirectSmem& sm
         // ---- classify and prepare one point per thread --------------------
         int cls = CLS_OUT;
         float4 c0 = make_float4(0, 0, 0, 0), c1 = make_float4(0, 0, 0, 0);
-        float rowc = 0.f;   // Fresnel: g_y; RS: 2 dy_c
+        float mu = 0.f;
         int4 b = make_int4(0, 0, 0, 0);
         double px = 0, py = 0, z = 0, cyc0 = 0, kamp = 0, q = 0;
         const int64_t p = base + tid;
@@ -258,8 +280,8 @@ __device__ __attribute__((always_inline)) inline void direct_tile(DirectSmem& sm
                     double ph = cyc0 + (dxc * dxc + dyc * dyc) * q;
                     ph -= floor(ph);
                     const float qf = (float)q;
-                    rowc = (float)(2.0 * dyc * q);
                     c0 = make_float4((float)ph, (float)(2.0 * dxc * q), qf, (float)kamp);
+                    c1 = make_float4((float)(2.0 * dyc * q), 0.f, 0.f, 0.f);
                 } else {
                     const double zz = z * z;
                     const double rho2 = dxc * dxc + dyc * dyc;
@@ -289,16 +311,20 @@ __device__ __attribute__((always_inline)) inline void direct_tile(DirectSmem& sm
                             range_ok = g.hx_coef * sqrt(dymin * dymin + zz) - dxmax > tol &&
                                        g.hy_coef * sqrt(dxmin * dxmin + zz) - dymax > tol;
                         }
-                        cls = range_ok ? CLS_FAST : CLS_MASKED;
                         double ph = cyc0 + (rho2 / (rc + z)) * g.inv_lambda;
                         ph -= floor(ph);
                         const double hp = 0.5 * g.inv_lambda / rc;          // 1/(2 r_c lambda)
                         const double c2 = hp / (4.0 * rc2);                  // 1/(8 r_c^3 lambda)
                         const double az = kamp / rc2;                        // a z/(lambda r_c^2)
                         const double m = az / rc2;
+                        // cross term -2 c2 N_x N_y of the phase (rad) over the tile
+                        const double nxm = 2.0 * umax * fabs(dxc) + (double)umax * umax;
+                        const double nym = 2.0 * vmax * fabs(dyc) + (double)vmax * vmax;
+                        const bool sep = 2.0 * TWO_PI * c2 * nxm * nym <= MAX_EPS;
+                        cls = range_ok ? (sep ? CLS_FAST : CLS_FASTNS) : CLS_MASKED;
                         c0 = make_float4((float)ph, (float)(2.0 * dxc), (float)(-c2), (float)hp);
-                        c1 = make_float4((float)(-m), (float)az, 0.f, 0.f);
-                        rowc = (float)(2.0 * dyc);
+                        c1 = make_float4((float)(-m), (float)az, (float)(-2.0 * TWO_PI * c2), (float)(2.0 * dyc));
+                        mu = (float)(-1.0 / rc2);
                     }
                 }
             }
@@ -329,24 +355,13 @@ __device__ __attribute__((always_inline)) inline void direct_tile(DirectSmem& sm
             const int rank = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
                                                        __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
             const int slot = (cls == CLS_FAST) ? offF + rank : PCH - 1 - (offS + rank);
+            s_box[slot] = b;
             if (cls != CLS_EXACT) {
                 s_c0[slot] = c0;
                 s_c1[slot] = c1;
-                // row terms for the 32 rows of the tile: Fresnel v (g_y + q v),
-                // RS v (2 dy_c + v), with v = y_row - y_c
-#pragma unroll 2
-                for (int h = 0; h < TH / 4; ++h) {
-                    float t[4];
-#pragma unroll
-                    for (int e = 0; e < 4; ++e) {
-                        const float v = (float)(-(double)(4 * h + e - TH / 2) * g.pp_y);
-                        t[e] = (DIFF == OHX_DIFF_FRESNEL) ? v * fmaf(c0.z, v, rowc) : v * (rowc + v);
-                    }
-                    s_R[h / 2][slot][h & 1] = make_float4(t[0], t[1], t[2], t[3]);
-                }
+                if (DIFF == OHX_DIFF_RS) sm.mu[slot] = mu;
             }
             if (cls >= CLS_MASKED) {
-                s_box[slot] = b;
                 s_cls[slot] = cls;
                 s_d[0][slot] = px;
                 s_d[1][slot] = py;
@@ -358,32 +373,86 @@ __device__ __attribute__((always_inline)) inline void direct_tile(DirectSmem& sm
         }
         __syncthreads();
 
-        // ---- fast list: whole tile inside the mask --------------------------
+        // ---- row table: each wave fills its own 8 rows for every listed point
+        // Fresnel: B = amp (cos, sin)(2 pi v (g_y + q v)), zero outside the
+        // box rows.  RS: N_y = v (2 dy_c + v) for every non-EXACT point, and
+        // for the separable list B = (1 + mu N_y) (cos, sin)(2 pi N_y (h + c_z N_y)).
+        for (int jj = lane; jj < nF + nS; jj += TW) {
+            const int slot = jj < nF ? jj : PCH - 1 - (jj - nF);
+            if (jj >= nF && s_cls[slot] == CLS_EXACT) continue;
+            const float4 c = s_c0[slot], d = s_c1[slot];
+            float2 o[RPT];
+            if (DIFF == OHX_DIFF_FRESNEL) {
+                const int4 bb = s_box[slot];
+#pragma unroll
+                for (int k = 0; k < RPT; ++k) {
+                    const float t = vr[k] * fmaf(c.z, vr[k], d.x);
+                    const int row = rowb + k;
+                    const float a = (row >= bb.z && row < bb.w) ? c.w : 0.f;
+                    o[k] = make_float2(a * cos_rev(t), a * sin_rev(t));
+                }
+            } else {
+                float ny[RPT];
+#pragma unroll
+                for (int k = 0; k < RPT; ++k) ny[k] = vr[k] * (d.w + vr[k]);
+                float4* dn = (float4*)&sm.Ny[wave][slot][0];
+                dn[0] = make_float4(ny[0], ny[1], ny[2], ny[3]);
+                dn[1] = make_float4(ny[4], ny[5], ny[6], ny[7]);
+                if (jj >= nF) continue;
+                const float m1 = sm.mu[slot];
+#pragma unroll
+                for (int k = 0; k < RPT; ++k) {
+                    const float t = ny[k] * fmaf(c.z, ny[k], c.w);
+                    const float a = fmaf(m1, ny[k], 1.f);
+                    o[k] = make_float2(a * cos_rev(t), a * sin_rev(t));
+                }
+            }
+            float4* dst = (float4*)&s_R[wave][slot][0];
+#pragma unroll
+            for (int i = 0; i < RPT / 2; ++i) dst[i] = make_float4(o[2 * i].x, o[2 * i].y, o[2 * i + 1].x, o[2 * i + 1].y);
+        }
+        __syncthreads();
+
+        // ---- fast list: whole tile inside the mask, separable form ----------
+        // Fresnel: sum amp sin/cos(f_col + t_row) = A(col) x B(row), 4 FMAs per
+        // pair with one v_sin/v_cos per lane and point.  RS: the phase
+        // c0 + c_w N + c_z N^2 and amplitude az (1 + mu N), N = n_x + N_y, give
+        // A(col) = az (1 + mu n_x) e^{j 2pi (c0 + c_w n_x + c_z n_x^2)} and the
+        // tabulated B(row); the cross term 2 c_z n_x N_y enters as
+        // B (1 + j eps), eps = k_x N_y (|eps| <= MAX_EPS), and the amplitude's
+        // mu^2 n_x N_y is below 1e-7: 7 FMAs per pair, no v_sin/v_cos.
         float pr[RPT], pim[RPT];
 #pragma unroll
         for (int k = 0; k < RPT; ++k) pr[k] = pim[k] = 0.f;
         for (int j = 0; j < nF; ++j) {
             const float4 c = s_c0[j];
-            const float4 ra = s_R[wave][j][0], rb = s_R[wave][j][1];
-            const float Rv[RPT] = {ra.x, ra.y, ra.z, ra.w, rb.x, rb.y, rb.z, rb.w};
+            const float4* Rp = (const float4*)&s_R[wave][j][0];
+            const float4 r0 = Rp[0], r1 = Rp[1], r2 = Rp[2], r3 = Rp[3];
+            const float Rx[RPT] = {r0.x, r0.z, r1.x, r1.z, r2.x, r2.z, r3.x, r3.z};
+            const float Ry[RPT] = {r0.y, r0.w, r1.y, r1.w, r2.y, r2.w, r3.y, r3.w};
             if (DIFF == OHX_DIFF_FRESNEL) {
-                const float bse = fmaf(fmaf(c.z, u, c.y), u, c.x);
+                const float f = fmaf(fmaf(c.z, u, c.y), u, c.x);
+                const float sa = sin_rev(f), ca = cos_rev(f);
 #pragma unroll
                 for (int k = 0; k < RPT; ++k) {
-                    const float f = bse + Rv[k];
-                    pr[k] = fmaf(c.w, sin_rev(f), pr[k]);
-                    pim[k] = fmaf(c.w, cos_rev(f), pim[k]);
+                    pr[k] = fmaf(sa, Rx[k], fmaf(ca, Ry[k], pr[k]));
+                    pim[k] = fmaf(ca, Rx[k], fmaf(-sa, Ry[k], pim[k]));
                 }
             } else {
                 const float4 d = s_c1[j];
+                const float4 n0 = *(const float4*)&sm.Ny[wave][j][0], n1 = *(const float4*)&sm.Ny[wave][j][4];
+                const float Nv[RPT] = {n0.x, n0.y, n0.z, n0.w, n1.x, n1.y, n1.z, n1.w};
                 const float nx = fmaf(c.y, u, uu);
+                const float fa = fmaf(fmaf(c.z, nx, c.w), nx, c.x);
+                const float am = fmaf(d.x, nx, d.y);
+                const float kx = d.z * nx;
+                const float sa = am * sin_rev(fa), ca = am * cos_rev(fa);
 #pragma unroll
                 for (int k = 0; k < RPT; ++k) {
-                    const float N = nx + Rv[k];
-                    const float f = fmaf(fmaf(c.z, N, c.w), N, c.x);
-                    const float amp = fmaf(d.x, N, d.y);
-                    pr[k] = fmaf(amp, sin_rev(f), pr[k]);
-                    pim[k] = fmaf(amp, cos_rev(f), pim[k]);
+                    const float e = kx * Nv[k];
+                    const float br = fmaf(-e, Ry[k], Rx[k]), bi = fmaf(e, Rx[k], Ry[k]);
+                    pr[k] = fmaf(sa, br, fmaf(ca, bi, pr[k]));
+                    pim[k] = fmaf(ca, br, fmaf(-sa, bi, pim[k]));
                 }
             }
         }
@@ -394,43 +463,70 @@ __device__ __attribute__((always_inline)) inline void direct_tile(DirectSmem& sm
             const int4 bb = s_box[slot];
             const double qpx = s_d[0][slot], qpy = s_d[1][slot], qz = s_d[2][slot];
             const bool colin = col >= bb.x && col < bb.y;
-            if (cl == CLS_MASKED) {
+            if (cl == CLS_MASKED && DIFF == OHX_DIFF_FRESNEL) {
+                // separable mask: B is zero outside the box rows, A outside its columns
                 const float4 c = s_c0[slot];
-                const float* Rs = (const float*)&s_R[wave][slot][0];
+                const float2* Rs = &s_R[wave][slot][0];
+                const float f = fmaf(fmaf(c.z, u, c.y), u, c.x);
+                const float sa = colin ? sin_rev(f) : 0.f, ca = colin ? cos_rev(f) : 0.f;
+#pragma unroll
+                for (int k = 0; k < RPT; ++k) {
+                    const float2 bk = Rs[k];
+                    pr[k] = fmaf(sa, bk.x, fmaf(ca, bk.y, pr[k]));
+                    pim[k] = fmaf(ca, bk.x, fmaf(-sa, bk.y, pim[k]));
+                }
+            } else if (DIFF == OHX_DIFF_RS && cl == CLS_FASTNS) {
+                // whole tile in the mask, phase per pixel
+                const float4 c = s_c0[slot];
+                const float4 d = s_c1[slot];
+                const float* Ns = &sm.Ny[wave][slot][0];
+                const float nx = fmaf(c.y, u, uu);
+#pragma unroll
+                for (int k = 0; k < RPT; ++k) {
+                    const float N = nx + Ns[k];
+                    const float f = fmaf(fmaf(c.z, N, c.w), N, c.x);
+                    const float amp = fmaf(d.x, N, d.y);
+                    pr[k] = fmaf(amp, sin_rev(f), pr[k]);
+                    pim[k] = fmaf(amp, cos_rev(f), pim[k]);
+                }
+            } else if (cl == CLS_MASKED) {
+                const float4 c = s_c0[slot];
+                const float* Ns = &sm.Ny[wave][slot][0];
                 const float4 d = s_c1[slot];
                 const double zz = qz * qz;
 #pragma unroll
                 for (int k = 0; k < RPT; ++k) {
                     const int row = rowb + k;
                     bool in = colin && row >= bb.z && row < bb.w;
-                    if (DIFF == OHX_DIFF_RS)
-                        in = in && rs_range_ok(qpx, qpy, zz, g.hx_coef, g.hy_coef, xxx, pix_y(g, row));
-                    float f, amp;
-                    if (DIFF == OHX_DIFF_FRESNEL) {
-                        f = fmaf(fmaf(c.z, u, c.y), u, c.x) + Rs[k];
-                        amp = c.w;
-                    } else {
-                        const float N = fmaf(c.y, u, uu) + Rs[k];
-                        f = fmaf(fmaf(c.z, N, c.w), N, c.x);
-                        amp = fmaf(d.x, N, d.y);
-                    }
+                    in = in && rs_range_ok(qpx, qpy, zz, g.hx_coef, g.hy_coef, xxx, pix_y(g, row));
+                    const float N = fmaf(c.y, u, uu) + Ns[k];
+                    const float f = fmaf(fmaf(c.z, N, c.w), N, c.x);
+                    float amp = fmaf(d.x, N, d.y);
                     amp = in ? amp : 0.f;
                     pr[k] = fmaf(amp, sin_rev(f), pr[k]);
                     pim[k] = fmaf(amp, cos_rev(f), pim[k]);
                     __builtin_amdgcn_sched_barrier(0);  // keep one row live at a time
                 }
             } else {
+                // FP64 terms, one row at a time (a rolled loop keeps the kernel's
+                // register count at the fast path's; the row's term is added to
+                // its accumulator by selection)
                 const double zz = qz * qz;
                 const double qc = s_d[3][slot], qk = s_d[4][slot], qq = s_d[5][slot];
-#pragma unroll
+#pragma unroll 1
                 for (int k = 0; k < RPT; ++k) {
                     const int row = rowb + k;
                     const double yyy = pix_y(g, row);
                     bool in = colin && row >= bb.z && row < bb.w;
                     if (DIFF == OHX_DIFF_RS)
                         in = in && rs_range_ok(qpx, qpy, zz, g.hx_coef, g.hy_coef, xxx, yyy);
-                    if (in) exact_term<DIFF>(xxx - qpx, yyy - qpy, qz, qc, qk, qq, g.inv_lambda, accr[k], acci[k]);
-                    __builtin_amdgcn_sched_barrier(0);
+                    double re = 0.0, im = 0.0;
+                    if (in) exact_term<DIFF>(xxx - qpx, yyy - qpy, qz, qc, qk, qq, g.inv_lambda, re, im);
+#pragma unroll
+                    for (int kk = 0; kk < RPT; ++kk) {
+                        accr[kk] += (kk == k) ? re : 0.0;
+                        acci[kk] += (kk == k) ? im : 0.0;
+                    }
                 }
             }
         }
@@ -478,12 +574,30 @@ __device__ __attribute__((always_inline)) inline void direct_tile(DirectSmem& sm
 //    to the per-pixel list (exact_term under the reference's mask).
 constexpr int PCD = 32;   // points per chunk on the FP64 path
 constexpr int NCS = 12;   // constants per point of the separable list
-constexpr double TWO_PI = 6.283185307179586476925286766559;
 // separable-list constant slots
 enum { F_PX = 0, F_PY, F_CYC, F_Q, F_AMP };
 enum { R_C0 = 0, R_H, R_G2X, R_AMP, R_K2, R_K3, R_K4, R_M1, R_M2, R_M3, R_G2Y };
 
+constexpr int NTAB = 256;  // angles of the FP64 sin/cos table
+
+// sin and cos of 2 pi t, t in [0, 1], in FP64: the nearest table angle
+// 2 pi i / NTAB plus Taylor series of the remainder |x| <= pi / NTAB
+// (dropped terms below 2e-20), combined by angle addition; about 17 FP64
+// instructions and one LDS read against ~40 for sincospi.
+__device__ inline void sincos2pi(double t, const double2* __restrict__ tab, double& s, double& c)
+{
+    const double ti = rint(t * NTAB);
+    const double x = fma(ti, -1.0 / NTAB, t) * TWO_PI;
+    const double2 a = tab[(int)ti & (NTAB - 1)];   // (cos, sin)(2 pi i / NTAB)
+    const double x2 = x * x;
+    const double cb = fma(x2, fma(x2, fma(x2, -1.0 / 720.0, 1.0 / 24.0), -0.5), 1.0);
+    const double sb = x * fma(x2, fma(x2, fma(x2, -1.0 / 5040.0, 1.0 / 120.0), -1.0 / 6.0), 1.0);
+    s = fma(a.y, cb, a.x * sb);
+    c = fma(a.x, cb, -(a.y * sb));
+}
+
 struct DirectSmem64 {
+    double2 tab[NTAB];     // (cos, sin)(2 pi i / NTAB)
     double2 B[TH][PCD];    // row factors B(row) of the separable list
     double Ny[TH][PCD];    // RS: N_y(row)
     double c[PCD][NCS];    // separable list: per-point constants
@@ -509,6 +623,13 @@ __device__ __attribute__((always_inline)) inline void direct_tile64(DirectSmem64
     const double u = xxx - xc;
     // |u| <= 32 pitches and |v| <= 16 pitches over the tile
     const double umax = (double)(TW / 2) * g.pp_x, vmax = (double)(TH / 2) * g.pp_y;
+
+    // sin/cos table (first read after the classification barrier below)
+    for (int i = tid; i < NTAB; i += NT) {
+        double sn, cs;
+        sincospi(2.0 * i / NTAB, &sn, &cs);
+        sm.tab[i] = make_double2(cs, sn);
+    }
 
     double accr[RPT], acci[RPT];
 #pragma unroll
@@ -638,7 +759,7 @@ __device__ __attribute__((always_inline)) inline void direct_tile64(DirectSmem64
                     }
                     ph -= floor(ph);
                     double sn, cs;
-                    sincospi(2.0 * ph, &sn, &cs);
+                    sincos2pi(ph, sm.tab, sn, cs);
                     sm.B[rr][j] = rin ? make_double2(cs, sn) : make_double2(0.0, 0.0);
                     if (DIFF == OHX_DIFF_RS) sm.Ny[rr][j] = ny;
                 }
@@ -656,7 +777,7 @@ __device__ __attribute__((always_inline)) inline void direct_tile64(DirectSmem64
                 double ph = fma(q * dx, dx, cyc0);
                 ph -= floor(ph);
                 double sn, cs;
-                sincospi(2.0 * ph, &sn, &cs);
+                sincos2pi(ph, sm.tab, sn, cs);
                 const double a = (col >= bb.x && col < bb.y) ? sm.c[j][F_AMP] : 0.0;
                 Ar = a * sn;
                 Ai = -a * cs;
@@ -671,7 +792,7 @@ __device__ __attribute__((always_inline)) inline void direct_tile64(DirectSmem64
                 double ph = fma(sm.c[j][R_H], nx, sm.c[j][R_C0]);
                 ph -= floor(ph);
                 double sn, cs;
-                sincospi(2.0 * ph, &sn, &cs);
+                sincos2pi(ph, sm.tab, sn, cs);
                 const double a0 = sm.c[j][R_AMP];
                 Ar = a0 * sn;
                 Ai = -a0 * cs;
@@ -734,7 +855,8 @@ __device__ __attribute__((always_inline)) inline void direct_tile64(DirectSmem64
 
 // LDS of the multi-job kernel, which runs both paths
 union TileSmem {
-    DirectSmem f32;
+    DirectSmem<OHX_DIFF_RS> rs;
+    DirectSmem<OHX_DIFF_FRESNEL> fr;
     DirectSmem64 f64;
 };
 
@@ -743,7 +865,7 @@ template <int DIFF>
 __global__ __launch_bounds__(NT, (DIFF == OHX_DIFF_FRESNEL) ? 3 : 2) void k_pc_direct(
     Geometry g, PointSoA s, double2* __restrict__ field, int tiles_x)
 {
-    __shared__ DirectSmem sm;
+    __shared__ DirectSmem<DIFF> sm;
     direct_tile<DIFF>(sm, g, s, 0, s.n, field, blockIdx.x, tiles_x);
 }
 
@@ -771,10 +893,10 @@ __global__ __launch_bounds__(NT) void k_pc_direct_jobs(const DirectJob* __restri
         const int tile = item - jb.item_base;
         if (jb.g.diff == OHX_DIFF_RS) {
             if (jb.fp64) direct_tile64<OHX_DIFF_RS>(sm.f64, jb.g, jb.s, jb.p0, jb.p1, jb.field, tile, jb.tiles_x);
-            else direct_tile<OHX_DIFF_RS>(sm.f32, jb.g, jb.s, jb.p0, jb.p1, jb.field, tile, jb.tiles_x);
+            else direct_tile<OHX_DIFF_RS>(sm.rs, jb.g, jb.s, jb.p0, jb.p1, jb.field, tile, jb.tiles_x);
         } else {
             if (jb.fp64) direct_tile64<OHX_DIFF_FRESNEL>(sm.f64, jb.g, jb.s, jb.p0, jb.p1, jb.field, tile, jb.tiles_x);
-            else direct_tile<OHX_DIFF_FRESNEL>(sm.f32, jb.g, jb.s, jb.p0, jb.p1, jb.field, tile, jb.tiles_x);
+            else direct_tile<OHX_DIFF_FRESNEL>(sm.fr, jb.g, jb.s, jb.p0, jb.p1, jb.field, tile, jb.tiles_x);
         }
         __syncthreads();   // LDS reuse across items
     }

@@ -17,7 +17,8 @@ struct ohx_ctx {
     // per-point SoA scratch (grown on demand)
     void* pts = nullptr;
     size_t pts_bytes = 0;
-    // 16-byte min/max scratch of the normaliser
+    // small device scratch: [0..1] min/max of the normaliser, [4] pixels in
+    // the mask boxes of the last launch, [5] FP32 direct kernel left EXACT pairs
     unsigned long long* minmax = nullptr;
     // split-K partial sums of the matrix-core Fresnel path (grown on demand)
     void* gemm_scratch = nullptr;
@@ -27,6 +28,11 @@ struct ohx_ctx {
 };
 
 namespace ohx {
+
+// Points per LDS chunk of the FP32 direct kernel: its FP32 partial sums are
+// folded into FP64 at chunk ends, so a point slice that is a whole number of
+// chunks sums exactly as the one-shot launch does.
+constexpr int DIRECT_CHUNK = 120;
 
 // Per-point constants produced by the prep kernel (structure of arrays).
 struct PointSoA {

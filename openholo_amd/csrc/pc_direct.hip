@@ -53,7 +53,7 @@ constexpr int WAVES = 4;
 constexpr int RPT = 8;            // rows per lane
 constexpr int TH = WAVES * RPT;   // tile height
 constexpr int NT = TW * WAVES;    // threads per workgroup
-constexpr int PCH = 128;          // points per LDS chunk
+constexpr int PCH = DIRECT_CHUNK; // points per LDS chunk (three workgroups per CU fit the LDS)
 
 // FAST: whole tile inside the mask, separable form; MASKED: per-pixel mask;
 // EXACT: FP64 terms; FASTNS (RS): whole tile inside the mask, but the
@@ -210,24 +210,36 @@ struct DirectSmem {
     float Ny[WAVES][PCH][DIFF == OHX_DIFF_RS ? RPT : 1];   // RS: N_y(row)
     float mu[PCH];               // RS: -1 / r_c^2
     int4 box[PCH];
-    double d[6][PCH];   // px, py, z, cyc0, kamp, q (slow list)
     int cls[PCH];
     int cnt[2][WAVES];
 };
+
+// What one direct_tile instance does with the (point, tile) pairs that need
+// FP64 terms (EXACT class): EX_INLINE evaluates them in place (multi-job
+// kernel); EX_FLAG skips them and raises *exact_flag, which keeps the FP64
+// code and its registers out of the FP32 kernel (three waves per SIMD);
+// EX_ONLY evaluates nothing but them (the follow-up kernel).
+enum { EX_INLINE = 0, EX_FLAG = 1, EX_ONLY = 2 };
+
+// FP64 point data of a slow-list slot (px, py, z, cyc0, kamp, q) sits in that
+// slot's row-table entries of wave 0, which these slots leave unused: the
+// RS slow classes use N_y only, and Fresnel fills B only for non-EXACT slots,
+// reading no FP64 data for them.
+__device__ inline double* slot_fp64(float2* row_entry) { return reinterpret_cast<double*>(row_entry); }
 
 // Accumulates points [p_begin, p_end) of s into one 64 x 32 tile of field.
 // Always inlined: the multi-job kernel calls all four variants, and a real
 // call would pin that kernel at the callees' worst-case register count
 // (one wave per SIMD) and add call overhead per tile.
-template <int DIFF>
+template <int DIFF, int EX>
 __device__ __attribute__((always_inline)) inline void direct_tile(DirectSmem<DIFF>& sm, const Geometry& g, const PointSoA& s, int64_t p_begin,
-                           int64_t p_end, double2* __restrict__ field, int tile, int tiles_x)
+                           int64_t p_end, double2* __restrict__ field, int tile, int tiles_x,
+                           unsigned long long* __restrict__ exact_flag)
 {
     float4* s_c0 = sm.c0;
     float4* s_c1 = sm.c1;
     auto& s_R = sm.R;
     int4* s_box = sm.box;
-    auto& s_d = sm.d;
     int* s_cls = sm.cls;
     auto& s_cnt = sm.cnt;
 
@@ -252,6 +264,7 @@ __device__ __attribute__((always_inline)) inline void direct_tile(DirectSmem<DIF
     double accr[RPT], acci[RPT];
 #pragma unroll
     for (int k = 0; k < RPT; ++k) accr[k] = acci[k] = 0.0;
+    bool touched = false;   // EX_ONLY: some slow-list point on this tile
 
     for (int64_t base = p_begin; base < p_end; base += PCH) {
         // ---- classify and prepare one point per thread --------------------
@@ -329,6 +342,11 @@ __device__ __attribute__((always_inline)) inline void direct_tile(DirectSmem<DIF
                 }
             }
         }
+        if (EX == EX_FLAG && cls == CLS_EXACT) {
+            *exact_flag = 1ull;   // left to the follow-up kernel
+            cls = CLS_OUT;
+        }
+        if (EX == EX_ONLY && cls != CLS_EXACT) cls = CLS_OUT;
         // ---- deterministic compaction (point order) ------------------------
         const unsigned long long bf = __ballot(cls == CLS_FAST);
         const unsigned long long bs = __ballot(cls >= CLS_MASKED);
@@ -350,6 +368,7 @@ __device__ __attribute__((always_inline)) inline void direct_tile(DirectSmem<DIF
         // the list lengths are workgroup-uniform: scalar loop control below
         nF = __builtin_amdgcn_readfirstlane(nF);
         nS = __builtin_amdgcn_readfirstlane(nS);
+        touched |= nS > 0;
         if (cls != CLS_OUT) {
             const unsigned long long m = (cls == CLS_FAST) ? bf : bs;
             const int rank = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
@@ -363,12 +382,15 @@ __device__ __attribute__((always_inline)) inline void direct_tile(DirectSmem<DIF
             }
             if (cls >= CLS_MASKED) {
                 s_cls[slot] = cls;
-                s_d[0][slot] = px;
-                s_d[1][slot] = py;
-                s_d[2][slot] = z;
-                s_d[3][slot] = cyc0;
-                s_d[4][slot] = kamp;
-                s_d[5][slot] = q;
+                if (cls == CLS_EXACT || DIFF == OHX_DIFF_RS) {
+                    double* fd = slot_fp64(&s_R[0][slot][0]);
+                    fd[0] = px;
+                    fd[1] = py;
+                    fd[2] = z;
+                    fd[3] = cyc0;
+                    fd[4] = kamp;
+                    fd[5] = q;
+                }
             }
         }
         __syncthreads();
@@ -461,7 +483,7 @@ __device__ __attribute__((always_inline)) inline void direct_tile(DirectSmem<DIF
             const int slot = PCH - 1 - j;
             const int cl = s_cls[slot];
             const int4 bb = s_box[slot];
-            const double qpx = s_d[0][slot], qpy = s_d[1][slot], qz = s_d[2][slot];
+            const double* fd = slot_fp64(&s_R[0][slot][0]);
             const bool colin = col >= bb.x && col < bb.y;
             if (cl == CLS_MASKED && DIFF == OHX_DIFF_FRESNEL) {
                 // separable mask: B is zero outside the box rows, A outside its columns
@@ -493,6 +515,7 @@ __device__ __attribute__((always_inline)) inline void direct_tile(DirectSmem<DIF
                 const float4 c = s_c0[slot];
                 const float* Ns = &sm.Ny[wave][slot][0];
                 const float4 d = s_c1[slot];
+                const double qpx = fd[0], qpy = fd[1], qz = fd[2];
                 const double zz = qz * qz;
 #pragma unroll
                 for (int k = 0; k < RPT; ++k) {
@@ -507,12 +530,13 @@ __device__ __attribute__((always_inline)) inline void direct_tile(DirectSmem<DIF
                     pim[k] = fmaf(amp, cos_rev(f), pim[k]);
                     __builtin_amdgcn_sched_barrier(0);  // keep one row live at a time
                 }
-            } else {
+            } else if (EX != EX_FLAG) {
                 // FP64 terms, one row at a time (a rolled loop keeps the kernel's
                 // register count at the fast path's; the row's term is added to
                 // its accumulator by selection)
+                const double qpx = fd[0], qpy = fd[1], qz = fd[2];
+                const double qc = fd[3], qk = fd[4], qq = fd[5];
                 const double zz = qz * qz;
-                const double qc = s_d[3][slot], qk = s_d[4][slot], qq = s_d[5][slot];
 #pragma unroll 1
                 for (int k = 0; k < RPT; ++k) {
                     const int row = rowb + k;
@@ -538,6 +562,7 @@ __device__ __attribute__((always_inline)) inline void direct_tile(DirectSmem<DIF
         __syncthreads();
     }
 
+    if (EX == EX_ONLY && !touched) return;   // uniform: nothing to add
     if (col < g.pn_x) {
 #pragma unroll
         for (int k = 0; k < RPT; ++k) {
@@ -860,13 +885,25 @@ union TileSmem {
     DirectSmem64 f64;
 };
 
-// Occupancy floor: the FP32 Fresnel loop fits three waves per SIMD, the others two.
+// FP32 kernel: three waves per SIMD (registers and LDS both fit); pairs that
+// need FP64 terms raise *exact_flag and are left to k_pc_direct_exact.
 template <int DIFF>
-__global__ __launch_bounds__(NT, (DIFF == OHX_DIFF_FRESNEL) ? 3 : 2) void k_pc_direct(
-    Geometry g, PointSoA s, double2* __restrict__ field, int tiles_x)
+__global__ __launch_bounds__(NT, 3) void k_pc_direct(Geometry g, PointSoA s, double2* __restrict__ field,
+                                                      int tiles_x, unsigned long long* __restrict__ exact_flag)
 {
     __shared__ DirectSmem<DIFF> sm;
-    direct_tile<DIFF>(sm, g, s, 0, s.n, field, blockIdx.x, tiles_x);
+    direct_tile<DIFF, EX_FLAG>(sm, g, s, 0, s.n, field, blockIdx.x, tiles_x, exact_flag);
+}
+
+// The FP64 pairs the FP32 kernel left (same classification, same tiles);
+// returns at once when it left none.
+template <int DIFF>
+__global__ __launch_bounds__(NT) void k_pc_direct_exact(Geometry g, PointSoA s, double2* __restrict__ field,
+                                                         int tiles_x, const unsigned long long* __restrict__ exact_flag)
+{
+    if (*exact_flag == 0ull) return;
+    __shared__ DirectSmem<DIFF> sm;
+    direct_tile<DIFF, EX_ONLY>(sm, g, s, 0, s.n, field, blockIdx.x, tiles_x, nullptr);
 }
 
 template <int DIFF>
@@ -893,10 +930,10 @@ __global__ __launch_bounds__(NT) void k_pc_direct_jobs(const DirectJob* __restri
         const int tile = item - jb.item_base;
         if (jb.g.diff == OHX_DIFF_RS) {
             if (jb.fp64) direct_tile64<OHX_DIFF_RS>(sm.f64, jb.g, jb.s, jb.p0, jb.p1, jb.field, tile, jb.tiles_x);
-            else direct_tile<OHX_DIFF_RS>(sm.rs, jb.g, jb.s, jb.p0, jb.p1, jb.field, tile, jb.tiles_x);
+            else direct_tile<OHX_DIFF_RS, EX_INLINE>(sm.rs, jb.g, jb.s, jb.p0, jb.p1, jb.field, tile, jb.tiles_x, nullptr);
         } else {
             if (jb.fp64) direct_tile64<OHX_DIFF_FRESNEL>(sm.f64, jb.g, jb.s, jb.p0, jb.p1, jb.field, tile, jb.tiles_x);
-            else direct_tile<OHX_DIFF_FRESNEL>(sm.fr, jb.g, jb.s, jb.p0, jb.p1, jb.field, tile, jb.tiles_x);
+            else direct_tile<OHX_DIFF_FRESNEL, EX_INLINE>(sm.fr, jb.g, jb.s, jb.p0, jb.p1, jb.field, tile, jb.tiles_x, nullptr);
         }
         __syncthreads();   // LDS reuse across items
     }
@@ -979,12 +1016,25 @@ int launch_direct(ohx_ctx* ctx, const Geometry& g, const PointSoA& soa, double* 
     if (nwg > 0x7fffffff) return set_error(ctx, OHX_ERR_INVALID_ARG, "hologram too large");
     double2* f = (double2*)d_field;
     const dim3 grid((unsigned)nwg), block(NT);
+    unsigned long long* flag = ctx->minmax + 5;   // EXACT pairs left by the FP32 kernel
     if (g.diff == OHX_DIFF_RS) {
-        if (fp64) hipLaunchKernelGGL((k_pc_direct64<OHX_DIFF_RS>), grid, block, 0, ctx->stream, g, soa, f, tiles_x);
-        else hipLaunchKernelGGL((k_pc_direct<OHX_DIFF_RS>), grid, block, 0, ctx->stream, g, soa, f, tiles_x);
+        if (fp64) {
+            hipLaunchKernelGGL((k_pc_direct64<OHX_DIFF_RS>), grid, block, 0, ctx->stream, g, soa, f, tiles_x);
+        } else {
+            hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, ctx->stream, flag);
+            hipLaunchKernelGGL((k_pc_direct<OHX_DIFF_RS>), grid, block, 0, ctx->stream, g, soa, f, tiles_x, flag);
+            hipLaunchKernelGGL((k_pc_direct_exact<OHX_DIFF_RS>), grid, block, 0, ctx->stream, g, soa, f, tiles_x,
+                               (const unsigned long long*)flag);
+        }
     } else {
-        if (fp64) hipLaunchKernelGGL((k_pc_direct64<OHX_DIFF_FRESNEL>), grid, block, 0, ctx->stream, g, soa, f, tiles_x);
-        else hipLaunchKernelGGL((k_pc_direct<OHX_DIFF_FRESNEL>), grid, block, 0, ctx->stream, g, soa, f, tiles_x);
+        if (fp64) {
+            hipLaunchKernelGGL((k_pc_direct64<OHX_DIFF_FRESNEL>), grid, block, 0, ctx->stream, g, soa, f, tiles_x);
+        } else {
+            hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, ctx->stream, flag);
+            hipLaunchKernelGGL((k_pc_direct<OHX_DIFF_FRESNEL>), grid, block, 0, ctx->stream, g, soa, f, tiles_x, flag);
+            hipLaunchKernelGGL((k_pc_direct_exact<OHX_DIFF_FRESNEL>), grid, block, 0, ctx->stream, g, soa, f,
+                               tiles_x, (const unsigned long long*)flag);
+        }
     }
     ctx->last.workgroups = nwg;
     ctx->last.kernel_kind = fp64 ? 1 : 0;

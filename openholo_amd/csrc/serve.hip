@@ -109,7 +109,7 @@ static int64_t slice_points(const SJob& j, const ohx_server_opts& o)
     const double pairs = j.mfma ? o.slice_pairs_mfma : o.slice_pairs;
     int64_t s = (int64_t)(pairs / npx);
     s = std::max<int64_t>(s, 256);
-    s = (s + 127) / 128 * 128;   // whole LDS chunks of the direct kernel
+    s = (s + DIRECT_CHUNK - 1) / DIRECT_CHUNK * DIRECT_CHUNK;   // whole LDS chunks of the direct kernel
     return std::min<int64_t>(s, std::max<int64_t>(j.n, 1));
 }
 

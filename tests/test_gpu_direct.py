@@ -137,6 +137,31 @@ def test_fp64_near_and_far_points_in_one_launch(ctx, orc, diff):
 
 
 @pytest.mark.parametrize("diff", [0, 1])
+def test_fp32_near_and_far_points_in_one_launch(ctx, orc, diff):
+    """FP32 mode with near points (z 2-6 mm: their pairs need FP64 terms,
+    which the FP32 kernel leaves to its follow-up kernel) and far points
+    (separable FP32 form) interleaved in one cloud.  Supports equal the
+    reference restatement's; values vs the extended-precision oracle."""
+    pn, pp, lam = (300, 170), (6e-6, 6e-6), 532e-9
+    near = synth.random_cloud(51 + diff, 40, 400e-6, (2e-3, 6e-3))
+    far = synth.random_cloud(53 + diff, 40, 400e-6, (0.30, 0.31))
+    v = np.concatenate([near[:20], far[:20], near[20:], far[20:]])
+    g = orc.make_geom(pn, pp)
+    want = orc.gen_restate(g, v, [lam], diff)[0]
+    got = capi.generate(ctx, v, pn, pp, [lam], diff, FP32)[0]
+    assert np.array_equal(got != 0, want != 0)
+    pix = np.random.default_rng(19).choice(pn[0] * pn[1], 3000, replace=False)
+    exact = orc.field_exact(g, v, lam, 0, diff, pix)
+    assert rel_l2(got.ravel()[pix], exact) < 1e-5
+    if diff == 0:
+        # RS near points alone: the series bound fails on every tile, so every
+        # pair is an FP64 term of the follow-up kernel
+        got_near = capi.generate(ctx, near, pn, pp, [lam], diff, FP32)[0]
+        exact_near = orc.field_exact(g, near, lam, 0, diff, pix)
+        assert rel_l2(got_near.ravel()[pix], exact_near) < 1e-11
+
+
+@pytest.mark.parametrize("diff", [0, 1])
 def test_probe_matches_exact_oracle(ctx, orc, diff):
     w = synth.WORKLOADS[2]
     v = synth.points(w, 500)

@@ -124,7 +124,7 @@ def cpu_baseline(w, target_s: float):
 # HBM bytes per launch of the direct kernels at the default workload
 # (rocprofv3 --pmc FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the scale factors read
 # off a 1 GiB copy kernel); profiles/k_pc_direct_rs_fresnel_100k_2160p.md.
-TRAFFIC = {0: 1.059e9, 1: 1.065e9}
+TRAFFIC = {0: 0.985e9, 1: 0.880e9}
 # Speed of light of the direct kernels' separable pair loops (per lane and point,
 # 8 rows: column factor + 8 pairs) on this
 # hardware (openholo_amd/bin/valu_rates); profiles/valu_mfma_rates_gfx950.md.

@@ -20,7 +20,8 @@ struct ohx_ctx {
     // small device scratch: [0..1] min/max of the normaliser, [4] pixels in
     // the mask boxes of the last launch, [5] FP32 direct kernel left EXACT pairs
     unsigned long long* minmax = nullptr;
-    // split-K partial sums of the matrix-core Fresnel path (grown on demand)
+    // split-K partial sums of the matrix-core Fresnel path and the point-split
+    // partial planes of the direct kernels (grown on demand)
     void* gemm_scratch = nullptr;
     size_t gemm_scratch_bytes = 0;
     ohx_launch_stats last{};

@@ -214,6 +214,30 @@ struct DirectSmem {
     int cnt[2][WAVES];
 };
 
+// End of a tile: add the lane's RPT pixels into the field, or, for a point
+// split other than the first (part != nullptr), store them into that split's
+// partial plane, which k_split_reduce adds later in split order.
+__device__ inline void store_tile(const Geometry& g, double2* __restrict__ field, double2* __restrict__ part,
+                                  int col, int rowb, const double* accr, const double* acci)
+{
+    if (col >= g.pn_x) return;
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) {
+        const int row = rowb + k;
+        if (row < g.pn_y) {
+            const int64_t idx = (int64_t)row * g.pn_x + col;
+            if (part) {
+                part[idx] = make_double2(accr[k], acci[k]);
+            } else {
+                double2 f = field[idx];
+                f.x += accr[k];
+                f.y += acci[k];
+                field[idx] = f;
+            }
+        }
+    }
+}
+
 // What one direct_tile instance does with the (point, tile) pairs that need
 // FP64 terms (EXACT class): EX_INLINE evaluates them in place (multi-job
 // kernel); EX_FLAG skips them and raises *exact_flag, which keeps the FP64
@@ -234,7 +258,7 @@ __device__ inline double* slot_fp64(float2* row_entry) { return reinterpret_cast
 template <int DIFF, int EX>
 __device__ __attribute__((always_inline)) inline void direct_tile(DirectSmem<DIFF>& sm, const Geometry& g, const PointSoA& s, int64_t p_begin,
                            int64_t p_end, double2* __restrict__ field, int tile, int tiles_x,
-                           unsigned long long* __restrict__ exact_flag)
+                           unsigned long long* __restrict__ exact_flag, double2* __restrict__ part = nullptr)
 {
     float4* s_c0 = sm.c0;
     float4* s_c1 = sm.c1;
@@ -563,19 +587,7 @@ __device__ __attribute__((always_inline)) inline void direct_tile(DirectSmem<DIF
     }
 
     if (EX == EX_ONLY && !touched) return;   // uniform: nothing to add
-    if (col < g.pn_x) {
-#pragma unroll
-        for (int k = 0; k < RPT; ++k) {
-            const int row = rowb + k;
-            if (row < g.pn_y) {
-                const int64_t idx = (int64_t)row * g.pn_x + col;
-                double2 f = field[idx];
-                f.x += accr[k];
-                f.y += acci[k];
-                field[idx] = f;
-            }
-        }
-    }
+    store_tile(g, field, part, col, rowb, accr, acci);
 }
 
 // ---------------------------------------------------------------------------
@@ -635,7 +647,8 @@ struct DirectSmem64 {
 template <int DIFF>
 __device__ __attribute__((always_inline)) inline void direct_tile64(DirectSmem64& sm, const Geometry& g,
                                                                     const PointSoA& s, int64_t p_begin, int64_t p_end,
-                                                                    double2* __restrict__ field, int tile, int tiles_x)
+                                                                    double2* __restrict__ field, int tile, int tiles_x,
+                                                                    double2* __restrict__ part = nullptr)
 {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int tx = tile % tiles_x, ty = tile / tiles_x;
@@ -863,19 +876,7 @@ __device__ __attribute__((always_inline)) inline void direct_tile64(DirectSmem64
         __syncthreads();
     }
 
-    if (col < g.pn_x) {
-#pragma unroll
-        for (int k = 0; k < RPT; ++k) {
-            const int row = rowb + k;
-            if (row < g.pn_y) {
-                const int64_t idx = (int64_t)row * g.pn_x + col;
-                double2 f = field[idx];
-                f.x += accr[k];
-                f.y += acci[k];
-                field[idx] = f;
-            }
-        }
-    }
+    store_tile(g, field, part, col, rowb, accr, acci);
 }
 
 // LDS of the multi-job kernel, which runs both paths
@@ -887,12 +888,51 @@ union TileSmem {
 
 // FP32 kernel: three waves per SIMD (registers and LDS both fit); pairs that
 // need FP64 terms raise *exact_flag and are left to k_pc_direct_exact.
+// Workgroup = (tile, point split): blockIdx = split * tiles + tile; split 0
+// adds into the field, the others store into their partial planes.
+struct SplitArgs {
+    int tiles;           // tiles per split
+    int64_t split_pts;   // points per split (a multiple of the LDS chunk)
+    double2* part;       // (splits - 1) partial planes of pn_x * pn_y pixels
+};
+
+__device__ inline void split_item(const Geometry& g, const PointSoA& s, const SplitArgs& sa, int& tile,
+                                  int64_t& p0, int64_t& p1, double2*& part)
+{
+    const int split = blockIdx.x / sa.tiles;
+    tile = blockIdx.x - split * sa.tiles;
+    p0 = (int64_t)split * sa.split_pts;
+    p1 = min(s.n, p0 + sa.split_pts);
+    part = split == 0 ? nullptr : sa.part + (int64_t)(split - 1) * ((int64_t)g.pn_x * g.pn_y);
+}
+
 template <int DIFF>
 __global__ __launch_bounds__(NT, 3) void k_pc_direct(Geometry g, PointSoA s, double2* __restrict__ field,
-                                                      int tiles_x, unsigned long long* __restrict__ exact_flag)
+                                                      int tiles_x, unsigned long long* __restrict__ exact_flag,
+                                                      SplitArgs sa)
 {
     __shared__ DirectSmem<DIFF> sm;
-    direct_tile<DIFF, EX_FLAG>(sm, g, s, 0, s.n, field, blockIdx.x, tiles_x, exact_flag);
+    int tile;
+    int64_t p0, p1;
+    double2* part;
+    split_item(g, s, sa, tile, p0, p1, part);
+    direct_tile<DIFF, EX_FLAG>(sm, g, s, p0, p1, field, tile, tiles_x, exact_flag, part);
+}
+
+// field += partial planes 1..splits-1, in split order (deterministic)
+__global__ __launch_bounds__(256) void k_split_reduce(double2* __restrict__ field, const double2* __restrict__ part,
+                                                      int64_t npx, int nparts)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < npx; i += stride) {
+        double2 f = field[i];
+        for (int k = 0; k < nparts; ++k) {
+            const double2 q = part[(int64_t)k * npx + i];
+            f.x += q.x;
+            f.y += q.y;
+        }
+        field[i] = f;
+    }
 }
 
 // The FP64 pairs the FP32 kernel left (same classification, same tiles);
@@ -908,10 +948,14 @@ __global__ __launch_bounds__(NT) void k_pc_direct_exact(Geometry g, PointSoA s, 
 
 template <int DIFF>
 __global__ __launch_bounds__(NT, 2) void k_pc_direct64(Geometry g, PointSoA s, double2* __restrict__ field,
-                                                        int tiles_x)
+                                                        int tiles_x, SplitArgs sa)
 {
     __shared__ DirectSmem64 sm;
-    direct_tile64<DIFF>(sm, g, s, 0, s.n, field, blockIdx.x, tiles_x);
+    int tile;
+    int64_t p0, p1;
+    double2* part;
+    split_item(g, s, sa, tile, p0, p1, part);
+    direct_tile64<DIFF>(sm, g, s, p0, p1, field, tile, tiles_x, part);
 }
 
 // Persistent multi-job form used by the serving engine: workgroups stride
@@ -1009,34 +1053,91 @@ int launch_prep(ohx_ctx* ctx, const void* d_vertices, const Geometry& g, const P
     return check_hip(ctx, hipGetLastError(), "k_pc_prep");
 }
 
+// Workgroups the device holds at once for kernel k (occupancy x CUs).
+static int resident_slots(ohx_ctx* ctx, const void* k)
+{
+    int per_cu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, NT, 0) != hipSuccess || per_cu < 1) per_cu = 1;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess || cus < 1)
+        cus = 1;
+    return per_cu * cus;
+}
+
+// Point splits per tile so that the last round of equal-cost workgroups is
+// nearly full: time ~ ceil(tiles * S / slots) / S (3840x2160: 4080 tiles on
+// 768 slots leave the sixth round 31 % full at S = 1; S = 3 fills 99 %).
+// Each split keeps at least four LDS chunks, and the partial planes stay
+// under 2 GiB.
+static int choose_splits(int64_t tiles, int64_t n, int slots, int64_t npx, int chunk)
+{
+    int best = 1;
+    double best_t = std::ceil((double)tiles / slots);
+    for (int S = 2; S <= 4; ++S) {
+        if (n < (int64_t)S * 4 * chunk) break;
+        if ((double)(S - 1) * npx * 16.0 > 2.0 * (1ull << 30)) break;
+        const double t = std::ceil((double)tiles * S / slots) / S;
+        if (t < best_t * 0.97) {
+            best = S;
+            best_t = t;
+        }
+    }
+    return best;
+}
+
 int launch_direct(ohx_ctx* ctx, const Geometry& g, const PointSoA& soa, double* d_field, bool fp64)
 {
     const int tiles_x = (g.pn_x + TW - 1) / TW, tiles_y = (g.pn_y + TH - 1) / TH;
-    const int64_t nwg = (int64_t)tiles_x * tiles_y;
-    if (nwg > 0x7fffffff) return set_error(ctx, OHX_ERR_INVALID_ARG, "hologram too large");
+    const int64_t tiles = (int64_t)tiles_x * tiles_y;
+    const int64_t npx = (int64_t)g.pn_x * g.pn_y;
+    if (tiles * 4 > 0x7fffffff) return set_error(ctx, OHX_ERR_INVALID_ARG, "hologram too large");
     double2* f = (double2*)d_field;
-    const dim3 grid((unsigned)nwg), block(NT);
+    const bool rs = g.diff == OHX_DIFF_RS;
+    const void* kern = fp64 ? (rs ? (const void*)k_pc_direct64<OHX_DIFF_RS> : (const void*)k_pc_direct64<OHX_DIFF_FRESNEL>)
+                            : (rs ? (const void*)k_pc_direct<OHX_DIFF_RS> : (const void*)k_pc_direct<OHX_DIFF_FRESNEL>);
+    const int chunk = fp64 ? PCD : PCH;
+    int splits = choose_splits(tiles, soa.n, resident_slots(ctx, kern), npx, chunk);
+    SplitArgs sa{(int)tiles, soa.n, nullptr};
+    if (splits > 1) {
+        // split boundaries on chunk boundaries: a split sums exactly as the
+        // serving engine's point slices do
+        const int64_t per = (soa.n + splits - 1) / splits;
+        sa.split_pts = (per + chunk - 1) / chunk * chunk;
+        splits = (int)((soa.n + sa.split_pts - 1) / sa.split_pts);
+    }
+    if (splits > 1) {
+        int st = ensure_gemm_scratch(ctx, (size_t)(splits - 1) * npx * sizeof(double2));
+        if (st) return st;
+        sa.part = (double2*)ctx->gemm_scratch;
+    }
+    const dim3 grid((unsigned)(tiles * splits)), block(NT);
+    const dim3 tgrid((unsigned)tiles);
     unsigned long long* flag = ctx->minmax + 5;   // EXACT pairs left by the FP32 kernel
-    if (g.diff == OHX_DIFF_RS) {
+    if (rs) {
         if (fp64) {
-            hipLaunchKernelGGL((k_pc_direct64<OHX_DIFF_RS>), grid, block, 0, ctx->stream, g, soa, f, tiles_x);
+            hipLaunchKernelGGL((k_pc_direct64<OHX_DIFF_RS>), grid, block, 0, ctx->stream, g, soa, f, tiles_x, sa);
         } else {
             hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, ctx->stream, flag);
-            hipLaunchKernelGGL((k_pc_direct<OHX_DIFF_RS>), grid, block, 0, ctx->stream, g, soa, f, tiles_x, flag);
-            hipLaunchKernelGGL((k_pc_direct_exact<OHX_DIFF_RS>), grid, block, 0, ctx->stream, g, soa, f, tiles_x,
+            hipLaunchKernelGGL((k_pc_direct<OHX_DIFF_RS>), grid, block, 0, ctx->stream, g, soa, f, tiles_x, flag, sa);
+            hipLaunchKernelGGL((k_pc_direct_exact<OHX_DIFF_RS>), tgrid, block, 0, ctx->stream, g, soa, f, tiles_x,
                                (const unsigned long long*)flag);
         }
     } else {
         if (fp64) {
-            hipLaunchKernelGGL((k_pc_direct64<OHX_DIFF_FRESNEL>), grid, block, 0, ctx->stream, g, soa, f, tiles_x);
+            hipLaunchKernelGGL((k_pc_direct64<OHX_DIFF_FRESNEL>), grid, block, 0, ctx->stream, g, soa, f, tiles_x, sa);
         } else {
             hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, ctx->stream, flag);
-            hipLaunchKernelGGL((k_pc_direct<OHX_DIFF_FRESNEL>), grid, block, 0, ctx->stream, g, soa, f, tiles_x, flag);
-            hipLaunchKernelGGL((k_pc_direct_exact<OHX_DIFF_FRESNEL>), grid, block, 0, ctx->stream, g, soa, f,
+            hipLaunchKernelGGL((k_pc_direct<OHX_DIFF_FRESNEL>), grid, block, 0, ctx->stream, g, soa, f, tiles_x, flag,
+                               sa);
+            hipLaunchKernelGGL((k_pc_direct_exact<OHX_DIFF_FRESNEL>), tgrid, block, 0, ctx->stream, g, soa, f,
                                tiles_x, (const unsigned long long*)flag);
         }
     }
-    ctx->last.workgroups = nwg;
+    if (splits > 1) {
+        const int64_t rb = std::min<int64_t>((npx + 255) / 256, 4096);
+        hipLaunchKernelGGL(k_split_reduce, dim3((unsigned)rb), dim3(256), 0, ctx->stream, f,
+                           (const double2*)sa.part, npx, splits - 1);
+    }
+    ctx->last.workgroups = tiles * splits;
     ctx->last.kernel_kind = fp64 ? 1 : 0;
     return check_hip(ctx, hipGetLastError(), "k_pc_direct");
 }

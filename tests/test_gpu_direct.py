@@ -161,6 +161,29 @@ def test_fp32_near_and_far_points_in_one_launch(ctx, orc, diff):
         assert rel_l2(got_near.ravel()[pix], exact_near) < 1e-11
 
 
+@pytest.mark.parametrize("mode", [FP32, FP64])
+@pytest.mark.parametrize("diff", [0, 1])
+def test_point_splits(ctx, orc, diff, mode):
+    """1920x1200 is 1140 tiles: with 2, 3 or 4 resident workgroups per CU
+    (512-1024 at once) the last round would run part empty, so the launch
+    splits each tile's points into parts (partial planes added in split
+    order).  The split launch matches the extended-precision oracle and is
+    deterministic."""
+    w = synth.WORKLOADS[1]
+    pn = (1920, 1200)
+    v = synth.points(w, 3000)
+    g = orc.make_geom(pn, w.pp)
+    got = capi.generate(ctx, v, pn, w.pp, [w.lambdas[0]], diff, mode)[0].ravel()
+    st = ctx.last_stats()
+    tiles = (1920 // 64) * (1200 // 32 + 1)
+    assert st.workgroups % tiles == 0 and st.workgroups > tiles      # really split
+    again = capi.generate(ctx, v, pn, w.pp, [w.lambdas[0]], diff, mode)[0].ravel()
+    assert np.array_equal(got, again)
+    pix = np.random.default_rng(23).choice(pn[0] * pn[1], 2000, replace=False)
+    want = orc.field_exact(g, v, w.lambdas[0], 0, diff, pix)
+    assert rel_l2(got[pix], want) < (1e-5 if mode == FP32 else 1e-11)
+
+
 @pytest.mark.parametrize("diff", [0, 1])
 def test_probe_matches_exact_oracle(ctx, orc, diff):
     w = synth.WORKLOADS[2]

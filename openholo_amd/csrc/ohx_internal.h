@@ -67,7 +67,7 @@ struct DirectJob {
     PointSoA s;
     double2* field;
     int64_t p0, p1;       // point slice of this iteration
-    int tiles_x, tiles;   // 64 x 32 tiles of the hologram
+    int tiles_x, tiles;   // tiles of the hologram (64 x 32 FP32, 128 x 16 FP64)
     int item_base;        // prefix sum of tiles over the table
     int fp64;
 };
@@ -85,7 +85,7 @@ int launch_prep(ohx_ctx* ctx, const void* d_vertices, const Geometry& g, const P
                 unsigned long long* boxsum = nullptr);
 int launch_direct(ohx_ctx* ctx, const Geometry& g, const PointSoA& soa, double* d_field, bool fp64);
 int launch_fresnel_gemm(ohx_ctx* ctx, const Geometry& g, const PointSoA& soa, double* d_field, uint32_t mode);
-int direct_tiles(const Geometry& g, int* tiles_x);
+int direct_tiles(const Geometry& g, bool fp64, int* tiles_x);
 int launch_direct_jobs(hipStream_t stream, const DirectJob* d_jobs, const int* d_njobs, int grid);
 int launch_fresnel_gemm_range(hipStream_t stream, void* scratch, size_t scratch_bytes, const Geometry& g,
                               const PointSoA& soa, int64_t p0, int64_t p1, double* d_field, uint32_t mode);

@@ -616,6 +616,12 @@ enum { F_PX = 0, F_PY, F_CYC, F_Q, F_AMP };
 enum { R_C0 = 0, R_H, R_G2X, R_AMP, R_K2, R_K3, R_K4, R_M1, R_M2, R_M3, R_G2Y };
 
 constexpr int NTAB = 256;  // angles of the FP64 sin/cos table
+// FP64 tiles are 128 x 16: the two waves of a wave row share its 8 rows, so
+// the row factors are computed once per row for both (the FP32 path keeps
+// 64 x 32: wider tiles double its FP32 phase magnitudes at the edges).
+constexpr int WX64 = 2;
+constexpr int TW64 = WX64 * TW;
+constexpr int TH64 = (WAVES / WX64) * RPT;
 
 // sin and cos of 2 pi t, t in [0, 1], in FP64: the nearest table angle
 // 2 pi i / NTAB plus Taylor series of the remainder |x| <= pi / NTAB
@@ -635,8 +641,8 @@ __device__ inline void sincos2pi(double t, const double2* __restrict__ tab, doub
 
 struct DirectSmem64 {
     double2 tab[NTAB];     // (cos, sin)(2 pi i / NTAB)
-    double2 B[TH][PCD];    // row factors B(row) of the separable list
-    double Ny[TH][PCD];    // RS: N_y(row)
+    double2 B[TH64][PCD];  // row factors B(row) of the separable list
+    double Ny[TH64][PCD];  // RS: N_y(row)
     double c[PCD][NCS];    // separable list: per-point constants
     int4 sbox[PCD];        // separable list: mask box (Fresnel)
     double d[6][PCD];      // per-pixel list: px, py, z, cyc0, kamp, q
@@ -651,16 +657,17 @@ __device__ __attribute__((always_inline)) inline void direct_tile64(DirectSmem64
                                                                     double2* __restrict__ part = nullptr)
 {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int cw = wave % WX64, rw = wave / WX64;   // wave (cw, rw): 64 columns x 8 rows
     const int tx = tile % tiles_x, ty = tile / tiles_x;
-    const int x0 = tx * TW, y0 = ty * TH;
-    const int cx1 = min(x0 + TW, g.pn_x), ry1 = min(y0 + TH, g.pn_y);
-    const int col = x0 + lane;
-    const int rowb = y0 + wave * RPT;
-    const double xc = pix_x(g, x0 + TW / 2), yc = pix_y(g, y0 + TH / 2);
+    const int x0 = tx * TW64, y0 = ty * TH64;
+    const int cx1 = min(x0 + TW64, g.pn_x), ry1 = min(y0 + TH64, g.pn_y);
+    const int col = x0 + cw * TW + lane;
+    const int rowb = y0 + rw * RPT;
+    const double xc = pix_x(g, x0 + TW64 / 2), yc = pix_y(g, y0 + TH64 / 2);
     const double xxx = pix_x(g, col);
     const double u = xxx - xc;
-    // |u| <= 32 pitches and |v| <= 16 pitches over the tile
-    const double umax = (double)(TW / 2) * g.pp_x, vmax = (double)(TH / 2) * g.pp_y;
+    // |u| <= 64 pitches and |v| <= 8 pitches over the tile
+    const double umax = (double)(TW64 / 2) * g.pp_x, vmax = (double)(TH64 / 2) * g.pp_y;
 
     // sin/cos table (first read after the classification barrier below)
     for (int i = tid; i < NTAB; i += NT) {
@@ -773,13 +780,14 @@ __device__ __attribute__((always_inline)) inline void direct_tile64(DirectSmem64
         const int nS = __builtin_amdgcn_readfirstlane(sm.cnt[0]);
         const int nE = __builtin_amdgcn_readfirstlane(sm.cnt[1]);
 
-        // ---- row factors: lanes j and j + 32 take half of this wave's rows of point j
+        // ---- row factors: the two waves of a wave row split its 8 rows; lanes
+        // j and j + 32 take half of a wave's share for point j
         {
             const int j = lane & (PCD - 1);
-            const int r0 = wave * RPT + (lane >> 5) * (RPT / 2);
+            const int r0 = rw * RPT + cw * (RPT / WX64) + (lane >> 5) * (RPT / WX64 / 2);
             if (j < nS) {
 #pragma unroll
-                for (int k = 0; k < RPT / 2; ++k) {
+                for (int k = 0; k < RPT / WX64 / 2; ++k) {
                     const int rr = r0 + k;
                     const int row = y0 + rr;
                     const double yyy = pix_y(g, row);
@@ -821,7 +829,7 @@ __device__ __attribute__((always_inline)) inline void direct_tile64(DirectSmem64
                 Ai = -a * cs;
 #pragma unroll
                 for (int k = 0; k < RPT; ++k) {
-                    const double2 bk = sm.B[wave * RPT + k][j];
+                    const double2 bk = sm.B[rw * RPT + k][j];
                     accr[k] = fma(Ar, bk.x, fma(-Ai, bk.y, accr[k]));
                     acci[k] = fma(Ar, bk.y, fma(Ai, bk.x, acci[k]));
                 }
@@ -838,8 +846,8 @@ __device__ __attribute__((always_inline)) inline void direct_tile64(DirectSmem64
                 const double m1 = sm.c[j][R_M1], m2 = sm.c[j][R_M2], m3 = sm.c[j][R_M3];
 #pragma unroll
                 for (int k = 0; k < RPT; ++k) {
-                    const double2 bk = sm.B[wave * RPT + k][j];
-                    const double N = nx + sm.Ny[wave * RPT + k][j];
+                    const double2 bk = sm.B[rw * RPT + k][j];
+                    const double N = nx + sm.Ny[rw * RPT + k][j];
                     const double t = fma(fma(k4, N, k3), N, k2);
                     const double ph2 = (N * N) * t;                 // 2 pi phi(N), radians
                     const double p2 = ph2 * ph2;
@@ -1086,7 +1094,8 @@ static int choose_splits(int64_t tiles, int64_t n, int slots, int64_t npx, int c
 
 int launch_direct(ohx_ctx* ctx, const Geometry& g, const PointSoA& soa, double* d_field, bool fp64)
 {
-    const int tiles_x = (g.pn_x + TW - 1) / TW, tiles_y = (g.pn_y + TH - 1) / TH;
+    const int tw = fp64 ? TW64 : TW, th = fp64 ? TH64 : TH;
+    const int tiles_x = (g.pn_x + tw - 1) / tw, tiles_y = (g.pn_y + th - 1) / th;
     const int64_t tiles = (int64_t)tiles_x * tiles_y;
     const int64_t npx = (int64_t)g.pn_x * g.pn_y;
     if (tiles * 4 > 0x7fffffff) return set_error(ctx, OHX_ERR_INVALID_ARG, "hologram too large");
@@ -1142,10 +1151,11 @@ int launch_direct(ohx_ctx* ctx, const Geometry& g, const PointSoA& soa, double* 
     return check_hip(ctx, hipGetLastError(), "k_pc_direct");
 }
 
-int direct_tiles(const Geometry& g, int* tiles_x)
+int direct_tiles(const Geometry& g, bool fp64, int* tiles_x)
 {
-    *tiles_x = (g.pn_x + TW - 1) / TW;
-    return *tiles_x * ((g.pn_y + TH - 1) / TH);
+    const int tw = fp64 ? TW64 : TW, th = fp64 ? TH64 : TH;
+    *tiles_x = (g.pn_x + tw - 1) / tw;
+    return *tiles_x * ((g.pn_y + th - 1) / th);
 }
 
 int launch_direct_jobs(hipStream_t stream, const DirectJob* d_jobs, const int* d_njobs, int grid)

@@ -284,7 +284,7 @@ static void worker_loop(ohx_server* sv)
                 d.field = (double2*)j->field;
                 d.p0 = p0;
                 d.p1 = p1;
-                d.tiles = direct_tiles(j->g, &d.tiles_x);
+                d.tiles = direct_tiles(j->g, j->fp64, &d.tiles_x);
                 d.item_base = items;
                 d.fp64 = j->fp64 ? 1 : 0;
                 items += d.tiles;

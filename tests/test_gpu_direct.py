@@ -164,7 +164,8 @@ def test_fp32_near_and_far_points_in_one_launch(ctx, orc, diff):
 @pytest.mark.parametrize("mode", [FP32, FP64])
 @pytest.mark.parametrize("diff", [0, 1])
 def test_point_splits(ctx, orc, diff, mode):
-    """1920x1200 is 1140 tiles: with 2, 3 or 4 resident workgroups per CU
+    """1920x1200 is 1140 tiles of 64x32 (FP32) or 1125 of 128x16 (FP64):
+    with 2, 3 or 4 resident workgroups per CU
     (512-1024 at once) the last round would run part empty, so the launch
     splits each tile's points into parts (partial planes added in split
     order).  The split launch matches the extended-precision oracle and is
@@ -175,7 +176,7 @@ def test_point_splits(ctx, orc, diff, mode):
     g = orc.make_geom(pn, w.pp)
     got = capi.generate(ctx, v, pn, w.pp, [w.lambdas[0]], diff, mode)[0].ravel()
     st = ctx.last_stats()
-    tiles = (1920 // 64) * (1200 // 32 + 1)
+    tiles = (1920 // 64) * (1200 // 32 + 1) if mode == FP32 else (1920 // 128) * (1200 // 16)
     assert st.workgroups % tiles == 0 and st.workgroups > tiles      # really split
     again = capi.generate(ctx, v, pn, w.pp, [w.lambdas[0]], diff, mode)[0].ravel()
     assert np.array_equal(got, again)

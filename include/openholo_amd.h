@@ -129,6 +129,35 @@ int ohx_pc_probe_f64(ohx_ctx* ctx, const void* d_vertices, int64_t n_points,
                      const ohx_pc_params* params, uint32_t diff_flag,
                      const int64_t* d_pix, int64_t n_pix, double* d_out);
 
+/* ---- WRP (wavefront recording plane) generator -----------------------------
+ * One channel of ophWRP::generateHologram (ophgen/src/ophWRP.cpp:437-463):
+ * autoScaling (:69-118: positions times scale, zmax), calculateWRPCPU/GPU
+ * (:321-435, ophWRP_GPU.cpp:48-200: every point adds a (2w)^2 window of
+ * spherical-wave terms at the common depth dz = wrp_location - zmax) and
+ * ophGen::Fresnel_FFT (ophgen/src/ophGen.cpp:516-581: zero padding to
+ * 2 pn_x x 2 pn_y, centred FFT, angular-spectrum transfer function, centred
+ * unnormalised inverse FFT, centre crop).  FP64 throughout; the centring
+ * shifts are out of place (the reference shifts in place there, which
+ * corrupts its input, Openholo/src/Openholo.cpp:716); only random_phase = 0
+ * is supported (the reference's random phase is not reproducible). */
+typedef struct ohx_wrp_params {
+    int32_t pn_x, pn_y;           /* pixel_number                                */
+    double pp_x, pp_y;            /* pixel_pitch (m); the WRP uses the same grid  */
+    double scale[3];              /* WRPConfig::scale                            */
+    double lambda;                /* wave length of this channel (m)             */
+    int32_t channel;              /* index into Vertex::color (0..2)             */
+    int32_t random_phase;         /* must be 0                                   */
+    double wrp_location;          /* WRPConfig::wrp_location (m)                 */
+    double propagation_distance;  /* WRPConfig::propagation_distance (m)         */
+} ohx_wrp_params;
+
+/* d_field (pn_x*pn_y interleaved complex FP64) is overwritten with the
+ * hologram of this channel (complex_H[ch]); d_wrp, if not null, receives the
+ * WRP plane itself (ophWRP::getWRPBuff).  d_vertices are the 56-byte Vertex
+ * records on the device. */
+int ohx_wrp_generate(ohx_ctx* ctx, const void* d_vertices, int64_t n_points,
+                     const ohx_wrp_params* params, double* d_field, double* d_wrp);
+
 /* ---- encoding and 8-bit normalisation ------------------------------------
  * ophGen::encoding(ENCODE_FLAG) per channel (ophgen/src/ophGen.cpp:816-849,
  * encoders :1821-1864): d_enc[i] = f(d_field[i]). */
@@ -145,7 +174,7 @@ int ohx_normalize_u8(ohx_ctx* ctx, const double* const* d_enc, int n_channels,
 typedef struct ohx_launch_stats {
     int64_t pairs_nominal;   /* n_points * pn_x * pn_y                        */
     int64_t workgroups;      /* workgroups of the diffraction kernel          */
-    int32_t kernel_kind;     /* 0 FP32 direct, 1 FP64 direct, 2 MFMA GEMM     */
+    int32_t kernel_kind;     /* 0 FP32 direct, 1 FP64 direct, 2 MFMA GEMM, 3 WRP */
     int32_t reserved;
     int64_t pairs_in_box;    /* point-pixel pairs inside the points' anti-
                                 aliasing boxes (ophGen.cpp:384-407,467-490):

@@ -27,7 +27,7 @@ EXPORTS = [
     "ohx_ctx_create", "ohx_ctx_destroy", "ohx_ctx_stream", "ohx_ctx_device", "ohx_status_string",
     "ohx_last_error", "ohx_mem_info", "ohx_malloc", "ohx_free", "ohx_memcpy_h2d", "ohx_memcpy_d2h",
     "ohx_memcpy_d2d", "ohx_synchronize", "ohx_field_zero", "ohx_pc_diffract", "ohx_field_accumulate",
-    "ohx_pc_probe_f64", "ohx_encode", "ohx_normalize_u8", "ohx_last_launch_stats",
+    "ohx_pc_probe_f64", "ohx_wrp_generate", "ohx_encode", "ohx_normalize_u8", "ohx_last_launch_stats",
     "ohx_server_create", "ohx_server_submit", "ohx_server_wait", "ohx_server_cancel",
     "ohx_server_counters_get", "ohx_server_destroy",
 ]
@@ -43,6 +43,30 @@ class PcParams(ctypes.Structure):
         ("lambda_", ctypes.c_double),
         ("channel", ctypes.c_int32), ("use_phase", ctypes.c_int32),
     ]
+
+
+class WrpParams(ctypes.Structure):
+    _fields_ = [
+        ("pn_x", ctypes.c_int32), ("pn_y", ctypes.c_int32),
+        ("pp_x", ctypes.c_double), ("pp_y", ctypes.c_double),
+        ("scale", ctypes.c_double * 3), ("lambda_", ctypes.c_double),
+        ("channel", ctypes.c_int32), ("random_phase", ctypes.c_int32),
+        ("wrp_location", ctypes.c_double), ("propagation_distance", ctypes.c_double),
+    ]
+
+
+def wrp_params(pn, pp, lam, wrp_location, distance, channel=0, scale=(1.0, 1.0, 1.0)) -> WrpParams:
+    p = WrpParams()
+    p.pn_x, p.pn_y = int(pn[0]), int(pn[1])
+    p.pp_x, p.pp_y = float(pp[0]), float(pp[1])
+    for i in range(3):
+        p.scale[i] = float(scale[i])
+    p.lambda_ = float(lam)
+    p.channel = int(channel)
+    p.random_phase = 0
+    p.wrp_location = float(wrp_location)
+    p.propagation_distance = float(distance)
+    return p
 
 
 class LaunchStats(ctypes.Structure):
@@ -126,6 +150,7 @@ def lib():
             "ohx_pc_diffract": ([vp, vp, i64, vp, ctypes.POINTER(PcParams), u32, u32], i32),
             "ohx_field_accumulate": ([vp, vp, vp, i64], i32),
             "ohx_pc_probe_f64": ([vp, vp, i64, ctypes.POINTER(PcParams), u32, vp, i64, vp], i32),
+            "ohx_wrp_generate": ([vp, vp, i64, ctypes.POINTER(WrpParams), vp, vp], i32),
             "ohx_encode": ([vp, vp, i64, vp, u32], i32),
             "ohx_normalize_u8": ([vp, ctypes.POINTER(vp), i32, i64, ctypes.POINTER(vp)], i32),
             "ohx_last_launch_stats": ([vp, ctypes.POINTER(LaunchStats)], i32),
@@ -234,6 +259,11 @@ class Context:
             dpix.free()
             out.free()
 
+    def wrp_generate(self, d_vertices: DeviceBuffer, n_points: int, p: WrpParams, field: DeviceBuffer,
+                     wrp: DeviceBuffer | None = None):
+        self._check(lib().ohx_wrp_generate(self.h, d_vertices.ptr, int(n_points), ctypes.byref(p), field.ptr,
+                                           wrp.ptr if wrp is not None else None), "ohx_wrp_generate")
+
     def encode(self, field: DeviceBuffer, n_px: int, enc: DeviceBuffer, flag: int):
         self._check(lib().ohx_encode(self.h, field.ptr, int(n_px), enc.ptr, int(flag)), "ohx_encode")
 
@@ -329,3 +359,21 @@ def generate(ctx: Context, vertices: np.ndarray, pn, pp, lambdas, diff: int, mod
         dv.free()
         fld.free()
     return out
+
+
+def generate_wrp(ctx: Context, vertices: np.ndarray, pn, pp, lam, wrp_location, distance, channel=0,
+                 scale=(1.0, 1.0, 1.0)):
+    """Convenience: one WRP channel; returns (WRP plane, hologram field) as
+    complex128 (pn_y, pn_x) arrays."""
+    n = len(vertices)
+    dv = ctx.to_device(vertices)
+    npx = int(pn[0]) * int(pn[1])
+    fld, wrp = ctx.alloc(16 * npx), ctx.alloc(16 * npx)
+    try:
+        ctx.wrp_generate(dv, n, wrp_params(pn, pp, lam, wrp_location, distance, channel, scale), fld, wrp)
+        shape = (int(pn[1]), int(pn[0]))
+        return (wrp.download(np.complex128, npx).reshape(shape), fld.download(np.complex128, npx).reshape(shape))
+    finally:
+        dv.free()
+        fld.free()
+        wrp.free()

@@ -204,6 +204,7 @@ int ohx_ctx_destroy(ohx_ctx* c)
     if (c->pts) (void)hipFree(c->pts);
     if (c->minmax) (void)hipFree(c->minmax);
     if (c->gemm_scratch) (void)hipFree(c->gemm_scratch);
+    wrp_release(c);
     (void)hipStreamDestroy(c->stream);
     delete c;
     return OHX_OK;
@@ -321,6 +322,25 @@ int ohx_pc_probe_f64(ohx_ctx* c, const void* d_vertices, int64_t n, const ohx_pc
     soa.n = n;
     if (n > 0 && (st = launch_prep(c, d_vertices, g, soa))) return st;
     return launch_probe(c, g, soa, d_pix, n_pix, d_out);
+}
+
+int ohx_wrp_generate(ohx_ctx* c, const void* d_vertices, int64_t n, const ohx_wrp_params* p, double* d_field,
+                     double* d_wrp)
+{
+    if (!c || !p) return OHX_ERR_INVALID_ARG;
+    if (n < 0 || (n > 0 && !d_vertices) || !d_field)
+        return set_error(c, OHX_ERR_INVALID_ARG, "null buffer or negative point count");
+    if (p->pn_x < 1 || p->pn_y < 1 || (int64_t)p->pn_x * p->pn_y > (int64_t)1 << 28)
+        return set_error(c, OHX_ERR_INVALID_ARG, "pixel number out of range");
+    if (!(p->pp_x > 0) || !(p->pp_y > 0) || !(p->lambda > 0) || !std::isfinite(p->wrp_location) ||
+        !std::isfinite(p->propagation_distance))
+        return set_error(c, OHX_ERR_INVALID_ARG, "pitch, wave length or distances not positive/finite");
+    if (p->channel < 0 || p->channel > 2) return set_error(c, OHX_ERR_INVALID_ARG, "channel must be 0..2");
+    if (p->random_phase) return set_error(c, OHX_ERR_INVALID_ARG, "random phase is not supported");
+    std::lock_guard<std::mutex> lk(c->mu);
+    (void)hipSetDevice(c->device);
+    c->boxsum_pending = false;
+    return wrp_generate(c, d_vertices, n, p, d_field, d_wrp);
 }
 
 int ohx_last_launch_stats(ohx_ctx* c, ohx_launch_stats* out)

@@ -24,6 +24,9 @@ struct ohx_ctx {
     // partial planes of the direct kernels (grown on demand)
     void* gemm_scratch = nullptr;
     size_t gemm_scratch_bytes = 0;
+    // hipFFT plan of the WRP propagation (hipfftHandle), for an L x M grid
+    void* wrp_plan = nullptr;
+    int wrp_plan_L = 0, wrp_plan_M = 0;
     ohx_launch_stats last{};
     bool boxsum_pending = false;   // last.pairs_in_box still on the device (minmax[4])
 };
@@ -86,6 +89,9 @@ int launch_prep(ohx_ctx* ctx, const void* d_vertices, const Geometry& g, const P
 int launch_direct(ohx_ctx* ctx, const Geometry& g, const PointSoA& soa, double* d_field, bool fp64);
 int launch_fresnel_gemm(ohx_ctx* ctx, const Geometry& g, const PointSoA& soa, double* d_field, uint32_t mode);
 int direct_tiles(const Geometry& g, bool fp64, int* tiles_x);
+int wrp_generate(ohx_ctx* ctx, const void* d_vertices, int64_t n, const ohx_wrp_params* p, double* d_field,
+                 double* d_wrp);
+void wrp_release(ohx_ctx* ctx);
 int launch_direct_jobs(hipStream_t stream, const DirectJob* d_jobs, const int* d_njobs, int grid);
 int launch_fresnel_gemm_range(hipStream_t stream, void* scratch, size_t scratch_bytes, const Geometry& g,
                               const PointSoA& soa, int64_t p0, int64_t p1, double* d_field, uint32_t mode);

@@ -61,6 +61,7 @@ def parse():
     ap.add_argument("--serving", type=int, default=1,
                     help="also run the mixed Poisson request stream on the serving engine (0 = off)")
     ap.add_argument("--requests", type=int, default=120)
+    ap.add_argument("--wrp", type=int, default=1, help="also time the WRP generator (0 = off)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--probe-pixels", type=int, default=512)
@@ -129,6 +130,43 @@ TRAFFIC = {0: 0.985e9, 1: 0.880e9}
 # 8 rows: column factor + 8 pairs) on this
 # hardware (openholo_amd/bin/valu_rates); profiles/valu_mfma_rates_gfx950.md.
 MIX_SOL_PAIRS_PER_S = {0: 6.67e12, 1: 11.43e12}
+
+
+def wrp_block(ctx, capi, synth, Event, launches=3):
+    """The WRP generator (ohx_wrp_generate) on the reference's WRP example
+    geometry (Colab/Config/Generation_WRP (RGB).xml: 1920x1080 at 10 um, WRP
+    2.7 mm in front of the deepest point, hologram 0.5 m behind it) with
+    10k and 100k synthetic points, three channels: ms per hologram (HIP events
+    on the launch stream) and window terms per second."""
+    out = {"geometry": "1920x1080 at 10 um, RGB 638/520/450 nm, dz 2.7 mm, d 0.5 m"}
+    pn, pp, lams = (1920, 1080), (1e-5, 1e-5), (638e-9, 520e-9, 450e-9)
+    stream = ctx.stream
+    for n in (10_000, 100_000):
+        v = synth.random_cloud(20261202 + 6, n, 1.5e-3, (0.0, 0.3e-3))
+        zmax = float(v["pos"][:, 2].max())
+        dv = ctx.to_device(v)
+        fld = ctx.alloc(16 * pn[0] * pn[1])
+        ps = [capi.wrp_params(pn, pp, lam, zmax + 2.7e-3, 0.5, channel=ch) for ch, lam in enumerate(lams)]
+        for p in ps:
+            ctx.wrp_generate(dv, n, p, fld)
+        ctx.sync()
+        terms = 0
+        ev = []
+        for _ in range(launches):
+            a, b = Event(), Event()
+            a.record(stream)
+            for p in ps:
+                ctx.wrp_generate(dv, n, p, fld)
+                terms += ctx.last_stats().pairs_nominal
+            b.record(stream)
+            ev.append((a, b))
+        ctx.sync()
+        ms = float(np.median([a.elapsed_ms(b) for a, b in ev]))
+        out[f"points_{n}"] = {"ms_per_hologram": round(ms, 3),
+                              "window_terms_per_s": float(f"{terms / launches / (ms / 1e3):.4g}")}
+        dv.free()
+        fld.free()
+    return out
 
 
 def fresnel_variants(ctx, capi, synth, Event, launches=2):
@@ -430,6 +468,9 @@ def main():
     for b in [dv] + fields + encs + u8s:
         b.free()
     var = fresnel_variants(ctx, capi, synth, Event) if args.variants else None
+    wrp = wrp_block(ctx, capi, synth, Event) if args.wrp else None
+    if wrp:
+        print(f"# wrp: {wrp}", file=sys.stderr, flush=True)
     ctx.close()
     serving = None
     if args.serving:
@@ -467,6 +508,8 @@ def main():
     }
     if var:
         line["variants"] = var
+    if wrp:
+        line["wrp"] = wrp
     if serving:
         line["serving"] = serving
     print(json.dumps(line), flush=True)

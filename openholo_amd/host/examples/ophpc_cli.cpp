@@ -1,9 +1,10 @@
-// Command-line point-cloud generator with the reference CLI's flags
-// (Colab/OpenholoGeneration.cpp:119-195, point-cloud algorithm only):
-//   -a 0  -c config.xml  -i points.ply  -e encode  -m 0|1  -p 0|1  -f 0|1
-// -m 0: MODE_FLOAT|MODE_FASTMATH, -m 1: MODE_DOUBLE; -p 1: MODE_GPU (both -p
-// values run on the GPU here); -f 0: RS, 1: Fresnel.  Writes
-// <out>/PointCloud_<R-S|Fresnel>_<Phase|...>.bmp like ex_PointCloud.
+// Command-line generator with the reference CLI's flags
+// (Colab/OpenholoGeneration.cpp:119-195, point-cloud and WRP algorithms):
+//   -a 0|4  -c config.xml  -i points.ply  -e encode  -m 0|1  -p 0|1  -f 0|1
+// -a 0: point cloud, -a 4: WRP; -m 0: MODE_FLOAT|MODE_FASTMATH, -m 1:
+// MODE_DOUBLE; -p 1: MODE_GPU (both -p values run on the GPU here); -f 0: RS,
+// 1: Fresnel.  Writes <out>/PointCloud_<R-S|Fresnel>_<Phase|...>.bmp like
+// ex_PointCloud, or <out>/WRP_<Phase|...>.bmp like ex_WRP (:100-117).
 // Extensions used by the test suite:
 //   -o DIR            output directory (default Result)
 //   --vtx FILE        raw 56-byte Vertex records instead of -i
@@ -15,6 +16,7 @@
 //   --no-generate     stop after loading (with --dump-points / --dump-config)
 //   --dump-points F   raw 56-byte Vertex records as loaded
 //   --dump-config     print the parsed configuration as one JSON line
+//   --dump-wrp F      (WRP) raw complex128 WRP plane of the last channel
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -23,6 +25,7 @@
 #include <vector>
 
 #include "ophPointCloud.h"
+#include "ophWRP.h"
 
 static const char* encode_name(unsigned e)
 {
@@ -35,9 +38,64 @@ static const char* encode_name(unsigned e)
     }
 }
 
+static bool write_raw(const std::string& path, const void* p, size_t bytes)
+{
+    FILE* fp = std::fopen(path.c_str(), "wb");
+    if (!fp) return false;
+    const bool ok = std::fwrite(p, 1, bytes, fp) == bytes;
+    std::fclose(fp);
+    return ok;
+}
+
+// ex_WRP (Colab/OpenholoGeneration.cpp:100-117): readConfig, loadPointCloud,
+// generateHologram, encoding, normalize, save (waveNum x 8 bits per pixel)
+static int run_wrp(const std::string& config, const std::string& input, unsigned mode, unsigned encode,
+                   const std::string& outdir, const std::string& dump_points, const std::string& dump_field,
+                   const std::string& dump_enc, const std::string& dump_u8, const std::string& dump_wrp)
+{
+    ophWRP* h = new ophWRP();
+    h->SetMode(mode);
+    if (!h->readConfig(config.c_str())) return 1;
+    if (h->loadPointCloud(input.c_str()) < 0) return 1;
+    if (!dump_points.empty() &&
+        !write_raw(dump_points, h->getPointCloudModel(), sizeof(Vertex) * h->getNumberOfPoints()))
+        return 1;
+    h->generateHologram();
+    if (!h->lastError().empty()) {
+        std::fprintf(stderr, "generateHologram failed: %s\n", h->lastError().c_str());
+        return 1;
+    }
+    const uint nch = h->getContext().waveNum;
+    const long long npx = (long long)h->getContext().pixel_number[_X] * h->getContext().pixel_number[_Y];
+    if (!dump_field.empty()) {
+        FILE* fp = std::fopen(dump_field.c_str(), "wb");
+        for (uint c = 0; c < nch; ++c) std::fwrite(h->getComplexField()[c], 16, npx, fp);
+        std::fclose(fp);
+    }
+    if (!dump_wrp.empty() && !write_raw(dump_wrp, h->getWRPBuff(), 16 * (size_t)npx)) return 1;
+    h->encoding(encode);
+    h->normalize();
+    if (!dump_enc.empty()) {
+        FILE* fp = std::fopen(dump_enc.c_str(), "wb");
+        for (uint c = 0; c < nch; ++c) std::fwrite(h->getEncodedBuffer()[c], 8, npx, fp);
+        std::fclose(fp);
+    }
+    if (!dump_u8.empty()) {
+        FILE* fp = std::fopen(dump_u8.c_str(), "wb");
+        for (uint c = 0; c < nch; ++c) std::fwrite(h->getNormalizedBuffer()[c], 1, npx, fp);
+        std::fclose(fp);
+    }
+    ::mkdir(outdir.c_str(), 0755);
+    char path[1024];
+    std::snprintf(path, sizeof path, "%s/WRP_%s.bmp", outdir.c_str(), encode_name(encode));
+    const bool ok = h->save(path, (uint8_t)(nch * 8));
+    h->release();
+    return ok ? 0 : 1;
+}
+
 int main(int argc, char** argv)
 {
-    std::string config, input, vtx, outdir = "Result", dump_field, dump_enc, dump_u8, ohc, dump_points;
+    std::string config, input, vtx, outdir = "Result", dump_field, dump_enc, dump_u8, ohc, dump_points, dump_wrp;
     bool no_generate = false, dump_config = false;
     int alg = 0, flag = 0;
     unsigned encode = 0, mode = 0;
@@ -69,13 +127,16 @@ int main(int argc, char** argv)
         else if (a == "--dump-points") dump_points = v, ++i;
         else if (a == "--no-generate") no_generate = true;
         else if (a == "--dump-config") dump_config = true;
+        else if (a == "--dump-wrp") dump_wrp = v, ++i;
         else {
             std::fprintf(stderr, "unknown argument %s\n", a.c_str());
             return 2;
         }
     }
+    if (alg == 4)
+        return run_wrp(config, input, mode, encode, outdir, dump_points, dump_field, dump_enc, dump_u8, dump_wrp);
     if (alg != 0) {
-        std::fprintf(stderr, "only the point-cloud algorithm (-a 0) is provided\n");
+        std::fprintf(stderr, "only the point-cloud (-a 0) and WRP (-a 4) algorithms are provided\n");
         return 2;
     }
     ophPointCloud* h = new ophPointCloud();

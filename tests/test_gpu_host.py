@@ -105,3 +105,39 @@ def test_cli_mfma_fresnel(tmp_path, orc):
     pix = np.random.default_rng(9).choice(1920 * 1080, 1500, replace=False)
     for ch, lam in enumerate([638e-9, 520e-9, 450e-9]):
         assert rel_l2(f[ch][pix], orc.field_exact(g, v, lam, ch, 1, pix)) < 1e-4
+
+
+def test_cli_wrp_end_to_end(tmp_path, orc):
+    """ex_WRP's call sequence (-a 4) on the reference's WRP configuration and
+    point cloud: each channel's hologram and the last channel's WRP against
+    the numpy oracle (oracle/wrp_oracle.py), then the encoder, normaliser and
+    merged 24-bit BMP."""
+    from oracle import wrp_oracle as wo
+    d = tmp_path
+    cfg = os.path.join(GOLD, "Generation_WRP_RGB.xml")
+    run(os.path.join(BIN, "ophpc_cli"), "-a", "4", "-c", cfg, "-i", PLY, "-e", "0", "-m", "1", "-p", "1",
+        "-o", str(d / "Result"), "--dump-points", str(d / "p.bin"), "--dump-field", str(d / "f.bin"),
+        "--dump-enc", str(d / "e.bin"), "--dump-u8", str(d / "u.bin"), "--dump-wrp", str(d / "w.bin"))
+    pn, pp, lams = (1920, 1080), (1e-5, 1e-5), [638e-9, 520e-9, 450e-9]
+    scale, loc, dist = (0.0015, 0.0015, 0.0003), 0.003, 0.5
+    npx = pn[0] * pn[1]
+    v = np.fromfile(d / "p.bin", VERTEX)
+    f = np.fromfile(d / "f.bin", np.complex128).reshape(3, pn[1], pn[0])
+    e = np.fromfile(d / "e.bin", np.float64).reshape(3, npx)
+    u = np.fromfile(d / "u.bin", np.uint8).reshape(3, npx)
+    wrp = np.fromfile(d / "w.bin", np.complex128).reshape(pn[1], pn[0])
+    for ch, lam in enumerate(lams):
+        want_wrp, want = wo.generate(v, pn, pp, scale, lam, ch, loc, dist)
+        assert rel_l2(f[ch], want) < 1e-10, ch
+        if ch == 2:
+            assert rel_l2(wrp, want_wrp) < 1e-12
+        assert np.allclose(e[ch], orc.encode(f[ch].ravel(), orc.ENCODE_PHASE), rtol=0, atol=4e-16)
+    want_u = orc.normalize(list(e))
+    for ch in range(3):
+        assert np.array_equal(u[ch], want_u[ch])
+    bmp = open(d / "Result" / "WRP_Phase.bmp", "rb").read()
+    size, _, off = struct.unpack("<IIi", bmp[2:14])
+    w, h, planes, bpp = struct.unpack("<iiHH", bmp[18:30])
+    assert (size, w, h, planes, bpp) == (len(bmp), 1920, 1080, 1, 24)
+    pixels = np.frombuffer(bmp[off:], np.uint8).reshape(1080, 1920, 3)
+    assert np.array_equal(pixels, np.stack([u[2], u[1], u[0]], axis=-1).reshape(1080, 1920, 3))

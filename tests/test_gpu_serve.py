@@ -83,6 +83,28 @@ def test_alone_equals_in_mix(use_graph):
         assert (c.graph_replays > 0) == use_graph
 
 
+def test_fp64_jobs_in_mix():
+    """MODE_DOUBLE jobs (the FP64 tiles of the multi-job kernel, 128x16) next to
+    FP32 and matrix-core jobs: each equals the one-shot FP64 launch to FP64
+    rounding and is bit-identical alone or in the mix."""
+    fp64 = capi.MODE_GPU
+    specs = [(61, 1800, (1920, 1080), (8e-6, 8e-6), 532e-9, 0, fp64),
+             (62, 6000, (1000, 700), (8e-6, 8e-6), 638e-9, 1, fp64)]
+    with capi.Server(slice_pairs=2e9, slice_pairs_mfma=5e9) as srv:
+        alone = []
+        for spec in specs:
+            st, stats, f, _ = srv.wait(submit(srv, spec), 120)
+            assert st == capi.OK and stats.iterations >= 2
+            seed, n, pn, pp, lam, diff, mode = spec
+            ref = oneshot(cloud(seed, n), capi.params(pn, pp, lam), diff, mode)
+            assert rel_l2(f, ref) < 1e-12
+            alone.append(f)
+        ids = [submit(srv, s) for s in [JOBS[0], specs[0], JOBS[2], specs[1]]]
+        res = [srv.wait(j, 300) for j in ids]
+        assert all(r[0] == capi.OK for r in res)
+        assert np.array_equal(res[1][2], alone[0]) and np.array_equal(res[3][2], alone[1])
+
+
 def test_graph_and_direct_launch_agree():
     fields = []
     for g in (True, False):

@@ -24,8 +24,11 @@ struct ohx_ctx {
     // partial planes of the direct kernels (grown on demand)
     void* gemm_scratch = nullptr;
     size_t gemm_scratch_bytes = 0;
-    // hipFFT plan of the WRP propagation (hipfftHandle), for an L x M grid
-    void* wrp_plan = nullptr;
+    // hipFFT plans of the WRP propagation (hipfftHandle) for an L x M grid:
+    // length-L transforms of ceil(M/4) contiguous rows of the transposed
+    // grid, length-M transforms of all L rows
+    void* wrp_plan_t = nullptr;
+    void* wrp_plan_a = nullptr;
     int wrp_plan_L = 0, wrp_plan_M = 0;
     ohx_launch_stats last{};
     bool boxsum_pending = false;   // last.pairs_in_box still on the device (minmax[4])

@@ -21,12 +21,18 @@
 //    pixel is amplitude x table entry.
 //  * Deterministic gather instead of the reference's atomics: points are
 //    sorted by the cell of their window centre (hipCUB radix sort, stable:
-//    point order within a cell), and each padded-plane pixel sums, in cell
-//    then point order, the points whose window covers it (k_wrp_gather).  The
-//    gather writes the zero-padded, centre-shifted FFT input directly.
-//  * hipFFT Z2Z forward / inverse in place; k_wrp_asm multiplies by the
-//    transfer function at the shifted frequency index, k_wrp_crop undoes the
-//    output shift while cropping.
+//    point order within a cell); a workgroup per 8 x 8 pixel tile stages
+//    the points that reach its tile in LDS, in sorted order, and each pixel
+//    sums those whose window covers it (k_wrp_gather), with the window table
+//    in LDS.  The gather writes the plane samples of the zero-padded,
+//    centre-shifted FFT input directly, k_wrp_pad_zero the rest.
+//  * hipFFT Z2Z forward / inverse, each as two passes of contiguous 1D
+//    transforms with a tiled LDS transpose between them; the input is held
+//    transposed so that the pass along columns comes first and covers only
+//    the columns that hold the plane (forward) or that the crop reads
+//    (inverse).  k_wrp_asm multiplies by the transfer function at the
+//    shifted frequency index, k_wrp_crop undoes the output shift while
+//    cropping and transposing back.
 // Shifts are out of place (the reference's Openholo::fft2 calls fftShift
 // with src == dst there, Openholo/src/Openholo.cpp:716, which overwrites half
 // of every quadrant swap; the shifts here do what that function intends).
@@ -117,17 +123,21 @@ __global__ __launch_bounds__(256) void k_wrp_prep(const double* __restrict__ vtx
     amp[i] = v[3 + wg.channel];
 }
 
-__global__ __launch_bounds__(256) void k_wrp_count(const uint32_t* __restrict__ key, int64_t n, uint32_t ncells,
-                                                  uint32_t* __restrict__ count)
+// start[c] = first sorted position whose key is >= c, c in [0, ncells]:
+// the points of cell c are [start[c], start[c + 1]) (binary search over the
+// sorted keys; the off-plane key ncells sorts last)
+__global__ __launch_bounds__(256) void k_wrp_cell_start(const uint32_t* __restrict__ key_s, int64_t n,
+                                                       uint32_t ncells, uint32_t* __restrict__ start)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n && key[i] < ncells) atomicAdd(&count[key[i]], 1u);
-}
-
-__global__ __launch_bounds__(256) void k_wrp_zero_u32(uint32_t* __restrict__ p, int64_t n)
-{
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0u;
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c > (int64_t)ncells) return;
+    int64_t lo = 0, hi = n;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (key_s[mid] < (uint32_t)c) lo = mid + 1;
+        else hi = mid;
+    }
+    start[c] = (uint32_t)lo;
 }
 
 // point data in cell order
@@ -161,47 +171,208 @@ __global__ __launch_bounds__(256) void k_wrp_table(int w, double pp_x, double pp
     tab[i] = make_double2((cos(kr) * cos(rp)) / r, (-sin(kr) * sin(rp)) / r);
 }
 
-// One thread per sample of the shifted, zero-padded FFT input (L = 2 pn_y
-// rows, M = 2 pn_x columns).  Sample (a, b) holds padded pixel
-// (Y, X) = ((a + pn_y) mod L, (b + pn_x) mod M), i.e. WRP pixel
-// (Y - pn_y/2, X - pn_x/2) when that lies on the plane, else zero.
-__global__ __launch_bounds__(256) void k_wrp_gather(WrpGeom wg, const uint32_t* __restrict__ cell_start,
-                                                   const int2* __restrict__ stxy, const double* __restrict__ samp,
-                                                   const double2* __restrict__ tab, double2* __restrict__ fft_in,
-                                                   double2* __restrict__ wrp_out)
+// Padded FFT input: sample (a, b) of the L x M grid (L = 2 pn_y, M = 2 pn_x)
+// holds padded pixel (Y, X) = ((a + pn_y) mod L, (b + pn_x) mod M), i.e. WRP
+// pixel (Y - pn_y/2, X - pn_x/2) when that lies on the plane, else zero.  It
+// is stored transposed, T[b][a] (M rows of L samples), so that the first
+// (column) pass of the FFT runs over contiguous rows.  Nonzero samples have
+// a in S(L, cy) and b in S(M, cx), where S(n, c) = [n - c, n) u [0, c) and
+// cx = ceil(pn_x / 2), cy = ceil(pn_y / 2); position q in [0, 2c) of S(n, c)
+// is index sel(q) below.
+__device__ __forceinline__ int sel(int q, int n, int c) { return q < c ? n - c + q : q - c; }
+
+// zero samples of the rows T[b], b in S(M, cx) (the rows the first pass
+// reads); k_wrp_gather writes the plane samples
+__global__ __launch_bounds__(256) void k_wrp_pad_zero(int pn_x, int pn_y, double2* __restrict__ ft)
 {
-    const int L = 2 * wg.pn_y, M = 2 * wg.pn_x;
-    const int b = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int a = blockIdx.y * 4 + (threadIdx.x >> 6);
-    if (a >= L || b >= M) return;
-    const int Y = a + wg.pn_y < L ? a + wg.pn_y : a + wg.pn_y - L;
-    const int X = b + wg.pn_x < M ? b + wg.pn_x : b + wg.pn_x - M;
-    const int row = Y - (wg.pn_y >> 1), col = X - (wg.pn_x >> 1);
+    const int L = 2 * pn_y, M = 2 * pn_x, cx = (pn_x + 1) / 2;
+    const int a = blockIdx.x * 256 + threadIdx.x;
+    const int q = blockIdx.y;
+    if (a >= L || q >= 2 * cx) return;
+    const int b = sel(q, M, cx);
+    const int Y = a + pn_y < L ? a + pn_y : a + pn_y - L;
+    const int X = b + pn_x < M ? b + pn_x : b + pn_x - M;
+    const int row = Y - (pn_y >> 1), col = X - (pn_x >> 1);
+    if (row >= 0 && row < pn_y && col >= 0 && col < pn_x) return;
+    ft[(int64_t)b * L + a] = make_double2(0.0, 0.0);
+}
+
+// 32 x 32 tiles through LDS, 256 threads (32 x 8)
+constexpr int TT = 32;
+
+// A[a][b] = T[b][a] for b in S(M, cx), 0 elsewhere (after the first pass)
+__global__ __launch_bounds__(256) void k_wrp_t2a(const double2* __restrict__ ft, double2* __restrict__ fa, int L,
+                                                int M, int cx)
+{
+    __shared__ double2 s[TT][TT + 1];
+    const int a0 = blockIdx.x * TT, b0 = blockIdx.y * TT;
+    const int tx = threadIdx.x & (TT - 1), ty = threadIdx.x / TT;
+    for (int i = ty; i < TT; i += 256 / TT) {
+        const int b = b0 + i, a = a0 + tx;
+        double2 v = make_double2(0.0, 0.0);
+        if (b < M && a < L && (b >= M - cx || b < cx)) v = ft[(int64_t)b * L + a];
+        s[i][tx] = v;
+    }
+    __syncthreads();
+    for (int j = ty; j < TT; j += 256 / TT) {
+        const int a = a0 + j, b = b0 + tx;
+        if (a < L && b < M) fa[(int64_t)a * M + b] = s[tx][j];
+    }
+}
+
+// T[b][a] = A[a][b] for b in S(M, cx), all a (before the last pass)
+__global__ __launch_bounds__(256) void k_wrp_a2t(const double2* __restrict__ fa, double2* __restrict__ ft, int L,
+                                                int M, int cx)
+{
+    __shared__ double2 s[TT][TT + 1];
+    const int a0 = blockIdx.x * TT, q0 = blockIdx.y * TT;
+    const int tx = threadIdx.x & (TT - 1), ty = threadIdx.x / TT;
+    for (int j = ty; j < TT; j += 256 / TT) {
+        const int a = a0 + j, q = q0 + tx;
+        if (a < L && q < 2 * cx) s[j][tx] = fa[(int64_t)a * M + sel(q, M, cx)];
+    }
+    __syncthreads();
+    for (int i = ty; i < TT; i += 256 / TT) {
+        const int q = q0 + i, a = a0 + tx;
+        if (q < 2 * cx && a < L) ft[(int64_t)sel(q, M, cx) * L + a] = s[tx][i];
+    }
+}
+
+constexpr int GT = 8;           // gather tile edge: 8 x 8 pixels, one pixel per lane
+constexpr int GW = 4;           // wavefronts per tile, each adding every GW-th group of 4 kept points
+constexpr int GCAP = 512;       // kept points staged in LDS per pass (+ one 256-point append + padding)
+constexpr int GTAB_LDS = 3600;  // window tables up to this many entries (w <= 30, 56 KB) are staged in LDS
+
+struct alignas(16) WrpCand {
+    int tx, ty;   // window centre
+    double a;     // amplitude
+};
+
+// One workgroup of GW wavefronts per 8 x 8 tile of the WRP plane.  The
+// points whose window centre (tx, ty) can reach the tile,
+// tx in [c0 - w + 1, c0 + 7 + w] (same for y), lie in a few rows of cells;
+// each row of cells is one contiguous range of the cell-sorted points.  The
+// workgroup scans those ranges 256 points at a time and keeps, in scan order
+// (ballot + prefix counts), the points that reach the tile.  Wavefront q adds
+// the kept points of groups q, q + GW, q + 2 GW, ... (4 points a group, in
+// order) for all 64 pixels; the GW partial sums are then added in q order.
+// The order of every addition is fixed by the sorted point order, so the
+// result is the same on every launch.  The add loop is branch-free: a point
+// that misses the lane's pixel adds amplitude 0 times table entry 0 (finite,
+// as r >= |dz| > 0), which leaves the partial sum bit-unchanged.  Window
+// table in LDS when TL (4 w^2 <= GTAB_LDS; dynamic LDS of 4 w^2 entries),
+// else read from global memory.
+template<bool TL>
+__global__ __launch_bounds__(64 * GW) void k_wrp_gather(WrpGeom wg, const uint32_t* __restrict__ cell_start,
+                                                       const int2* __restrict__ stxy,
+                                                       const double* __restrict__ samp,
+                                                       const double2* __restrict__ tab,
+                                                       double2* __restrict__ fft_in, double2* __restrict__ wrp_out)
+{
+    extern __shared__ double2 s_tab[];
+    __shared__ WrpCand s_c[GCAP + 64 * GW + 4 * GW];
+    __shared__ int s_wcnt[GW];
+    __shared__ double2 s_part[GW - 1][64];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int c0 = blockIdx.x * GT, r0 = blockIdx.y * GT;
+    const int col = c0 + (lane & (GT - 1)), row = r0 + lane / GT;
+    const int w = wg.w, n2 = 2 * w;
+    const int txlo = c0 - w + 1, txhi = c0 + GT - 1 + w;
+    const int tylo = r0 - w + 1, tyhi = r0 + GT - 1 + w;
+    // cells hold window centres by (t + w) / cell, t + w >= 0
+    const int cx0 = (c0 + 1) / wg.cell, cx1 = min((c0 + GT - 1 + n2) / wg.cell, wg.cells_x - 1);
+    const int cy0 = (r0 + 1) / wg.cell, cy1 = min((r0 + GT - 1 + n2) / wg.cell, wg.cells_y - 1);
+    uint32_t total = 0;
+    for (int cy = cy0; cy <= cy1; ++cy)
+        total += cell_start[cy * wg.cells_x + cx1 + 1] - cell_start[cy * wg.cells_x + cx0];
+    if (total == 0) {   // uniform over the workgroup
+        if (wave == 0 && row < wg.pn_y && col < wg.pn_x) {
+            const double2 z = make_double2(0.0, 0.0);
+            if (wrp_out) wrp_out[(int64_t)row * wg.pn_x + col] = z;
+            const int Y = row + (wg.pn_y >> 1), X = col + (wg.pn_x >> 1);
+            const int a = Y >= wg.pn_y ? Y - wg.pn_y : Y + wg.pn_y;
+            const int b = X >= wg.pn_x ? X - wg.pn_x : X + wg.pn_x;
+            fft_in[(int64_t)b * (2 * wg.pn_y) + a] = z;
+        }
+        return;
+    }
+
+    if constexpr (TL)
+        for (int i = tid; i < n2 * n2; i += 64 * GW) s_tab[i] = tab[i];
     double2 acc = make_double2(0.0, 0.0);
-    if (row >= 0 && row < wg.pn_y && col >= 0 && col < wg.pn_x) {
-        const int w = wg.w, n = 2 * w;
-        // window centres tx in [col - w + 1, col + w], i.e. tx + w in [col + 1, col + 2w]
-        const int cx0 = (col + 1) / wg.cell, cx1 = min((col + n) / wg.cell, wg.cells_x - 1);
-        const int cy0 = (row + 1) / wg.cell, cy1 = min((row + n) / wg.cell, wg.cells_y - 1);
-        for (int cy = cy0; cy <= cy1; ++cy) {
-            for (int cx = cx0; cx <= cx1; ++cx) {
-                const int c = cy * wg.cells_x + cx;
-                const uint32_t p0 = cell_start[c], p1 = cell_start[c + 1];
-                for (uint32_t p = p0; p < p1; ++p) {
-                    const int2 t = stxy[p];
-                    const int wx = col - t.x, wy = row - t.y;
-                    if (wx >= -w && wx < w && wy >= -w && wy < w) {
-                        const double2 k = tab[(wy + w) * n + (wx + w)];
-                        const double am = samp[p];
-                        acc.x = fma(am, k.x, acc.x);
-                        acc.y = fma(am, k.y, acc.y);
-                    }
-                }
+    const int cx = col + w, cy = row + w;
+    auto add = [&](const WrpCand& q) {
+        const int wx = cx - q.tx, wy = cy - q.ty;
+        const bool hit = (unsigned)wx < (unsigned)n2 && (unsigned)wy < (unsigned)n2;
+        const int ti = hit ? wy * n2 + wx : 0;
+        const double am = hit ? q.a : 0.0;
+        double2 k;
+        if constexpr (TL) k = s_tab[ti];
+        else k = tab[ti];
+        acc.x = fma(am, k.x, acc.x);
+        acc.y = fma(am, k.y, acc.y);
+    };
+    auto consume = [&](int m) {
+        // pad to whole groups of 4 with points no pixel reaches
+        if (tid < 4) s_c[m + tid] = WrpCand{-(1 << 29), -(1 << 29), 0.0};
+        __syncthreads();
+        for (int j = 4 * wave; j < m; j += 4 * GW) {
+            const WrpCand q0 = s_c[j], q1 = s_c[j + 1], q2 = s_c[j + 2], q3 = s_c[j + 3];
+            add(q0);
+            add(q1);
+            add(q2);
+            add(q3);
+        }
+        __syncthreads();
+    };
+    int cnt = 0;   // kept points staged; the same value in every lane
+    for (int cyr = cy0; cyr <= cy1; ++cyr) {
+        const uint32_t s = cell_start[cyr * wg.cells_x + cx0], e = cell_start[cyr * wg.cells_x + cx1 + 1];
+        for (uint32_t base = s; base < e; base += 64 * GW) {
+            const uint32_t p = base + tid;
+            bool take = false;
+            int2 t = make_int2(0, 0);
+            if (p < e) {
+                t = stxy[p];
+                take = t.x >= txlo && t.x <= txhi && t.y >= tylo && t.y <= tyhi;
+            }
+            const unsigned long long m = __ballot(take);
+            if (lane == 0) s_wcnt[wave] = __popcll(m);
+            __syncthreads();
+            int off = cnt, all = 0;
+            for (int k = 0; k < GW; ++k) {
+                const int ck = s_wcnt[k];
+                off += k < wave ? ck : 0;
+                all += ck;
+            }
+            if (take) s_c[off + __popcll(m & ((1ull << lane) - 1ull))] = WrpCand{t.x, t.y, samp[p]};
+            cnt += all;
+            __syncthreads();
+            if (cnt >= GCAP) {
+                consume(cnt);
+                cnt = 0;
             }
         }
-        if (wrp_out) wrp_out[(int64_t)row * wg.pn_x + col] = acc;
     }
-    fft_in[(int64_t)a * M + b] = acc;
+    if (cnt > 0) consume(cnt);
+
+    // partial sums of waves 1..GW-1 to wave 0, added in wave order
+    if (wave > 0) s_part[wave - 1][lane] = acc;
+    __syncthreads();
+    if (wave == 0) {
+        for (int q = 0; q < GW - 1; ++q) {
+            const double2 v = s_part[q][lane];
+            acc.x += v.x;
+            acc.y += v.y;
+        }
+        if (row < wg.pn_y && col < wg.pn_x) {
+            if (wrp_out) wrp_out[(int64_t)row * wg.pn_x + col] = acc;
+            const int Y = row + (wg.pn_y >> 1), X = col + (wg.pn_x >> 1);
+            const int a = Y >= wg.pn_y ? Y - wg.pn_y : Y + wg.pn_y;
+            const int b = X >= wg.pn_x ? X - wg.pn_x : X + wg.pn_x;
+            fft_in[(int64_t)b * (2 * wg.pn_y) + a] = acc;
+        }
+    }
 }
 
 // F[u][v] *= exp(j 2pi d sqrt(1/lambda^2 - fx_j^2 - fy_i^2)) with the
@@ -231,18 +402,29 @@ __global__ __launch_bounds__(256) void k_wrp_asm(double2* __restrict__ f, int pn
     f[id] = make_double2(z.x * cs - z.y * sn, z.x * sn + z.y * cs);
 }
 
-// out[y][x] = G[(y + pn_y/2 + pn_y) mod L][(x + pn_x/2 + pn_x) mod M]
-__global__ __launch_bounds__(256) void k_wrp_crop(const double2* __restrict__ g, int pn_x, int pn_y,
+// out[y][x] = G[(y + pn_y/2 + pn_y) mod L][(x + pn_x/2 + pn_x) mod M], with
+// G stored transposed (T[b][a] = G[a][b])
+__global__ __launch_bounds__(256) void k_wrp_crop(const double2* __restrict__ ft, int pn_x, int pn_y,
                                                  double2* __restrict__ out)
 {
+    __shared__ double2 s[TT][TT + 1];
     const int L = 2 * pn_y, M = 2 * pn_x;
-    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
-    if (y >= pn_y || x >= pn_x) return;
-    int Y = y + (pn_y >> 1) + pn_y, X = x + (pn_x >> 1) + pn_x;
-    if (Y >= L) Y -= L;
-    if (X >= M) X -= M;
-    out[(int64_t)y * pn_x + x] = g[(int64_t)Y * M + X];
+    const int x0 = blockIdx.x * TT, y0 = blockIdx.y * TT;
+    const int tx = threadIdx.x & (TT - 1), ty = threadIdx.x / TT;
+    for (int i = ty; i < TT; i += 256 / TT) {
+        const int x = x0 + i, y = y0 + tx;
+        if (x < pn_x && y < pn_y) {
+            int a = y + (pn_y >> 1) + pn_y, b = x + (pn_x >> 1) + pn_x;
+            if (a >= L) a -= L;
+            if (b >= M) b -= M;
+            s[i][tx] = ft[(int64_t)b * L + a];
+        }
+    }
+    __syncthreads();
+    for (int j = ty; j < TT; j += 256 / TT) {
+        const int y = y0 + j, x = x0 + tx;
+        if (x < pn_x && y < pn_y) out[(int64_t)y * pn_x + x] = s[tx][j];
+    }
 }
 
 __global__ __launch_bounds__(256) void k_wrp_zero2(double2* __restrict__ p, int64_t n)
@@ -262,8 +444,8 @@ int wrp_generate(ohx_ctx* c, const void* d_vertices, int64_t n, const ohx_wrp_pa
     const int pn_x = p->pn_x, pn_y = p->pn_y;
     const int64_t npx = (int64_t)pn_x * pn_y;
     const int L = 2 * pn_y, M = 2 * pn_x;
+    const int cx = (pn_x + 1) / 2;   // the columns b in S(M, cx) hold the plane
     hipStream_t st = c->stream;
-    const unsigned gx = (unsigned)((pn_x + 63) / 64), gy = (unsigned)((pn_y + 3) / 4);
     const dim3 pgrid((unsigned)((M + 63) / 64), (unsigned)((L + 3) / 4));
 
     // zmax and the window
@@ -305,21 +487,24 @@ int wrp_generate(ohx_ctx* c, const void* d_vertices, int64_t n, const ohx_wrp_pa
     wg.cells_y = (pn_y + 2 * w + wg.cell - 1) / wg.cell;
     const uint32_t ncells = (uint32_t)wg.cells_x * (uint32_t)wg.cells_y;
     const int64_t np = std::max<int64_t>(n, 1);
-    size_t sort_bytes = 0, scan_bytes = 0;
-    hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, (uint32_t*)nullptr, (uint32_t*)nullptr,
-                                       (uint32_t*)nullptr, (uint32_t*)nullptr, (int)np, 0, 32, st);
-    hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, (uint32_t*)nullptr, (uint32_t*)nullptr,
-                                     (int)(ncells + 1), st);
+    int key_bits = 1;   // keys are in [0, ncells]
+    while (key_bits < 32 && (ncells >> key_bits) != 0) ++key_bits;
+    size_t sort_bytes = 0;
+    if (hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, (uint32_t*)nullptr, (uint32_t*)nullptr,
+                                           (uint32_t*)nullptr, (uint32_t*)nullptr, (int)np, 0, key_bits, st) !=
+        hipSuccess)
+        return set_error(c, OHX_ERR_DEVICE, "WRP sort size query failed");
     const size_t b_fft = align256((size_t)L * M * sizeof(double2));
     const size_t b_tab = align256((size_t)std::max(1, 4 * w * w) * sizeof(double2));
     const size_t b_u32 = align256((size_t)np * 4);
     const size_t b_i2 = align256((size_t)np * 8);
     const size_t b_cells = align256((size_t)(ncells + 1) * 4);
-    const size_t b_tmp = align256(std::max(sort_bytes, scan_bytes));
-    const size_t total = b_fft + b_tab + 4 * b_u32 + 2 * b_i2 + 2 * b_i2 + 2 * b_cells + b_tmp;
+    const size_t b_tmp = align256(sort_bytes);
+    const size_t total = 2 * b_fft + b_tab + 4 * b_u32 + 2 * b_i2 + 2 * b_i2 + b_cells + b_tmp;
     if ((st_ = ensure_gemm_scratch(c, total))) return st_;
     char* q = (char*)c->gemm_scratch;
-    double2* fft = (double2*)q;              q += b_fft;
+    double2* ft = (double2*)q;               q += b_fft;   // transposed grid T[b][a]
+    double2* fa = (double2*)q;               q += b_fft;   // grid A[a][b]
     double2* tab = (double2*)q;              q += b_tab;
     uint32_t* key = (uint32_t*)q;            q += b_u32;
     uint32_t* key_s = (uint32_t*)q;          q += b_u32;
@@ -329,7 +514,6 @@ int wrp_generate(ohx_ctx* c, const void* d_vertices, int64_t n, const ohx_wrp_pa
     double* amp = (double*)q;                q += b_i2;
     int2* stxy = (int2*)q;                   q += b_i2;
     double* samp = (double*)q;               q += b_i2;
-    uint32_t* count = (uint32_t*)q;          q += b_cells;
     uint32_t* start = (uint32_t*)q;          q += b_cells;
     void* tmp = (void*)q;
 
@@ -337,23 +521,28 @@ int wrp_generate(ohx_ctx* c, const void* d_vertices, int64_t n, const ohx_wrp_pa
         const unsigned pb = (unsigned)((n + 255) / 256);
         hipLaunchKernelGGL(k_wrp_prep, dim3(pb), dim3(256), 0, st, (const double*)d_vertices, n, wg, key, idx, txy,
                            amp);
-        if (hipcub::DeviceRadixSort::SortPairs(tmp, sort_bytes, key, key_s, idx, idx_s, (int)n, 0, 32, st) !=
+        if (hipcub::DeviceRadixSort::SortPairs(tmp, sort_bytes, key, key_s, idx, idx_s, (int)n, 0, key_bits, st) !=
             hipSuccess)
             return set_error(c, OHX_ERR_DEVICE, "WRP point sort failed");
-        hipLaunchKernelGGL(k_wrp_zero_u32, dim3((ncells + 1 + 255) / 256), dim3(256), 0, st, count,
-                           (int64_t)ncells + 1);
-        hipLaunchKernelGGL(k_wrp_count, dim3(pb), dim3(256), 0, st, (const uint32_t*)key, n, ncells, count);
-        if (hipcub::DeviceScan::ExclusiveSum(tmp, scan_bytes, count, start, (int)(ncells + 1), st) != hipSuccess)
-            return set_error(c, OHX_ERR_DEVICE, "WRP cell scan failed");
+        hipLaunchKernelGGL(k_wrp_cell_start, dim3((ncells + 1 + 255) / 256), dim3(256), 0, st,
+                           (const uint32_t*)key_s, n, ncells, start);
         hipLaunchKernelGGL(k_wrp_permute, dim3(pb), dim3(256), 0, st, (const uint32_t*)idx_s, n,
                            (const int2*)txy, (const double*)amp, stxy, samp);
         const int nt = 4 * w * w;
         hipLaunchKernelGGL(k_wrp_table, dim3((nt + 255) / 256), dim3(256), 0, st, w, p->pp_x, p->pp_y, dz, k, tab);
-        hipLaunchKernelGGL(k_wrp_gather, pgrid, dim3(256), 0, st, wg, (const uint32_t*)start,
-                           (const int2*)stxy, (const double*)samp, (const double2*)tab, fft, (double2*)d_wrp);
+        hipLaunchKernelGGL(k_wrp_pad_zero, dim3((unsigned)((L + 255) / 256), (unsigned)(2 * cx)), dim3(256), 0, st,
+                           pn_x, pn_y, ft);
+        const dim3 tgrid((unsigned)((pn_x + GT - 1) / GT), (unsigned)((pn_y + GT - 1) / GT));
+        if (nt <= GTAB_LDS)
+            hipLaunchKernelGGL(k_wrp_gather<true>, tgrid, dim3(64 * GW), (size_t)nt * sizeof(double2), st, wg,
+                               (const uint32_t*)start, (const int2*)stxy, (const double*)samp, (const double2*)tab,
+                               ft, (double2*)d_wrp);
+        else
+            hipLaunchKernelGGL(k_wrp_gather<false>, tgrid, dim3(64 * GW), 0, st, wg, (const uint32_t*)start,
+                               (const int2*)stxy, (const double*)samp, (const double2*)tab, ft, (double2*)d_wrp);
     } else {
         // no window (w = 0) or no points: the WRP and everything after it are zero
-        hipLaunchKernelGGL(k_wrp_zero2, dim3((unsigned)(((int64_t)L * M + 255) / 256)), dim3(256), 0, st, fft,
+        hipLaunchKernelGGL(k_wrp_zero2, dim3((unsigned)(((int64_t)L * M + 255) / 256)), dim3(256), 0, st, ft,
                            (int64_t)L * M);
         if (d_wrp)
             hipLaunchKernelGGL(k_wrp_zero2, dim3((unsigned)((npx + 255) / 256)), dim3(256), 0, st, (double2*)d_wrp,
@@ -361,34 +550,60 @@ int wrp_generate(ohx_ctx* c, const void* d_vertices, int64_t n, const ohx_wrp_pa
     }
     if ((st_ = check_hip(c, hipGetLastError(), "WRP accumulation"))) return st_;
 
-    // propagation: FFT, transfer function, inverse FFT, centre crop
-    if (!c->wrp_plan || c->wrp_plan_L != L || c->wrp_plan_M != M) {
-        if (c->wrp_plan) hipfftDestroy((hipfftHandle)c->wrp_plan);
-        c->wrp_plan = nullptr;
-        hipfftHandle h;
-        if (hipfftPlan2d(&h, L, M, HIPFFT_Z2Z) != HIPFFT_SUCCESS)
-            return set_error(c, OHX_ERR_DEVICE, "hipfftPlan2d failed");
-        c->wrp_plan = (void*)h;
+    // propagation: FFT, transfer function, inverse FFT, centre crop.  Each 2D
+    // transform is two passes of contiguous 1D transforms with a transpose
+    // between them, pruned to the rows that matter:
+    //   forward: along a over the rows T[b], b in S(M, cx) (the others are
+    //   zero); T -> A (zeros for the other b); along b over all rows of A;
+    //   transfer function on A;
+    //   inverse: along b over all rows of A; A -> T for b in S(M, cx) (the
+    //   only columns the crop reads); along a over those rows; crop from T.
+    if (!c->wrp_plan_t || c->wrp_plan_L != L || c->wrp_plan_M != M) {
+        wrp_release(c);
+        hipfftHandle ht, ha;
+        int nl[1] = {L}, nm[1] = {M};
+        if (hipfftPlanMany(&ht, 1, nl, nl, 1, L, nl, 1, L, HIPFFT_Z2Z, cx) != HIPFFT_SUCCESS)
+            return set_error(c, OHX_ERR_DEVICE, "hipfftPlanMany (length 2 pn_y) failed");
+        c->wrp_plan_t = (void*)ht;
+        if (hipfftPlanMany(&ha, 1, nm, nm, 1, M, nm, 1, M, HIPFFT_Z2Z, L) != HIPFFT_SUCCESS)
+            return set_error(c, OHX_ERR_DEVICE, "hipfftPlanMany (length 2 pn_x) failed");
+        c->wrp_plan_a = (void*)ha;
         c->wrp_plan_L = L;
         c->wrp_plan_M = M;
     }
-    hipfftHandle h = (hipfftHandle)c->wrp_plan;
-    if (hipfftSetStream(h, st) != HIPFFT_SUCCESS) return set_error(c, OHX_ERR_DEVICE, "hipfftSetStream failed");
-    if (hipfftExecZ2Z(h, (hipfftDoubleComplex*)fft, (hipfftDoubleComplex*)fft, HIPFFT_FORWARD) != HIPFFT_SUCCESS)
+    const hipfftHandle ht = (hipfftHandle)c->wrp_plan_t, ha = (hipfftHandle)c->wrp_plan_a;
+    if (hipfftSetStream(ht, st) != HIPFFT_SUCCESS || hipfftSetStream(ha, st) != HIPFFT_SUCCESS)
+        return set_error(c, OHX_ERR_DEVICE, "hipfftSetStream failed");
+    hipfftDoubleComplex* const t0 = (hipfftDoubleComplex*)ft;                    // rows T[0 .. cx)
+    hipfftDoubleComplex* const t1 = t0 + (int64_t)(M - cx) * L;                  // rows T[M - cx .. M)
+    hipfftDoubleComplex* const a0 = (hipfftDoubleComplex*)fa;
+    auto tpass = [&](int dir) {
+        return hipfftExecZ2Z(ht, t1, t1, dir) == HIPFFT_SUCCESS && hipfftExecZ2Z(ht, t0, t0, dir) == HIPFFT_SUCCESS;
+    };
+    const dim3 t2a_grid((unsigned)((L + TT - 1) / TT), (unsigned)((M + TT - 1) / TT));
+    const dim3 a2t_grid((unsigned)((L + TT - 1) / TT), (unsigned)((2 * cx + TT - 1) / TT));
+    if (!tpass(HIPFFT_FORWARD)) return set_error(c, OHX_ERR_DEVICE, "forward FFT failed");
+    hipLaunchKernelGGL(k_wrp_t2a, t2a_grid, dim3(256), 0, st, (const double2*)ft, fa, L, M, cx);
+    if (hipfftExecZ2Z(ha, a0, a0, HIPFFT_FORWARD) != HIPFFT_SUCCESS)
         return set_error(c, OHX_ERR_DEVICE, "forward FFT failed");
-    hipLaunchKernelGGL(k_wrp_asm, pgrid, dim3(256), 0, st, fft, pn_x, pn_y, p->pp_x, p->pp_y, p->lambda,
+    hipLaunchKernelGGL(k_wrp_asm, pgrid, dim3(256), 0, st, fa, pn_x, pn_y, p->pp_x, p->pp_y, p->lambda,
                        p->propagation_distance);
-    if (hipfftExecZ2Z(h, (hipfftDoubleComplex*)fft, (hipfftDoubleComplex*)fft, HIPFFT_BACKWARD) != HIPFFT_SUCCESS)
+    if (hipfftExecZ2Z(ha, a0, a0, HIPFFT_BACKWARD) != HIPFFT_SUCCESS)
         return set_error(c, OHX_ERR_DEVICE, "inverse FFT failed");
-    hipLaunchKernelGGL(k_wrp_crop, dim3(gx, gy), dim3(256), 0, st, (const double2*)fft, pn_x, pn_y,
-                       (double2*)d_field);
+    hipLaunchKernelGGL(k_wrp_a2t, a2t_grid, dim3(256), 0, st, (const double2*)fa, ft, L, M, cx);
+    if (!tpass(HIPFFT_BACKWARD)) return set_error(c, OHX_ERR_DEVICE, "inverse FFT failed");
+    hipLaunchKernelGGL(k_wrp_crop, dim3((unsigned)((pn_x + TT - 1) / TT), (unsigned)((pn_y + TT - 1) / TT)),
+                       dim3(256), 0, st, (const double2*)ft, pn_x, pn_y, (double2*)d_field);
     return check_hip(c, hipGetLastError(), "WRP propagation");
 }
 
 void wrp_release(ohx_ctx* c)
 {
-    if (c->wrp_plan) hipfftDestroy((hipfftHandle)c->wrp_plan);
-    c->wrp_plan = nullptr;
+    if (c->wrp_plan_t) hipfftDestroy((hipfftHandle)c->wrp_plan_t);
+    if (c->wrp_plan_a) hipfftDestroy((hipfftHandle)c->wrp_plan_a);
+    c->wrp_plan_t = nullptr;
+    c->wrp_plan_a = nullptr;
+    c->wrp_plan_L = c->wrp_plan_M = 0;
 }
 
 }  // namespace ohx

@@ -38,6 +38,7 @@ def cloud(seed, n, xy, z):
     ((256, 192), (8e-6, 8e-6), 532e-9, 0, (1.0, 1.0, 1.0), 1.0e-3, 0.05),
     ((200, 150), (6.4e-6, 8e-6), 633e-9, 1, (2.0, 1.5, 1.0), -0.8e-3, 0.02),   # dz < 0, odd sizes
     ((128, 96), (10e-6, 10e-6), 473e-9, 2, (1.0, 1.0, 3.0), 2.5e-3, -0.03),    # back-propagation
+    ((160, 120), (8e-6, 8e-6), 532e-9, 0, (1.0, 1.0, 1.0), 4.0e-3, 0.03),      # w = 33: table in global memory
 ])
 def test_wrp_matches_oracle(ctx, case):
     pn, pp, lam, ch, scale, dz, dist = case
@@ -64,6 +65,19 @@ def test_wrp_is_deterministic_and_clips_far_points(ctx):
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
     want = wo.wrp_plane(v, pn, pp, (1.0, 1.0, 1.0), lam, 0, zmax + 1.2e-3)
     assert rel(a[0], want) < 1e-12
+
+
+def test_wrp_dense_cluster(ctx):
+    """3000 points within +-15 pixels: every tile near the cluster keeps more
+    points than one LDS pass holds (1024), so the gather runs several passes."""
+    pn, pp, lam = (96, 80), (8e-6, 8e-6), 532e-9
+    v = cloud(51, 3000, 0.12e-3, (0.0, 1e-3))
+    zmax = float(v["pos"][:, 2].max())
+    loc = zmax + 1.0e-3
+    wrp, field = capi.generate_wrp(ctx, v, pn, pp, lam, loc, 0.02)
+    want_wrp, want_field = wo.generate(v, pn, pp, (1.0, 1.0, 1.0), lam, 0, loc, 0.02)
+    assert rel(wrp, want_wrp) < 1e-12
+    assert rel(field, want_field) < 1e-10
 
 
 def test_wrp_zero_window_gives_zero_field(ctx):

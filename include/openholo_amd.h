@@ -154,7 +154,8 @@ typedef struct ohx_wrp_params {
 /* d_field (pn_x*pn_y interleaved complex FP64) is overwritten with the
  * hologram of this channel (complex_H[ch]); d_wrp, if not null, receives the
  * WRP plane itself (ophWRP::getWRPBuff).  d_vertices are the 56-byte Vertex
- * records on the device. */
+ * records on the device.  OHX_ERR_INVALID_ARG for n_points > 2^31 - 1,
+ * pn_x * pn_y > 2^28, a window wider than 8190 pixels, random_phase != 0. */
 int ohx_wrp_generate(ohx_ctx* ctx, const void* d_vertices, int64_t n_points,
                      const ohx_wrp_params* params, double* d_field, double* d_wrp);
 

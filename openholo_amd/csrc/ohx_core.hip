@@ -332,6 +332,7 @@ int ohx_wrp_generate(ohx_ctx* c, const void* d_vertices, int64_t n, const ohx_wr
         return set_error(c, OHX_ERR_INVALID_ARG, "null buffer or negative point count");
     if (p->pn_x < 1 || p->pn_y < 1 || (int64_t)p->pn_x * p->pn_y > (int64_t)1 << 28)
         return set_error(c, OHX_ERR_INVALID_ARG, "pixel number out of range");
+    if (n > INT32_MAX) return set_error(c, OHX_ERR_INVALID_ARG, "WRP point count above 2^31 - 1");
     if (!(p->pp_x > 0) || !(p->pp_y > 0) || !(p->lambda > 0) || !std::isfinite(p->wrp_location) ||
         !std::isfinite(p->propagation_distance))
         return set_error(c, OHX_ERR_INVALID_ARG, "pitch, wave length or distances not positive/finite");

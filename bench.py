@@ -413,6 +413,24 @@ def main():
                   "frac": round(enc_bytes / (enc_ms / 1e3) / 1e9 / 8000.0, 4),
                   "what": "k_encode<PHASE> per channel + k_minmax + k_to_u8 over all channels, HIP events"}
 
+    # single-side-band encoding (ophPointCloud::encodeHologram) of the same
+    # fields: 16 B read + 8 B written per pixel and channel
+    ssb_ev = []
+    for rep in range(4):
+        a, b = Event(), Event()
+        a.record(stream)
+        for ch in range(nch):
+            ctx.encode_ssb(fields[ch], w.pn, w.pp, encs[ch])
+        b.record(stream)
+        if rep:
+            ssb_ev.append((a, b))
+    ctx.sync()
+    ssb_ms = float(np.mean([a.elapsed_ms(b) for a, b in ssb_ev]))
+    ssb_bytes = 24 * npx * nch
+    encode_hbm["ssb"] = {"ms_per_hologram": round(ssb_ms, 4), "bytes": ssb_bytes,
+                         "GB_per_s": round(ssb_bytes / (ssb_ms / 1e3) / 1e9, 1),
+                         "what": "k_encode_ssb per channel (encodeHologram, default spectrum shift), HIP events"}
+
     # p50 / p99 per hologram (one mode, all channels, encode + normalise)
     hol = sorted(a.elapsed_ms(b) for a, b in holo_ev)
     p50 = float(np.percentile(hol, 50)) if hol else None

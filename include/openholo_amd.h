@@ -165,6 +165,19 @@ int ohx_wrp_generate(ohx_ctx* ctx, const void* d_vertices, int64_t n_points,
 int ohx_encode(ohx_ctx* ctx, const double* d_field, int64_t n_px, double* d_enc,
                uint32_t encode_flag);
 
+/* ophPointCloud::encodeHologram (ophgen/src/ophPointCloud.cpp:193-240), the
+ * single-side-band encoding behind ophPointCloud::encoding(ENCODE_SSB, passband)
+ * (:247-251), one channel.  The reference computes
+ *   T = shift(IFFT(shift(shift(FFT(shift(H))))))      (FFTW, unnormalised)
+ * with Openholo::fft2 / fftShift (Openholo/src/Openholo.cpp:559-585,698-772),
+ * then d_enc[i] = Re(T[i]) * cos(X + Y), X = pi x shift_x / pp_x,
+ * Y = pi y shift_y / pp_y at the pixel centre (x, y) of pixel i; band_limit is
+ * unused there.  Here T is evaluated in closed form, per axis of size n with
+ * h = n/2 (integer): T[k] = n H[k + 2h mod n], times exp(2 pi j (k + h) / n)
+ * when n is odd.  d_field, d_enc: pn_x * pn_y interleaved complex / real FP64. */
+int ohx_encode_ssb(ohx_ctx* ctx, const double* d_field, int32_t pn_x, int32_t pn_y, double pp_x, double pp_y,
+                   double shift_x, double shift_y, double* d_enc);
+
 /* ophGen::normalize (ophgen/src/ophGen.cpp:654-687): global min/max over all
  * n_channels encoded buffers, then u8 = (uint8)(((v-min)/gap)*255 + 0.5).
  * d_enc and d_out are host arrays of n_channels device pointers. */

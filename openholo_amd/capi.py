@@ -27,7 +27,7 @@ EXPORTS = [
     "ohx_ctx_create", "ohx_ctx_destroy", "ohx_ctx_stream", "ohx_ctx_device", "ohx_status_string",
     "ohx_last_error", "ohx_mem_info", "ohx_malloc", "ohx_free", "ohx_memcpy_h2d", "ohx_memcpy_d2h",
     "ohx_memcpy_d2d", "ohx_synchronize", "ohx_field_zero", "ohx_pc_diffract", "ohx_field_accumulate",
-    "ohx_pc_probe_f64", "ohx_wrp_generate", "ohx_encode", "ohx_normalize_u8", "ohx_last_launch_stats",
+    "ohx_pc_probe_f64", "ohx_wrp_generate", "ohx_encode", "ohx_encode_ssb", "ohx_normalize_u8", "ohx_last_launch_stats",
     "ohx_server_create", "ohx_server_submit", "ohx_server_wait", "ohx_server_cancel",
     "ohx_server_counters_get", "ohx_server_destroy",
 ]
@@ -152,6 +152,8 @@ def lib():
             "ohx_pc_probe_f64": ([vp, vp, i64, ctypes.POINTER(PcParams), u32, vp, i64, vp], i32),
             "ohx_wrp_generate": ([vp, vp, i64, ctypes.POINTER(WrpParams), vp, vp], i32),
             "ohx_encode": ([vp, vp, i64, vp, u32], i32),
+            "ohx_encode_ssb": ([vp, vp, i32, i32, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                ctypes.c_double, vp], i32),
             "ohx_normalize_u8": ([vp, ctypes.POINTER(vp), i32, i64, ctypes.POINTER(vp)], i32),
             "ohx_last_launch_stats": ([vp, ctypes.POINTER(LaunchStats)], i32),
             "ohx_server_create": ([ctypes.POINTER(ServerOpts), ctypes.POINTER(vp)], i32),
@@ -266,6 +268,13 @@ class Context:
 
     def encode(self, field: DeviceBuffer, n_px: int, enc: DeviceBuffer, flag: int):
         self._check(lib().ohx_encode(self.h, field.ptr, int(n_px), enc.ptr, int(flag)), "ohx_encode")
+
+    def encode_ssb(self, field: DeviceBuffer, pn, pp, enc: DeviceBuffer, spectrum_shift=(0.0, 0.5)):
+        """ophPointCloud::encodeHologram of one channel (band_limit is unused by
+        the reference and not taken)."""
+        self._check(lib().ohx_encode_ssb(self.h, field.ptr, int(pn[0]), int(pn[1]), float(pp[0]), float(pp[1]),
+                                         float(spectrum_shift[0]), float(spectrum_shift[1]), enc.ptr),
+                    "ohx_encode_ssb")
 
     def normalize_u8(self, encs, n_px: int, outs):
         a = (ctypes.c_void_p * len(encs))(*[e.ptr for e in encs])

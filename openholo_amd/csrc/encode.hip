@@ -12,6 +12,7 @@
 // atomics on an order-preserving integer image of the doubles; NaN is
 // skipped as the reference's < / > comparisons skip it.
 #include <algorithm>
+#include <cmath>
 #include <cfloat>
 #include <cstdint>
 
@@ -48,6 +49,66 @@ __global__ __launch_bounds__(EB) void k_encode(const double2* __restrict__ f, in
         else if (FLAG == OHX_ENCODE_REAL) v = h.x;
         else v = h.y;
         e[i] = v;
+    }
+}
+
+// Single-side-band encoding (ophPointCloud::encodeHologram,
+// ophgen/src/ophPointCloud.cpp:193-240).  The reference's FFT round trip
+// T = s(IFFT(s(s(FFT(s(H)))))), s the centring shift out[k] = in[k + n/2]
+// (Openholo.cpp:753-772), reduces per axis to T[k] = n H[k + 2h mod n] times
+// exp(2 pi j (k + h) / n) for odd n (h = n/2; for even n, 2h = n and the
+// factor is 1).  The carrier follows the reference's operation order
+// (:225-233): y = (ss_y - pp_y) - pp_y h, Y = (pi y s_y) / pp_y,
+// x = -ss_x/2 + pp_x w + pp_x/2, X = (pi x s_x) / pp_x,
+// enc = Re(T) (cos X cos Y - sin X sin Y).
+// X and the x factor depend on the column only, Y and the y factor on the
+// row only: k_ssb_tables evaluates them once per column and row
+// ({cos, sin, factor re, factor im}), k_encode_ssb combines them per pixel.
+__global__ __launch_bounds__(EB) void k_ssb_tables(int pn_x, int pn_y, double pp_x, double pp_y, double sx,
+                                                  double sy, double4* __restrict__ tx, double4* __restrict__ ty)
+{
+#pragma clang fp contract(off)
+    const double pi = 3.14159265358979323846;
+    const int k = blockIdx.x * EB + threadIdx.x;
+    if (k < pn_x) {
+        const double ss_x = pn_x * pp_x;
+        const double x = -(ss_x / 2) + pp_x * k + pp_x / 2;
+        const double X = (pi * x * sx) / pp_x;
+        double fs = 0.0, fc = 1.0;
+        if (pn_x & 1) sincospi(2.0 * (k + (pn_x >> 1)) / pn_x, &fs, &fc);
+        tx[k] = make_double4(cos(X), sin(X), fc, fs);
+    } else if (k < pn_x + pn_y) {
+        const int h = k - pn_x;
+        const double ss_y = pn_y * pp_y;
+        const double y = (ss_y - pp_y) - (pp_y * h);
+        const double Y = (pi * y * sy) / pp_y;
+        double fs = 0.0, fc = 1.0;
+        if (pn_y & 1) sincospi(2.0 * (h + (pn_y >> 1)) / pn_y, &fs, &fc);
+        ty[h] = make_double4(cos(Y), sin(Y), fc, fs);
+    }
+}
+
+__global__ __launch_bounds__(EB) void k_encode_ssb(const double2* __restrict__ f, int pn_x, int pn_y,
+                                                  const double4* __restrict__ tx, const double4* __restrict__ ty,
+                                                  double* __restrict__ e)
+{
+#pragma clang fp contract(off)
+    const int64_t n = (int64_t)pn_x * pn_y;
+    const int hx = pn_x >> 1, hy = pn_y >> 1;
+    const double nn = (double)pn_x * (double)pn_y;
+    int64_t i = (int64_t)blockIdx.x * EB + threadIdx.x;
+    const int64_t stride = (int64_t)gridDim.x * EB;
+    for (; i < n; i += stride) {
+        const int w = (int)(i % pn_x), h = (int)(i / pn_x);
+        int sw = w + 2 * hx, sh = h + 2 * hy;
+        if (sw >= pn_x) sw -= pn_x;
+        if (sh >= pn_y) sh -= pn_y;
+        double2 v = f[(int64_t)sh * pn_x + sw];
+        const double4 a = tx[w], b = ty[h];
+        if (pn_x & 1) v = make_double2(v.x * a.z - v.y * a.w, v.x * a.w + v.y * a.z);
+        if (pn_y & 1) v = make_double2(v.x * b.z - v.y * b.w, v.x * b.w + v.y * b.z);
+        const double carrier = 1.0 * (a.x * b.x - a.y * b.y);
+        e[i] = (nn * v.x) * carrier;
     }
 }
 
@@ -196,6 +257,32 @@ extern "C" int ohx_encode(ohx_ctx* c, const double* d_field, int64_t n, double* 
     default: hipLaunchKernelGGL(k_encode<OHX_ENCODE_IMAGINARY>, g, b, 0, c->stream, f, n, d_enc); break;
     }
     return check_hip(c, hipGetLastError(), "k_encode");
+}
+
+extern "C" int ohx_encode_ssb(ohx_ctx* c, const double* d_field, int32_t pn_x, int32_t pn_y, double pp_x,
+                              double pp_y, double shift_x, double shift_y, double* d_enc)
+{
+    if (!c) return OHX_ERR_INVALID_ARG;
+    if (pn_x < 1 || pn_y < 1 || pn_x > (1 << 30) || pn_y > (1 << 30) || (int64_t)pn_x * pn_y > ((int64_t)1 << 31) ||
+        !d_field || !d_enc)
+        return set_error(c, OHX_ERR_INVALID_ARG, "null buffer or pixel number out of range");
+    if (!(pp_x > 0) || !(pp_y > 0) || !std::isfinite(pp_x) || !std::isfinite(pp_y) || !std::isfinite(shift_x) ||
+        !std::isfinite(shift_y))
+        return set_error(c, OHX_ERR_INVALID_ARG, "pixel pitch not positive or shift not finite");
+    std::lock_guard<std::mutex> lk(c->mu);
+    (void)hipSetDevice(c->device);
+    const int64_t n = (int64_t)pn_x * pn_y;
+    // per-column and per-row tables in the context's scratch
+    const size_t tx_bytes = ((size_t)pn_x * sizeof(double4) + 255) / 256 * 256;
+    int st = ensure_gemm_scratch(c, tx_bytes + (size_t)pn_y * sizeof(double4));
+    if (st) return st;
+    double4* tx = (double4*)c->gemm_scratch;
+    double4* ty = (double4*)((char*)c->gemm_scratch + tx_bytes);
+    hipLaunchKernelGGL(k_ssb_tables, dim3((unsigned)((pn_x + pn_y + EB - 1) / EB)), dim3(EB), 0, c->stream, pn_x,
+                       pn_y, pp_x, pp_y, shift_x, shift_y, tx, ty);
+    hipLaunchKernelGGL(k_encode_ssb, dim3(egrid(n)), dim3(EB), 0, c->stream, (const double2*)d_field, pn_x, pn_y,
+                       (const double4*)tx, (const double4*)ty, d_enc);
+    return check_hip(c, hipGetLastError(), "k_encode_ssb");
 }
 
 extern "C" int ohx_normalize_u8(ohx_ctx* c, const double* const* d_enc, int nch, int64_t n,

@@ -17,6 +17,8 @@
 //   --dump-points F   raw 56-byte Vertex records as loaded
 //   --dump-config     print the parsed configuration as one JSON line
 //   --dump-wrp F      (WRP) raw complex128 WRP plane of the last channel
+//   --passband P      (point cloud) call encoding(encode, P), the two-argument
+//                     overload (-e 7: single-side-band encodeHologram)
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -34,6 +36,7 @@ static const char* encode_name(unsigned e)
     case 1: return "Amplitude";
     case 2: return "Real";
     case 3: return "Imaginary";
+    case 7: return "SSB";
     default: return "Unknown";
     }
 }
@@ -96,7 +99,8 @@ static int run_wrp(const std::string& config, const std::string& input, unsigned
 int main(int argc, char** argv)
 {
     std::string config, input, vtx, outdir = "Result", dump_field, dump_enc, dump_u8, ohc, dump_points, dump_wrp;
-    bool no_generate = false, dump_config = false;
+    bool no_generate = false, dump_config = false, two_arg_encoding = false;
+    unsigned passband = 0;
     int alg = 0, flag = 0;
     unsigned encode = 0, mode = 0;
     for (int i = 1; i < argc; ++i) {
@@ -128,6 +132,7 @@ int main(int argc, char** argv)
         else if (a == "--no-generate") no_generate = true;
         else if (a == "--dump-config") dump_config = true;
         else if (a == "--dump-wrp") dump_wrp = v, ++i;
+        else if (a == "--passband") passband = (unsigned)std::atoi(v), two_arg_encoding = true, ++i;
         else {
             std::fprintf(stderr, "unknown argument %s\n", a.c_str());
             return 2;
@@ -193,7 +198,12 @@ int main(int argc, char** argv)
         for (uint c = 0; c < nch; ++c) std::fwrite(h->getComplexField()[c], 16, npx, fp);
         std::fclose(fp);
     }
-    h->encoding(encode);
+    if (two_arg_encoding) h->encoding(encode, passband);
+    else h->encoding(encode);
+    if (!h->lastError().empty()) {
+        std::fprintf(stderr, "encoding failed: %s\n", h->lastError().c_str());
+        return 1;
+    }
     h->normalize();
     if (!dump_enc.empty()) {
         FILE* fp = std::fopen(dump_enc.c_str(), "wb");

@@ -49,6 +49,12 @@ public:
     bool readConfig(const char* cfg_file);
     Real generateHologram(uint diff_flag = PC_DIFF_RS);
     virtual void encoding(unsigned int ENCODE_FLAG);
+    // ENCODE_SSB -> encodeHologram(); ENCODE_OFFSSB (ophGen's side-band
+    // encoder) is not provided and fails (ophPointCloud.cpp:247-251)
+    virtual void encoding(unsigned int ENCODE_FLAG, unsigned int SSB_PASSBAND);
+    // single-side-band encoding of every channel on the GPU
+    // (ophgen/src/ophPointCloud.cpp:193-240); band_limit is unused, as there
+    void encodeHologram(vec2 band_limit = vec2(0.8, 0.5), vec2 spectrum_shift = vec2(0.0, 0.5));
     void setViewingWindow(bool is_ViewingWindow) { this->is_ViewingWindow = is_ViewingWindow; }
     uint* getProgress() { return &m_nProgress; }
 

@@ -158,6 +158,41 @@ bool ophPointCloud::genCghPointCloudGPU(uint diff_flag)
 
 void ophPointCloud::encoding(unsigned int ENCODE_FLAG) { ophGen::encoding(ENCODE_FLAG); }
 
+void ophPointCloud::encoding(unsigned int ENCODE_FLAG, unsigned int SSB_PASSBAND)
+{
+    // ophgen/src/ophPointCloud.cpp:247-251
+    (void)SSB_PASSBAND;
+    if (ENCODE_FLAG == ENCODE_SSB) encodeHologram();
+    else if (ENCODE_FLAG == ENCODE_OFFSSB) OPH_LOG("<FAILED> ENCODE_OFFSSB is not provided.\n");
+}
+
+void ophPointCloud::encodeHologram(vec2 band_limit, vec2 spectrum_shift)
+{
+    // ophgen/src/ophPointCloud.cpp:193-240 (the FFT round trip in closed form
+    // on the GPU; complex_H is left unchanged)
+    (void)band_limit;
+    if (complex_H == nullptr) {
+        OPH_LOG("<FAILED> Not found diffracted data.");
+        return;
+    }
+    const int pnX = context_.pixel_number[_X], pnY = context_.pixel_number[_Y];
+    m_vecEncodeSize = ivec2(pnX, pnY);
+    context_.ss[_X] = pnX * context_.pixel_pitch[_X];
+    context_.ss[_Y] = pnY * context_.pixel_pitch[_Y];
+    if (!uploadFields()) return;
+    ohx_ctx* c = device();
+    for (uint ch = 0; ch < context_.waveNum; ++ch) {
+        int st = ohx_encode_ssb(c, d_field[ch], pnX, pnY, context_.pixel_pitch[_X], context_.pixel_pitch[_Y],
+                                spectrum_shift[_X], spectrum_shift[_Y], d_enc[ch]);
+        if (!st) st = ohx_memcpy_d2h(c, m_lpEncoded[ch], d_enc[ch], 8 * (size_t)d_pixels);
+        if (st) {
+            fail("encodeHologram", st);
+            return;
+        }
+    }
+    d_enc_valid = true;
+}
+
 void ophPointCloud::ophFree(void)
 {
     ophGen::ophFree();

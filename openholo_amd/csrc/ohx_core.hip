@@ -37,6 +37,10 @@ int geometry_from_params(ohx_ctx* ctx, const ohx_pc_params* p, uint32_t diff, Ge
         return set_error(ctx, OHX_ERR_INVALID_ARG, "pixel_number out of range");
     if (!(p->pp_x > 0) || !(p->pp_y > 0) || !(p->lambda > 0))
         return set_error(ctx, OHX_ERR_INVALID_ARG, "pixel_pitch and lambda must be > 0");
+    if (!std::isfinite(p->pp_x) || !std::isfinite(p->pp_y) || !std::isfinite(p->lambda) ||
+        !std::isfinite(p->distance) || !std::isfinite(p->scale[0]) || !std::isfinite(p->scale[1]) ||
+        !std::isfinite(p->scale[2]))
+        return set_error(ctx, OHX_ERR_INVALID_ARG, "pitch, lambda, distance and scale must be finite");
     if (p->channel < 0 || p->channel > 2)
         return set_error(ctx, OHX_ERR_INVALID_ARG, "channel must index Vertex::color (0..2)");
     if (diff == OHX_DIFF_RS && !(p->lambda < 2 * p->pp_x && p->lambda < 2 * p->pp_y))
@@ -333,9 +337,11 @@ int ohx_wrp_generate(ohx_ctx* c, const void* d_vertices, int64_t n, const ohx_wr
     if (p->pn_x < 1 || p->pn_y < 1 || (int64_t)p->pn_x * p->pn_y > (int64_t)1 << 28)
         return set_error(c, OHX_ERR_INVALID_ARG, "pixel number out of range");
     if (n > INT32_MAX) return set_error(c, OHX_ERR_INVALID_ARG, "WRP point count above 2^31 - 1");
-    if (!(p->pp_x > 0) || !(p->pp_y > 0) || !(p->lambda > 0) || !std::isfinite(p->wrp_location) ||
-        !std::isfinite(p->propagation_distance))
-        return set_error(c, OHX_ERR_INVALID_ARG, "pitch, wave length or distances not positive/finite");
+    if (!(p->pp_x > 0) || !(p->pp_y > 0) || !(p->lambda > 0) || !std::isfinite(p->pp_x) ||
+        !std::isfinite(p->pp_y) || !std::isfinite(p->lambda) || !std::isfinite(p->wrp_location) ||
+        !std::isfinite(p->propagation_distance) || !std::isfinite(p->scale[0]) || !std::isfinite(p->scale[1]) ||
+        !std::isfinite(p->scale[2]))
+        return set_error(c, OHX_ERR_INVALID_ARG, "pitch, wave length, scale or distances not positive/finite");
     if (p->channel < 0 || p->channel > 2) return set_error(c, OHX_ERR_INVALID_ARG, "channel must be 0..2");
     if (p->random_phase) return set_error(c, OHX_ERR_INVALID_ARG, "random phase is not supported");
     std::lock_guard<std::mutex> lk(c->mu);

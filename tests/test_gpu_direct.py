@@ -395,6 +395,12 @@ def test_edge_cases(ctx):
             ctx.diffract(dv, 1, fld, capi.params(pn, pp, 532e-9, channel=3), 0, FP32)
         with pytest.raises(capi.OhxError):
             ctx.diffract(dv, 1, fld, capi.params(pn, (1e-7, 1e-7), 532e-9), 0, FP32)
+        for bad in (capi.params(pn, (float("inf"), 8e-6), 532e-9), capi.params(pn, pp, float("nan")),
+                    capi.params(pn, pp, 532e-9, distance=float("inf")),
+                    capi.params(pn, pp, 532e-9, scale=(1.0, float("nan"), 1.0))):
+            with pytest.raises(capi.OhxError) as e:
+                ctx.diffract(dv, 1, fld, bad, 1, FP32)
+            assert e.value.status == -1
     finally:
         dv.free()
         fld.free()

@@ -378,7 +378,8 @@ def main():
                     "avg_launch_ms": round(k_ms, 3),
                     "model": f"{fl} FP32 FLOP/pair in the separable pair loop (FMA=2); the per-lane "
                              f"column factor (v_sin/v_cos once per point, shared by 8 rows) is not "
-                             f"counted; published peak counts packed FP32, which is not emitted",
+                             f"counted; packed FP32 is not emitted (a VGPR-operand FMA stream "
+                             f"reaches 104 TFLOP/s on this part)",
                     "mix_sol_pairs_per_s": MIX_SOL_PAIRS_PER_S[dom],
                     "frac_of_mix_sol": round(pps / MIX_SOL_PAIRS_PER_S[dom], 4)}
     elif args.kernel in MFMA_TERMS and dom == 1:

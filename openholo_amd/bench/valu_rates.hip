@@ -104,6 +104,25 @@ __global__ __launch_bounds__(256) void k64(float* out, int iters, double b, doub
   float s = 0; for (int i = 0; i < CH / 2; ++i) s += (float)a[i] + f[i];
   out[blockIdx.x * 256 + threadIdx.x] = s;
 }
+// FP32 issue rate by instruction form: all-VGPR operands (the form of the
+// direct kernels' pair loops) against the SGPR-operand FMA of k<0>.
+template <int KIND>
+__global__ __launch_bounds__(256) void kv(float* out, int iters, float bs) {
+  float a[CH], r[CH];
+  const float b = threadIdx.x * 1e-7f + bs;   // a VGPR operand
+  for (int i = 0; i < CH; ++i) { a[i] = threadIdx.x * 1e-3f + i * 1e-2f; r[i] = i * 1e-3f; }
+  for (int it = 0; it < iters; ++it) {
+#pragma unroll
+    for (int i = 0; i < CH; ++i) {
+      if (KIND == 0) r[i] = fmaf(a[i], b, r[i]);                 // v_fmac_f32 (VOP2)
+      else if (KIND == 1) r[i] = fmaf(-a[i], b, r[i]);           // v_fma_f32, negated operand (VOP3)
+      else if (KIND == 2) r[i] = r[i] * b;                       // v_mul_f32
+      else { r[i] = fmaf(a[i], b, r[i]); a[i] = fmaf(-r[i], b, a[i]); }   // both FMA forms
+    }
+  }
+  float s = 0; for (int i = 0; i < CH; ++i) s += a[i] + r[i];
+  out[blockIdx.x * 256 + threadIdx.x] = s;
+}
 __global__ void kcopy(const float4* __restrict__ a, float4* __restrict__ b, size_t n) {
   size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x, st = (size_t)gridDim.x * blockDim.x;
   for (; i < n; i += st) b[i] = a[i];
@@ -112,7 +131,7 @@ int main() {
   const int blocks = 256 * 8, iters = 4096;
   float* out; CK(hipMalloc(&out, blocks * 256 * sizeof(float)));
   hipEvent_t e0, e1; CK(hipEventCreate(&e0)); CK(hipEventCreate(&e1));
-  const char* names[] = {"fma", "sin", "add+fract", "sin+fma", "sin+cos+fma", "3fma+sin", "add+rsq",
+  const char* names[] = {"fma (SGPR operand)", "sin", "add+fract", "sin+fma", "sin+cos+fma", "3fma+sin", "add+rsq",
                          "fresnel-pair", "rs-pair"};
   const int instr[] = {1, 1, 2, 2, 3, 4, 2, 5, 8};
   for (int kind = 0; kind < 9; ++kind) {
@@ -136,6 +155,21 @@ int main() {
       double lanes_ops = (double)blocks * 256 * iters * CH;   // per "element op group"
       if (rep) printf("%-12s %8.3f ms  %8.2f T elem-iter/s  %8.2f T VALU-instr/s (lane)\n", names[kind], ms,
                       lanes_ops / ms / 1e9, lanes_ops * instr[kind] / ms / 1e9);
+    }
+  }
+  const char* nv[] = {"fmac VGPR (VOP2)", "fma neg VGPR (VOP3)", "mul VGPR", "fmac + fma neg"};
+  for (int kind = 0; kind < 4; ++kind) {
+    for (int rep = 0; rep < 2; ++rep) {
+      CK(hipEventRecord(e0));
+      if (kind == 0) hipLaunchKernelGGL(kv<0>, dim3(blocks), dim3(256), 0, 0, out, 4 * iters, 0.999f);
+      if (kind == 1) hipLaunchKernelGGL(kv<1>, dim3(blocks), dim3(256), 0, 0, out, 4 * iters, 0.999f);
+      if (kind == 2) hipLaunchKernelGGL(kv<2>, dim3(blocks), dim3(256), 0, 0, out, 4 * iters, 0.999f);
+      if (kind == 3) hipLaunchKernelGGL(kv<3>, dim3(blocks), dim3(256), 0, 0, out, 4 * iters, 0.999f);
+      CK(hipGetLastError());
+      CK(hipEventRecord(e1)); CK(hipEventSynchronize(e1));
+      float ms; CK(hipEventElapsedTime(&ms, e0, e1));
+      const double ops = (double)blocks * 256 * 4 * iters * CH * (kind == 3 ? 2 : 1);
+      if (rep) printf("%-28s %8.3f ms  %8.2f T VALU-instr/s (lane)\n", nv[kind], ms, ops / ms / 1e9);
     }
   }
   const char* nsep[] = {"fresnel-sep (8 rows)", "rs-sep (8 rows)"};

@@ -57,16 +57,20 @@ static int run_wrp(const std::string& config, const std::string& input, unsigned
                    const std::string& dump_enc, const std::string& dump_u8, const std::string& dump_wrp)
 {
     ophWRP* h = new ophWRP();
+    auto fail = [&]() {
+        h->release();
+        return 1;
+    };
     h->SetMode(mode);
-    if (!h->readConfig(config.c_str())) return 1;
-    if (h->loadPointCloud(input.c_str()) < 0) return 1;
+    if (!h->readConfig(config.c_str())) return fail();
+    if (h->loadPointCloud(input.c_str()) < 0) return fail();
     if (!dump_points.empty() &&
         !write_raw(dump_points, h->getPointCloudModel(), sizeof(Vertex) * h->getNumberOfPoints()))
-        return 1;
+        return fail();
     h->generateHologram();
     if (!h->lastError().empty()) {
         std::fprintf(stderr, "generateHologram failed: %s\n", h->lastError().c_str());
-        return 1;
+        return fail();
     }
     const uint nch = h->getContext().waveNum;
     const long long npx = (long long)h->getContext().pixel_number[_X] * h->getContext().pixel_number[_Y];
@@ -75,7 +79,7 @@ static int run_wrp(const std::string& config, const std::string& input, unsigned
         for (uint c = 0; c < nch; ++c) std::fwrite(h->getComplexField()[c], 16, npx, fp);
         std::fclose(fp);
     }
-    if (!dump_wrp.empty() && !write_raw(dump_wrp, h->getWRPBuff(), 16 * (size_t)npx)) return 1;
+    if (!dump_wrp.empty() && !write_raw(dump_wrp, h->getWRPBuff(), 16 * (size_t)npx)) return fail();
     h->encoding(encode);
     h->normalize();
     if (!dump_enc.empty()) {
@@ -145,21 +149,28 @@ int main(int argc, char** argv)
         return 2;
     }
     ophPointCloud* h = new ophPointCloud();
+    auto fail = [&]() {
+        h->release();
+        return 1;
+    };
     h->SetMode(mode);
-    if (!h->readConfig(config.c_str())) return 1;
+    if (!h->readConfig(config.c_str())) return fail();
     if (!vtx.empty()) {
         FILE* fp = std::fopen(vtx.c_str(), "rb");
-        if (!fp) return 1;
+        if (!fp) return fail();
         std::fseek(fp, 0, SEEK_END);
         const long bytes = std::ftell(fp);
         std::fseek(fp, 0, SEEK_SET);
         std::vector<Vertex> v(bytes / sizeof(Vertex));
-        if (std::fread(v.data(), sizeof(Vertex), v.size(), fp) != v.size()) return 1;
+        if (std::fread(v.data(), sizeof(Vertex), v.size(), fp) != v.size()) {
+            std::fclose(fp);
+            return fail();
+        }
         std::fclose(fp);
         h->setNumberOfPoints(v.size());
         h->setPointCloudModel(v.data());
     } else if (h->loadPointCloud(input.c_str()) < 0) {
-        return 1;
+        return fail();
     }
     if (dump_config) {
         OphConfig& c = h->getContext();
@@ -176,7 +187,7 @@ int main(int argc, char** argv)
     }
     if (!dump_points.empty()) {
         FILE* fp = std::fopen(dump_points.c_str(), "wb");
-        if (!fp) return 1;
+        if (!fp) return fail();
         std::fwrite(h->getPointCloudModel(), sizeof(Vertex), h->getNumberOfPoints(), fp);
         std::fclose(fp);
     }
@@ -187,12 +198,12 @@ int main(int argc, char** argv)
     const Real t = h->generateHologram((uint)flag);
     if (t <= 0.0) {
         std::fprintf(stderr, "generateHologram failed: %s\n", h->lastError().c_str());
-        return 1;
+        return fail();
     }
     std::printf("generateHologram: %.6f s\n", t);
     const uint nch = h->getContext().waveNum;
     const long long npx = (long long)h->getContext().pixel_number[_X] * h->getContext().pixel_number[_Y];
-    if (!ohc.empty() && !h->saveAsOhc(ohc.c_str())) return 1;
+    if (!ohc.empty() && !h->saveAsOhc(ohc.c_str())) return fail();
     if (!dump_field.empty()) {
         FILE* fp = std::fopen(dump_field.c_str(), "wb");
         for (uint c = 0; c < nch; ++c) std::fwrite(h->getComplexField()[c], 16, npx, fp);
@@ -202,7 +213,7 @@ int main(int argc, char** argv)
     else h->encoding(encode);
     if (!h->lastError().empty()) {
         std::fprintf(stderr, "encoding failed: %s\n", h->lastError().c_str());
-        return 1;
+        return fail();
     }
     h->normalize();
     if (!dump_enc.empty()) {

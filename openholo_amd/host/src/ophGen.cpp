@@ -4,6 +4,7 @@
 #include "ophGen.h"
 
 #include <algorithm>
+#include <cmath>
 #include <cctype>
 #include <cstring>
 #include <fstream>
@@ -52,6 +53,12 @@ void ophGen::initialize(void)
     // normalised buffer per channel, zero-filled
     const uint nch = context_.waveNum;
     const long long n = (long long)resCfg[_X] * resCfg[_Y];
+    if (nch < 1 || nch > 3 || resCfg[_X] < 1 || resCfg[_Y] < 1 || n > (1LL << 28)) {
+        // the buffers stay as they were (the upstream code would allocate garbage sizes)
+        OPH_LOG("<FAILED> initialize: need 1..3 wavelengths and 1..2^28 pixels\n");
+        m_lastError = "initialize: need 1..3 wavelengths and 1..2^28 pixels";
+        return;
+    }
     if (complex_H) {
         for (uint i = 0; i < m_nOldChannel; ++i) delete[] complex_H[i];
         delete[] complex_H;
@@ -216,6 +223,11 @@ bool ophGen::readConfig(const char* fname)
         OPH_LOG("<FAILED> Not found node : 'SLM_WaveNum' (Integer)\n");
         ok = false;
     }
+    // a channel indexes Vertex::color[3], so 1..3 wavelengths
+    if (nWave < 1 || nWave > 3) {
+        OPH_LOG("<FAILED> SLM_WaveNum must be 1..3 (got %d)\n", nWave);
+        return false;
+    }
     setWaveNum(nWave);
     for (int i = 1; i <= nWave; ++i) {
         char name[32];
@@ -229,6 +241,18 @@ bool ophGen::readConfig(const char* fname)
     if (!q_int(m, "SLM_PixelNumY", &context_.pixel_number[_Y])) { OPH_LOG("<FAILED> Not found node : 'SLM_PixelNumY' (Integer)\n"); ok = false; }
     if (!q_double(m, "SLM_PixelPitchX", &context_.pixel_pitch[_X])) { OPH_LOG("<FAILED> Not found node : 'SLM_PixelPitchX' (Double)\n"); ok = false; }
     if (!q_double(m, "SLM_PixelPitchY", &context_.pixel_pitch[_Y])) { OPH_LOG("<FAILED> Not found node : 'SLM_PixelPitchY' (Double)\n"); ok = false; }
+    // the buffers are allocated from these: 1..2^20 pixels per axis, at most 2^28 in all
+    const int pnx = context_.pixel_number[_X], pny = context_.pixel_number[_Y];
+    if (pnx < 1 || pny < 1 || pnx > (1 << 20) || pny > (1 << 20) || (long long)pnx * pny > (1LL << 28)) {
+        OPH_LOG("<FAILED> SLM_PixelNumX/Y out of range (%d x %d)\n", pnx, pny);
+        context_.pixel_number = ivec2(0, 0);
+        return false;
+    }
+    if (!(context_.pixel_pitch[_X] > 0) || !(context_.pixel_pitch[_Y] > 0) ||
+        !std::isfinite(context_.pixel_pitch[_X]) || !std::isfinite(context_.pixel_pitch[_Y])) {
+        OPH_LOG("<FAILED> SLM_PixelPitchX/Y must be positive and finite\n");
+        return false;
+    }
     if (!q_bool(m, "IMG_Rotation", &imgCfg.rotate)) imgCfg.rotate = false;
     if (!q_bool(m, "IMG_Merge", &imgCfg.merge)) imgCfg.merge = false;
     if (!q_int(m, "IMG_Flip", &imgCfg.flip)) imgCfg.flip = 0;
@@ -337,6 +361,31 @@ int ophGen::loadPointCloud(const char* pc_file, OphPointCloudData* pc)
             break;
         }
     }
+    // counts the rest of the file cannot hold are rejected before allocating
+    // (a record takes at least its fixed-size fields in binary, or one
+    // character and a separator per property in ASCII)
+    {
+        const std::streampos body = f.tellg();
+        f.seekg(0, std::ios::end);
+        const long long rest = (long long)(f.tellg() - body);
+        f.seekg(body);
+        long long need = 0;
+        for (const PlyElem& e : elems) {   // the elements up to and including 'vertex'
+            if (e.count < 0) {
+                OPH_LOG("<FAILED> negative element count in ply header.\n");
+                return -1;
+            }
+            long long rec = 0;
+            for (const PlyProp& p : e.props) rec += binary ? type_size(p.list ? p.list_count_type : p.type) : 2;
+            rec = std::max(rec, 1LL);
+            if (e.count > (rest - need) / rec + 1) {
+                OPH_LOG("<FAILED> ply element count exceeds the file.\n");
+                return -1;
+            }
+            need += e.count * rec;
+            if (e.name == "vertex") break;
+        }
+    }
     int ev = -1;
     bool has_color_elem = false;
     for (size_t i = 0; i < elems.size(); ++i) {
@@ -351,6 +400,11 @@ int ophGen::loadPointCloud(const char* pc_file, OphPointCloudData* pc)
     };
     const int ix = pidx("x"), iy = pidx("y"), iz = pidx("z");
     if (ix < 0 || iy < 0 || iz < 0) return -1;
+    for (int j : {ix, iy, iz})
+        if (binary && type_size(elems[ev].props[j].type) == 0) {
+            OPH_LOG("<FAILED> unknown ply coordinate type '%s'.\n", elems[ev].props[j].type.c_str());
+            return -1;
+        }
     int ir = pidx("red"), ig = pidx("green"), ib = pidx("blue");
     if (ir < 0 || ig < 0 || ib < 0) {
         ir = pidx("diffuse_red");
@@ -370,11 +424,12 @@ int ophGen::loadPointCloud(const char* pc_file, OphPointCloudData* pc)
         for (long long k = 0; k < e.count; ++k) {
             vals.assign(e.props.size(), 0.0);
             if (binary) {
+                if (!f) break;   // truncated file: the remaining records stay zero
                 for (size_t j = 0; j < e.props.size(); ++j) {
                     const PlyProp& p = e.props[j];
                     if (p.list) {
-                        const long long cnt = (long long)read_bin(f, p.list_count_type, big);
-                        for (long long q = 0; q < cnt; ++q) read_bin(f, p.type, big);
+                        const double cnt = read_bin(f, p.list_count_type, big);
+                        for (long long q = 0; f && q < cnt; ++q) read_bin(f, p.type, big);
                     } else {
                         vals[j] = read_bin(f, p.type, big);
                     }
@@ -396,12 +451,16 @@ int ophGen::loadPointCloud(const char* pc_file, OphPointCloudData* pc)
             vx.point.pos[_Y] = vals[iy];
             vx.point.pos[_Z] = vals[iz];
             if (ir >= 0) {
-                vx.color.color[0] = (Real)((float)(int)vals[ir] / 255.f);
-                vx.color.color[1] = (Real)((float)(int)vals[ig] / 255.f);
-                vx.color.color[2] = (Real)((float)(int)vals[ib] / 255.f);
+                // integer colour channels (values outside int's range, possible only
+                // in a corrupt file, read as 0)
+                auto chan = [](double c) { return (c > -2147483649.0 && c < 2147483648.0) ? (int)c : 0; };
+                vx.color.color[0] = (Real)((float)chan(vals[ir]) / 255.f);
+                vx.color.color[1] = (Real)((float)chan(vals[ig]) / 255.f);
+                vx.color.color[2] = (Real)((float)chan(vals[ib]) / 255.f);
             }
             if (iph >= 0) vx.phase = (Real)(float)vals[iph];
         }
+        if ((int)ei == ev) break;   // nothing after the vertex element is used
     }
     if (!has_color_elem)
         for (long long k = 0; k < n; ++k)

@@ -148,3 +148,37 @@ def test_missing_ply_fails(tmp_path):
     r = run_cli("-c", os.path.join(GOLD, "Dice_PointCloud.xml"), "-i", str(tmp_path / "none.ply"),
                 "--no-generate", check=False)
     assert r.returncode == 1
+
+
+@pytest.mark.parametrize("count", ["-5", "4000000000"])
+def test_ply_impossible_vertex_count_rejected(tmp_path, count):
+    """A header count the file cannot hold (or a negative one) fails the load
+    instead of allocating for it."""
+    p = tmp_path / "bad.ply"
+    p.write_text("ply\nformat ascii 1.0\nelement vertex %s\nproperty float x\nproperty float y\n"
+                 "property float z\nend_header\n0 0 0\n1 1 1\n" % count)
+    r = run_cli("-c", os.path.join(GOLD, "Dice_PointCloud.xml"), "-i", str(p), "--no-generate", check=False)
+    assert r.returncode == 1
+
+
+def test_truncated_binary_ply_rejected(tmp_path):
+    head = ("ply\nformat binary_little_endian 1.0\nelement vertex 50\nproperty float x\nproperty float y\n"
+            "property float z\nend_header\n").encode()
+    p = tmp_path / "short.ply"
+    p.write_bytes(head + struct.pack("<30f", *range(30)))     # 10 of the 50 records
+    r = run_cli("-c", os.path.join(GOLD, "Dice_PointCloud.xml"), "-i", str(p), "--no-generate", check=False)
+    assert r.returncode == 1
+
+
+@pytest.mark.parametrize("tag,value", [("SLM_PixelNumX", "-5"), ("SLM_PixelNumX", "2000000"),
+                                       ("SLM_WaveNum", "0"), ("SLM_WaveNum", "4"), ("SLM_PixelPitchX", "nan")])
+def test_config_out_of_range_rejected(tmp_path, tag, value):
+    """Pixel numbers, wavelength count and pitch are checked before any buffer
+    is sized from them (a channel indexes Vertex::color[3])."""
+    text = open(os.path.join(GOLD, "Dice_PointCloud.xml")).read()
+    start = text.index("<%s>" % tag) + len(tag) + 2
+    end = text.index("</%s>" % tag)
+    cfg = tmp_path / "bad.xml"
+    cfg.write_text(text[:start] + value + text[end:])
+    r = run_cli("-c", str(cfg), "-i", os.path.join(GOLD, "pointcloud_1470.ply"), "--no-generate", check=False)
+    assert r.returncode == 1

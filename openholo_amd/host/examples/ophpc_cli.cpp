@@ -19,6 +19,8 @@
 //   --dump-wrp F      (WRP) raw complex128 WRP plane of the last channel
 //   --passband P      (point cloud) call encoding(encode, P), the two-argument
 //                     overload (-e 7: single-side-band encodeHologram)
+//   --load-ohc F      only load the OHC file F (Openholo::loadAsOhc), print its
+//                     configuration as one JSON line, write --dump-field
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -102,7 +104,8 @@ static int run_wrp(const std::string& config, const std::string& input, unsigned
 
 int main(int argc, char** argv)
 {
-    std::string config, input, vtx, outdir = "Result", dump_field, dump_enc, dump_u8, ohc, dump_points, dump_wrp;
+    std::string config, input, vtx, outdir = "Result", dump_field, dump_enc, dump_u8, ohc, dump_points, dump_wrp,
+        load_ohc;
     bool no_generate = false, dump_config = false, two_arg_encoding = false;
     unsigned passband = 0;
     int alg = 0, flag = 0;
@@ -136,11 +139,31 @@ int main(int argc, char** argv)
         else if (a == "--no-generate") no_generate = true;
         else if (a == "--dump-config") dump_config = true;
         else if (a == "--dump-wrp") dump_wrp = v, ++i;
+        else if (a == "--load-ohc") load_ohc = v, ++i;
         else if (a == "--passband") passband = (unsigned)std::atoi(v), two_arg_encoding = true, ++i;
         else {
             std::fprintf(stderr, "unknown argument %s\n", a.c_str());
             return 2;
         }
+    }
+    if (!load_ohc.empty()) {
+        ophPointCloud* h = new ophPointCloud();
+        const bool ok = h->loadAsOhc(load_ohc.c_str());
+        if (ok) {
+            OphConfig& c = h->getContext();
+            std::printf("{\"pn\": [%d, %d], \"pp\": [%.17g, %.17g], \"lambdas\": [", c.pixel_number[_X],
+                        c.pixel_number[_Y], c.pixel_pitch[_X], c.pixel_pitch[_Y]);
+            for (uint w = 0; w < c.waveNum; ++w) std::printf("%s%.17g", w ? ", " : "", c.wave_length[w]);
+            std::printf("]}\n");
+            if (!dump_field.empty()) {
+                const long long npx = (long long)c.pixel_number[_X] * c.pixel_number[_Y];
+                FILE* fp = std::fopen(dump_field.c_str(), "wb");
+                for (uint w = 0; fp && w < c.waveNum; ++w) std::fwrite(h->getComplexField()[w], 16, npx, fp);
+                if (fp) std::fclose(fp);
+            }
+        }
+        h->release();
+        return ok ? 0 : 1;
     }
     if (alg == 4)
         return run_wrp(config, input, mode, encode, outdir, dump_points, dump_field, dump_enc, dump_u8, dump_wrp);

@@ -240,8 +240,22 @@ bool Openholo::loadAsOhc(const char* fname)
         fi.code_type != CODE_RI || (fi.data_type != DT_F64 && fi.data_type != DT_F32))
         return false;
     const uint32_t nw = fi.wavlen_num;
+    const size_t es = fi.data_type == DT_F64 ? 8 : 4;
+    // header sizes are checked against the file before anything is sized from them
+    const std::streampos here = f.tellg();
+    f.seekg(0, std::ios::end);
+    const uint64_t file_bytes = (uint64_t)f.tellg();
+    f.seekg(here);
+    const uint64_t npx = (uint64_t)fi.px_num_x * fi.px_num_y;
+    if (nw > 64 || fi.px_num_x < 1 || fi.px_num_y < 1 || fi.px_num_x > (1u << 20) || fi.px_num_y > (1u << 20) ||
+        npx > (1ull << 28) || (uint64_t)fh.data_offset > file_bytes ||
+        2 * npx * nw * es > file_bytes - (uint64_t)fh.data_offset) {
+        OPH_LOG("<FAILED> OHC header sizes do not match the file.\n");
+        return false;
+    }
     std::vector<double> wl(nw);
     f.read((char*)wl.data(), 8 * nw);
+    if (!f) return false;
     f.seekg(fh.data_offset);
     if (complex_H) {
         for (uint i = 0; i < context_.waveNum; ++i) delete[] complex_H[i];
@@ -253,11 +267,19 @@ bool Openholo::loadAsOhc(const char* fname)
     context_.pixel_pitch = vec2(fi.px_pitch_x, fi.px_pitch_y);
     setPixelNumber(ivec2((int)fi.px_num_x, (int)fi.px_num_y));
     context_.k = (2 * M_PI) / context_.wave_length[0];
-    const uint64_t npx = (uint64_t)fi.px_num_x * fi.px_num_y;
     complex_H = new Complex<Real>*[nw];
     for (uint32_t w = 0; w < nw; ++w) complex_H[w] = new Complex<Real>[npx];
-    const size_t es = fi.data_type == DT_F64 ? 8 : 4;
     std::vector<char> buf(npx * es);
+    auto value = [es](const char* p) {
+        if (es == 8) {
+            double d;
+            std::memcpy(&d, p, 8);
+            return d;
+        }
+        float x;
+        std::memcpy(&x, p, 4);
+        return (double)x;
+    };
     for (int part = 0; part < 2; ++part)
         for (uint32_t w = 0; w < nw; ++w) {
             if (fi.color_arrange == ARR_SEQT) {
@@ -266,12 +288,11 @@ bool Openholo::loadAsOhc(const char* fname)
                     f.seekg(fh.data_offset + (std::streamoff)(part * npx * nw + i * nw + w) * es);
                     char v[8];
                     f.read(v, es);
-                    complex_H[w][i][part] = es == 8 ? *(double*)v : *(float*)v;
+                    complex_H[w][i][part] = value(v);
                 }
             } else {
                 f.read(buf.data(), buf.size());
-                for (uint64_t i = 0; i < npx; ++i)
-                    complex_H[w][i][part] = es == 8 ? ((double*)buf.data())[i] : ((float*)buf.data())[i];
+                for (uint64_t i = 0; i < npx; ++i) complex_H[w][i][part] = value(buf.data() + i * es);
             }
         }
     return (bool)f;

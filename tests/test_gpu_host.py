@@ -88,6 +88,9 @@ def test_example_program_ohc_and_bmp(tmp_path):
     assert struct.unpack("<d", ohc[89:97])[0] == 532e-9
     planes = np.frombuffer(ohc[data_off:], np.float64).reshape(2, npx)
     assert np.array_equal(planes[0], field.real) and np.array_equal(planes[1], field.imag)
+    # and back through Openholo::loadAsOhc
+    run(os.path.join(BIN, "ophpc_cli"), "--load-ohc", str(d / "out.ohc"), "--dump-field", str(d / "back.bin"))
+    assert np.array_equal(np.fromfile(d / "back.bin", np.complex128), field)
     # 8-bit palette BMP of the single channel
     bmp = open(d / "out.bmp", "rb").read()
     off = struct.unpack("<i", bmp[10:14])[0]

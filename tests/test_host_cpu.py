@@ -182,3 +182,59 @@ def test_config_out_of_range_rejected(tmp_path, tag, value):
     cfg.write_text(text[:start] + value + text[end:])
     r = run_cli("-c", str(cfg), "-i", os.path.join(GOLD, "pointcloud_1470.ply"), "--no-generate", check=False)
     assert r.returncode == 1
+
+
+def _ohc_bytes(fields, pp, lambdas, arrange=2, dtype=10, nw_header=None, px_header=None, offset_delta=0,
+               cut=0):
+    """An OHC file (Openholo/src/ImgCodecDefine.h:137-206): 24 B file header,
+    65 B field-info header, wavelength table, then the real planes and the
+    imaginary planes; arrange 2 = one plane per wavelength, 1 = wavelengths
+    interleaved per pixel; dtype 10 = FP64, 9 = FP32."""
+    nw, ny, nx = fields.shape
+    es = 8 if dtype == 10 else 4
+    header_size = 65 + 8 * nw
+    field_size = nx * ny * nw * 2 * es
+    off = 24 + header_size + offset_delta
+    fh = struct.pack("<2sQBBIII", b"OH", 24 + header_size + field_size, 3, 0, 0, 0, off)
+    pnx, pny = px_header or (nx, ny)
+    fi = struct.pack("<IIIddBIBBBBBBBddQB", header_size, pnx, pny, pp[0], pp[1], 1,
+                     nw if nw_header is None else nw_header, 2, arrange, 1, dtype, 1, 2, 0, -1.0, 1.0,
+                     field_size, 0)
+    wl = struct.pack("<%dd" % nw, *lambdas)
+    ft = "<f8" if dtype == 10 else "<f4"
+    planes = []
+    for part in (fields.real, fields.imag):
+        if arrange == 2:
+            planes += [part[w].astype(ft).tobytes() for w in range(nw)]
+        else:
+            planes.append(np.stack([part[w].ravel() for w in range(nw)], axis=1).astype(ft).tobytes())
+    data = fh + fi + wl + b"".join(planes)
+    return data[:len(data) - cut] if cut else data
+
+
+@pytest.mark.parametrize("arrange,dtype", [(2, 10), (1, 9)])
+def test_load_ohc(tmp_path, arrange, dtype):
+    """Openholo::loadAsOhc: per-wavelength planes (FP64) and per-pixel
+    interleaved wavelengths (FP32); pitch, wavelengths and pixel numbers from
+    the headers."""
+    rng = np.random.default_rng(arrange)
+    fields = rng.normal(size=(2, 5, 7)) + 1j * rng.normal(size=(2, 5, 7))
+    p = tmp_path / "f.ohc"
+    p.write_bytes(_ohc_bytes(fields, (8e-6, 6e-6), (638e-9, 532e-9), arrange, dtype))
+    r = run_cli("--load-ohc", str(p), "--dump-field", str(tmp_path / "f.bin"))
+    cfg = json.loads(r.stdout.strip().splitlines()[-1])
+    assert cfg["pn"] == [7, 5] and cfg["pp"] == [8e-6, 6e-6] and cfg["lambdas"] == [638e-9, 532e-9]
+    got = np.fromfile(tmp_path / "f.bin", np.complex128).reshape(2, 5, 7)
+    want = fields if dtype == 10 else (fields.real.astype(np.float32) + 1j * fields.imag.astype(np.float32))
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("bad", [dict(nw_header=1_000_000_000), dict(px_header=(1 << 21, 5)),
+                                 dict(px_header=(70000, 70000)), dict(offset_delta=10_000), dict(cut=16)])
+def test_load_ohc_rejects_inconsistent_headers(tmp_path, bad):
+    """Header sizes are checked against the file before any buffer is sized."""
+    fields = np.ones((1, 5, 7), np.complex128)
+    p = tmp_path / "bad.ohc"
+    p.write_bytes(_ohc_bytes(fields, (8e-6, 8e-6), (532e-9,), **bad))
+    r = run_cli("--load-ohc", str(p), check=False)
+    assert r.returncode == 1

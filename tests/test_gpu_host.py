@@ -144,3 +144,50 @@ def test_cli_wrp_end_to_end(tmp_path, orc):
     assert (size, w, h, planes, bpp) == (len(bmp), 1920, 1080, 1, 24)
     pixels = np.frombuffer(bmp[off:], np.uint8).reshape(1080, 1920, 3)
     assert np.array_equal(pixels, np.stack([u[2], u[1], u[0]], axis=-1).reshape(1080, 1920, 3))
+
+
+def _bmp_rows(path, w, h, ch):
+    bmp = open(path, "rb").read()
+    assert bmp[:2] == b"BM"
+    off = struct.unpack("<i", bmp[10:14])[0]
+    bw, bh, _, bpp = struct.unpack("<iiHH", bmp[18:30])
+    assert (bw, bh, bpp) == (w, h, 8 * ch)
+    stride = (w * ch + 3) & ~3
+    raw = np.frombuffer(bmp[off:], np.uint8)
+    assert raw.size == stride * h
+    rows = raw.reshape(h, stride)
+    assert not rows[:, w * ch:].any()                 # row padding is zero
+    return rows[:, :w * ch].reshape(h, w, ch)
+
+
+@pytest.mark.parametrize("merge,flip,rotate", [(0, 1, 0), (1, 2, 1)])
+def test_cli_odd_width_bmp(tmp_path, merge, flip, rotate):
+    """1001 x 601 hologram (rows padded to 4 bytes): per-channel 8-bit files
+    (IMG_Merge 0, '<i>_' prefixes) or one merged 24-bit file, flipped
+    (IMG_Flip 1 vertical, 2 horizontal) and rotated by 180 degrees."""
+    d = tmp_path
+    text = open(CFG_RGB).read()
+    for tag, val in [("SLM_PixelNumX", "1001"), ("SLM_PixelNumY", "601"), ("IMG_Merge", str(merge)),
+                     ("IMG_Flip", str(flip)), ("IMG_Rotation", str(rotate))]:
+        a = text.index("<%s>" % tag) + len(tag) + 2
+        text = text[:a] + val + text[text.index("</%s>" % tag):]
+    cfg = d / "odd.xml"
+    cfg.write_text(text)
+    run(os.path.join(BIN, "ophpc_cli"), "-c", str(cfg), "-i", PLY, "-m", "0", "-p", "1", "-f", "1",
+        "-o", str(d / "R"), "--dump-u8", str(d / "u.bin"))
+    w, h = 1001, 601
+    u = np.fromfile(d / "u.bin", np.uint8).reshape(3, h, w)
+    mode = flip ^ (3 if rotate else 0)
+    def tf(img):
+        if mode & 1:
+            img = img[::-1]
+        if mode & 2:
+            img = img[:, ::-1]
+        return img
+    if merge:
+        got = _bmp_rows(d / "R" / "PointCloud_Fresnel_Phase.bmp", w, h, 3)
+        assert np.array_equal(got, tf(np.stack([u[2], u[1], u[0]], axis=-1)))
+    else:
+        for ch in range(3):
+            got = _bmp_rows(d / "R" / f"{ch}_PointCloud_Fresnel_Phase.bmp", w, h, 1)
+            assert np.array_equal(got[..., 0], tf(u[ch]))

@@ -28,12 +28,11 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-# FP32 FLOPs per (point, pixel) pair executed by the direct kernels' fast
-# path (FMA = 2, every other VALU op incl. v_sin/v_cos = 1):
-#   RS:      add, 2 fma (phase series), sin, cos, fma (amplitude), 2 fma (accumulate) = 13
-#   Fresnel: add, sin, cos, 2 fma (accumulate)                                      = 7
-# FP32 separable pair loops (FMA = 2): RS mul + 6 FMA, Fresnel 4 FMA
-FLOP_PER_PAIR = {0: 13, 1: 8}
+# FP32 FLOPs per (point, pixel) pair in the direct kernels' separable pair
+# loops (FMA = 2): RS 6 FMA (the cross term folded into the column factor),
+# Fresnel 4 FMA.  The per-lane column factor (a few FMAs and one v_sin/v_cos
+# per point, shared by 8 rows) is not counted.
+FLOP_PER_PAIR = {0: 12, 1: 8}
 PEAK_FP32_TFLOPS = 157.3          # published MI355X FP32 vector peak
 # FP64 mode, pair loop of the separable form (FMA = 2): RS series correction
 # + complex products, Fresnel one complex multiply-add
@@ -129,7 +128,7 @@ TRAFFIC = {0: 0.985e9, 1: 0.880e9}
 # Speed of light of the direct kernels' separable pair loops (per lane and point,
 # 8 rows: column factor + 8 pairs) on this
 # hardware (openholo_amd/bin/valu_rates); profiles/valu_mfma_rates_gfx950.md.
-MIX_SOL_PAIRS_PER_S = {0: 6.67e12, 1: 11.43e12}
+MIX_SOL_PAIRS_PER_S = {0: 8.27e12, 1: 11.43e12}
 
 
 def wrp_block(ctx, capi, synth, Event, launches=3):

@@ -55,18 +55,28 @@ __global__ __launch_bounds__(256) void ksep(float* out, int iters, float b, floa
     if (KIND == 0) {
       const float f = fmaf(fmaf(x, b, c), x, c);
       sa = __builtin_amdgcn_sinf(f); ca = __builtin_amdgcn_cosf(f);
-    } else {
+    } else {   // RS
       const float fa = fmaf(fmaf(c, x, b), x, c);
       const float am = fmaf(c, x, b);
       kx = b * x;
       sa = am * __builtin_amdgcn_sinf(fa); ca = am * __builtin_amdgcn_cosf(fa);
     }
+    if (KIND == 2) {   // RS with the cross term folded into the column factor: 6 FMA per pair
+      const float cak = ca * kx, sak = sa * kx;
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      float r = br[k], i = bi[k];
-      if (KIND == 1) { const float e = kx * ny[k]; r = fmaf(-e, bi[k], br[k]); i = fmaf(e, br[k], bi[k]); }
-      pr[k] = fmaf(sa, r, fmaf(ca, i, pr[k]));
-      pim[k] = fmaf(ca, r, fmaf(-sa, i, pim[k]));
+      for (int k = 0; k < 8; ++k) {
+        const float P = fmaf(cak, ny[k], sa), Q = fmaf(-sak, ny[k], ca);
+        pr[k] = fmaf(P, br[k], fmaf(Q, bi[k], pr[k]));
+        pim[k] = fmaf(Q, br[k], fmaf(-P, bi[k], pim[k]));
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        float r = br[k], i = bi[k];
+        if (KIND == 1) { const float e = kx * ny[k]; r = fmaf(-e, bi[k], br[k]); i = fmaf(e, br[k], bi[k]); }
+        pr[k] = fmaf(sa, r, fmaf(ca, i, pr[k]));
+        pim[k] = fmaf(ca, r, fmaf(-sa, i, pim[k]));
+      }
     }
     x = fmaf(x, b, c);
   }
@@ -172,12 +182,13 @@ int main() {
       if (rep) printf("%-28s %8.3f ms  %8.2f T VALU-instr/s (lane)\n", nv[kind], ms, ops / ms / 1e9);
     }
   }
-  const char* nsep[] = {"fresnel-sep (8 rows)", "rs-sep (8 rows)"};
-  for (int kind = 0; kind < 2; ++kind) {
+  const char* nsep[] = {"fresnel-sep (8 rows)", "rs-sep, mul + 6 fma (8 rows)", "rs-sep, 6 fma (8 rows)"};
+  for (int kind = 0; kind < 3; ++kind) {
     for (int rep = 0; rep < 2; ++rep) {
       CK(hipEventRecord(e0));
       if (kind == 0) hipLaunchKernelGGL(ksep<0>, dim3(blocks), dim3(256), 0, 0, out, iters, 0.999f, 1e-4f);
-      else hipLaunchKernelGGL(ksep<1>, dim3(blocks), dim3(256), 0, 0, out, iters, 0.999f, 1e-4f);
+      else if (kind == 1) hipLaunchKernelGGL(ksep<1>, dim3(blocks), dim3(256), 0, 0, out, iters, 0.999f, 1e-4f);
+      else hipLaunchKernelGGL(ksep<2>, dim3(blocks), dim3(256), 0, 0, out, iters, 0.999f, 1e-4f);
       CK(hipGetLastError());
       CK(hipEventRecord(e1)); CK(hipEventSynchronize(e1));
       float ms; CK(hipEventElapsedTime(&ms, e0, e1));

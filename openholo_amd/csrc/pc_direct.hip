@@ -34,7 +34,8 @@
 //    reciprocal in the loop.  The phase splits into a column part, a row
 //    part and the cross term -2 c_2 n_x N_y cycles (c_2 = 1/(8 r_c^3 lambda)),
 //    at most MAX_EPS rad, taken as a first-order factor 1 + j eps (the
-//    amplitude's cross term is below 1e-7): 7 FMAs per pair, no v_sin or
+//    amplitude's cross term is below 1e-7), folded into the column factor
+//    per row (P = sa + ca eps, Q = ca - sa eps): 6 FMAs per pair, no v_sin or
 //    v_cos.  Pairs with a larger cross term (FASTNS) evaluate the phase per
 //    pixel: add, two FMAs, v_sin, v_cos, three FMAs.
 //  * FP32 partial sums per chunk are folded into FP64 accumulators; the
@@ -466,7 +467,7 @@ __device__ __attribute__((always_inline)) inline void direct_tile(DirectSmem<DIF
         // A(col) = az (1 + mu n_x) e^{j 2pi (c0 + c_w n_x + c_z n_x^2)} and the
         // tabulated B(row); the cross term 2 c_z n_x N_y enters as
         // B (1 + j eps), eps = k_x N_y (|eps| <= MAX_EPS), and the amplitude's
-        // mu^2 n_x N_y is below 1e-7: 7 FMAs per pair, no v_sin/v_cos.
+        // mu^2 n_x N_y is below 1e-7: 6 FMAs per pair, no v_sin/v_cos.
         float pr[RPT], pim[RPT];
 #pragma unroll
         for (int k = 0; k < RPT; ++k) pr[k] = pim[k] = 0.f;
@@ -493,12 +494,15 @@ __device__ __attribute__((always_inline)) inline void direct_tile(DirectSmem<DIF
                 const float am = fmaf(d.x, nx, d.y);
                 const float kx = d.z * nx;
                 const float sa = am * sin_rev(fa), ca = am * cos_rev(fa);
+                // (sa, ca) against B (1 + j eps), eps = k_x N_y, regrouped per row:
+                // P = sa + ca eps, Q = ca - sa eps, then pr += P Bx + Q By and
+                // pim += Q Bx - P By (6 FMAs per pair)
+                const float cak = ca * kx, sak = sa * kx;
 #pragma unroll
                 for (int k = 0; k < RPT; ++k) {
-                    const float e = kx * Nv[k];
-                    const float br = fmaf(-e, Ry[k], Rx[k]), bi = fmaf(e, Rx[k], Ry[k]);
-                    pr[k] = fmaf(sa, br, fmaf(ca, bi, pr[k]));
-                    pim[k] = fmaf(ca, br, fmaf(-sa, bi, pim[k]));
+                    const float P = fmaf(cak, Nv[k], sa), Q = fmaf(-sak, Nv[k], ca);
+                    pr[k] = fmaf(P, Rx[k], fmaf(Q, Ry[k], pr[k]));
+                    pim[k] = fmaf(Q, Rx[k], fmaf(-P, Ry[k], pim[k]));
                 }
             }
         }

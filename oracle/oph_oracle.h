@@ -3,7 +3,7 @@
  *
  * TEST INFRASTRUCTURE ONLY.  Nothing in the product library links or calls
  * this code; only the test suite, the smoke check and the CPU-baseline leg of
- * the benchmark load it, and only as the checker / the timed CPU baseline.
+ * the benchmark load it, and only to check results / as the timed CPU baseline.
  *
  * Two independent restatements live here:
  *

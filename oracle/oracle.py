@@ -1,7 +1,7 @@
 """ctypes front end of the CPU oracle (oracle/oph_oracle.c).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, by ``__graft_entry__.smoke()``
-as the checker, and by ``bench.py``'s CPU-baseline leg as the timed CPU
+to check the result, and by ``bench.py``'s CPU-baseline leg as the timed CPU
 reference restatement.  The product package never imports this module.
 """
 from __future__ import annotations

@@ -1,7 +1,7 @@
 """CPU oracle of the single-side-band encoder ophPointCloud::encodeHologram,
 in numpy FP64.
 
-TEST INFRASTRUCTURE ONLY: imported by tests/ as the checker.  The product
+TEST INFRASTRUCTURE ONLY: imported by tests/ as the reference check.  The product
 package never imports this module.
 
 Restates ophgen/src/ophPointCloud.cpp:193-240 per channel:

@@ -1,6 +1,6 @@
 """CPU oracle of the WRP (wavefront recording plane) generator, in numpy FP64.
 
-TEST INFRASTRUCTURE ONLY: imported by tests/ as the checker.  The product
+TEST INFRASTRUCTURE ONLY: imported by tests/ as the reference check.  The product
 package never imports this module.
 
 Restates the reference's CPU path:

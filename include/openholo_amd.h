@@ -98,6 +98,13 @@ int ohx_free(ohx_ctx* ctx, void* d_ptr);
 int ohx_memcpy_h2d(ohx_ctx* ctx, void* d_dst, const void* src, size_t bytes);
 int ohx_memcpy_d2h(ohx_ctx* ctx, void* dst, const void* d_src, size_t bytes);
 int ohx_memcpy_d2d(ohx_ctx* ctx, void* d_dst, const void* d_src, size_t bytes);
+/* Device-to-host copy of d_src once all work enqueued so far on ctx's stream
+ * has finished, on a second stream: work enqueued on ctx's stream later (the
+ * next channel) runs while the copy is in flight.  Returns when the copy is
+ * complete; that later work must not write d_src.  Replaces the per-channel
+ * cudaMemcpy of the field in ophgen/src/ophPointCloud_GPU.cpp:272, so that
+ * generateHologram's read-back of one channel overlaps the next channel. */
+int ohx_memcpy_d2h_after(ohx_ctx* ctx, void* dst, const void* d_src, size_t bytes);
 int ohx_synchronize(ohx_ctx* ctx);
 
 /* Zero n_px complex FP64 pixels with a fill kernel (resetBuffer,

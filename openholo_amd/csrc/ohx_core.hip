@@ -191,7 +191,11 @@ int ohx_ctx_create(int device, ohx_ctx** out)
     c->device = device;
     if (hipSetDevice(device) != hipSuccess ||
         hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->copy_ready, hipEventDisableTiming) != hipSuccess ||
         hipMalloc(&c->minmax, 8 * sizeof(unsigned long long)) != hipSuccess) {
+        if (c->copy_ready) (void)hipEventDestroy(c->copy_ready);
+        if (c->copy_stream) (void)hipStreamDestroy(c->copy_stream);
         if (c->stream) (void)hipStreamDestroy(c->stream);
         delete c;
         return OHX_ERR_DEVICE;
@@ -209,6 +213,9 @@ int ohx_ctx_destroy(ohx_ctx* c)
     if (c->minmax) (void)hipFree(c->minmax);
     if (c->gemm_scratch) (void)hipFree(c->gemm_scratch);
     wrp_release(c);
+    (void)hipStreamSynchronize(c->copy_stream);
+    (void)hipEventDestroy(c->copy_ready);
+    (void)hipStreamDestroy(c->copy_stream);
     (void)hipStreamDestroy(c->stream);
     delete c;
     return OHX_OK;
@@ -257,6 +264,22 @@ static int copy(ohx_ctx* c, void* d, const void* s, size_t n, hipMemcpyKind k)
 int ohx_memcpy_h2d(ohx_ctx* c, void* d, const void* s, size_t n) { return copy(c, d, s, n, hipMemcpyHostToDevice); }
 int ohx_memcpy_d2h(ohx_ctx* c, void* d, const void* s, size_t n) { return copy(c, d, s, n, hipMemcpyDeviceToHost); }
 int ohx_memcpy_d2d(ohx_ctx* c, void* d, const void* s, size_t n) { return copy(c, d, s, n, hipMemcpyDeviceToDevice); }
+
+int ohx_memcpy_d2h_after(ohx_ctx* c, void* d, const void* s, size_t n)
+{
+    if (!c || (n && (!d || !s))) return OHX_ERR_INVALID_ARG;
+    if (!n) return OHX_OK;
+    int st;
+    {
+        std::lock_guard<std::mutex> lk(c->mu);   // one copy_ready event per context
+        (void)hipSetDevice(c->device);
+        st = check_hip(c, hipEventRecord(c->copy_ready, c->stream), "hipEventRecord");
+        if (!st) st = check_hip(c, hipStreamWaitEvent(c->copy_stream, c->copy_ready, 0), "hipStreamWaitEvent");
+        if (!st) st = check_hip(c, hipMemcpyAsync(d, s, n, hipMemcpyDeviceToHost, c->copy_stream), "hipMemcpyAsync");
+    }
+    if (!st) st = check_hip(c, hipStreamSynchronize(c->copy_stream), "hipStreamSynchronize");
+    return st;
+}
 
 int ohx_synchronize(ohx_ctx* c)
 {

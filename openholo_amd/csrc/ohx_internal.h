@@ -12,6 +12,9 @@
 struct ohx_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    // field read-back that overlaps later work on `stream` (ohx_memcpy_d2h_after)
+    hipStream_t copy_stream = nullptr;
+    hipEvent_t copy_ready = nullptr;
     std::mutex mu;                 // serialises use of the scratch buffers
     std::string last_error;
     // per-point SoA scratch (grown on demand)

@@ -127,7 +127,8 @@ bool ophPointCloud::genCghPointCloudGPU(uint diff_flag)
         d_vertices_n = n;
         device_version = is_ViewingWindow ? 0 : model_version;
     }
-    for (uint ch = 0; ch < nch; ++ch) {
+    // channel ch's field is read back while channel ch + 1 computes
+    auto launch = [&](uint ch) {
         const Real lambda = context_.wave_length[ch];
         context_.k = (2 * M_PI) / lambda;
         ohx_pc_params p{};
@@ -144,12 +145,18 @@ bool ophPointCloud::genCghPointCloudGPU(uint diff_flag)
         p.use_phase = 0;
         int st = ohx_field_zero(c, d_field[ch], d_pixels);
         if (!st) st = ohx_pc_diffract(c, d_vertices, (int64_t)n, d_field[ch], &p, diff_flag, m_mode);
-        if (!st) st = ohx_memcpy_d2h(c, complex_H[ch], d_field[ch], 16 * (size_t)d_pixels);
-        if (st) {
-            fail("diffraction", st);
-            return false;
-        }
+        return st;
+    };
+    int st = launch(0);
+    for (uint ch = 0; !st && ch < nch; ++ch) {
+        if (ch + 1 < nch && (st = launch(ch + 1))) break;
+        st = ohx_memcpy_d2h_after(c, complex_H[ch], d_field[ch], 16 * (size_t)d_pixels);
         m_nProgress = (ch + 1) * 100 / nch;
+    }
+    if (st) {
+        (void)ohx_synchronize(c);
+        fail("diffraction", st);
+        return false;
     }
     d_field_valid = true;
     d_enc_valid = false;

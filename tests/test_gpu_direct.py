@@ -404,3 +404,29 @@ def test_edge_cases(ctx):
     finally:
         dv.free()
         fld.free()
+
+
+def test_d2h_after_overlaps_next_launch(ctx):
+    """ohx_memcpy_d2h_after returns field 0 as computed before the call, while
+    the launch enqueued after field 0 (into field 1) runs on: both fields match
+    plain synchronous read-backs."""
+    pn, pp = (512, 256), (8e-6, 8e-6)
+    npx = pn[0] * pn[1]
+    v = synth.random_cloud(77, 3000, 1.0e-3, (0.2, 0.3))
+    dv = ctx.to_device(v)
+    f0, f1 = ctx.alloc(16 * npx), ctx.alloc(16 * npx)
+    try:
+        p0, p1 = capi.params(pn, pp, 532e-9), capi.params(pn, pp, 638e-9)
+        ctx.field_zero(f0, npx)
+        ctx.diffract(dv, len(v), f0, p0, 0, FP32)
+        ctx.field_zero(f1, npx)
+        ctx.diffract(dv, len(v), f1, p1, 1, FP32)          # enqueued before the read-back
+        early = f0.download_after(np.empty(npx, np.complex128))
+        ctx.sync()
+        assert np.array_equal(early, f0.download(np.complex128, npx))
+        want1 = capi.generate(ctx, v, pn, pp, [638e-9], 1, FP32)[0].ravel()
+        assert np.array_equal(f1.download(np.complex128, npx), want1)
+    finally:
+        dv.free()
+        f0.free()
+        f1.free()

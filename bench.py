@@ -457,9 +457,11 @@ def main():
         in_box[("RS" if diff == 0 else "Fresnel")] = round(boxed / nominal, 6) if nominal else None
 
     # PCIe-inclusive rate of the drop-in: ophPointCloud::generateHologram takes the
-    # vertices from the host and hands every channel's FP64 field back in pageable
-    # host buffers; one step timed end to end on the wall clock (never the value).
-    host_fields = [np.empty(npx, np.complex128) for _ in range(nch)]
+    # vertices from the host and hands every channel's FP64 field back in its
+    # complex_H planes (page-locked host memory); one step timed end to end on the
+    # wall clock (never the value).
+    host_bufs = [capi.HostBuffer(16 * npx) for _ in range(nch)]
+    host_fields = [b.array(np.complex128, npx) for b in host_bufs]
     lib = capi.lib()
     ctx.sync()
     t0 = time.perf_counter()
@@ -480,10 +482,13 @@ def main():
     t0 = time.perf_counter()
     assert lib.ohx_memcpy_d2h(ctx.h, host_fields[0].ctypes.data, fields[0].ptr, 16 * npx) == capi.OK
     d2h_s = time.perf_counter() - t0
+    del host_fields
+    for b in host_bufs:
+        b.free()
     handover = {"ms_per_step": round(host_ms, 2), "Gpairs_per_s": round(pairs_step / (host_ms / 1e3) / 1e9, 1),
                 "field_d2h_GB_per_s": round(16 * npx / d2h_s / 1e9, 2),
-                "what": "vertex upload + per channel and mode: zero, diffract, FP64 field to a pageable "
-                        "host buffer overlapped with the next channel (the drop-in's generateHologram); "
+                "what": "vertex upload + per channel and mode: zero, diffract, FP64 field to a page-locked "
+                        "host plane overlapped with the next channel (the drop-in's generateHologram); "
                         "wall clock"}
     print(f"# host hand-over: {host_ms:.1f} ms/step, field D2H {handover['field_d2h_GB_per_s']} GB/s",
           file=sys.stderr, flush=True)

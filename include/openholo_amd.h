@@ -95,6 +95,13 @@ int ohx_mem_info(ohx_ctx* ctx, size_t* free_bytes, size_t* total_bytes);
 
 int ohx_malloc(ohx_ctx* ctx, size_t bytes, void** d_ptr);
 int ohx_free(ohx_ctx* ctx, void* d_ptr);
+/* Page-locked host memory, zero-filled (hipHostMalloc): read-back into it
+ * runs at full link rate and overlaps kernels.  The host library allocates
+ * the complex field complex_H this way where the reference uses new[]
+ * (ophgen/src/ophGen.cpp:153-168); OHX_ERR_DEVICE when no device is
+ * present (the caller then falls back to pageable memory). */
+int ohx_host_alloc(size_t bytes, void** ptr);
+int ohx_host_free(void* ptr);
 int ohx_memcpy_h2d(ohx_ctx* ctx, void* d_dst, const void* src, size_t bytes);
 int ohx_memcpy_d2h(ohx_ctx* ctx, void* dst, const void* d_src, size_t bytes);
 int ohx_memcpy_d2d(ohx_ctx* ctx, void* d_dst, const void* d_src, size_t bytes);

@@ -25,7 +25,7 @@ ENCODE_PHASE, ENCODE_AMPLITUDE, ENCODE_REAL, ENCODE_IMAGINARY = 0, 1, 2, 3
 # every symbol include/openholo_amd.h declares (checked by the CPU test suite)
 EXPORTS = [
     "ohx_ctx_create", "ohx_ctx_destroy", "ohx_ctx_stream", "ohx_ctx_device", "ohx_status_string",
-    "ohx_last_error", "ohx_mem_info", "ohx_malloc", "ohx_free", "ohx_memcpy_h2d", "ohx_memcpy_d2h",
+    "ohx_last_error", "ohx_mem_info", "ohx_malloc", "ohx_free", "ohx_host_alloc", "ohx_host_free", "ohx_memcpy_h2d", "ohx_memcpy_d2h",
     "ohx_memcpy_d2d", "ohx_memcpy_d2h_after", "ohx_synchronize", "ohx_field_zero", "ohx_pc_diffract", "ohx_field_accumulate",
     "ohx_pc_probe_f64", "ohx_wrp_generate", "ohx_encode", "ohx_encode_ssb", "ohx_normalize_u8", "ohx_last_launch_stats",
     "ohx_server_create", "ohx_server_submit", "ohx_server_wait", "ohx_server_cancel",
@@ -142,6 +142,8 @@ def lib():
             "ohx_mem_info": ([vp, ctypes.POINTER(sz), ctypes.POINTER(sz)], i32),
             "ohx_malloc": ([vp, sz, ctypes.POINTER(vp)], i32),
             "ohx_free": ([vp, vp], i32),
+            "ohx_host_alloc": ([sz, ctypes.POINTER(vp)], i32),
+            "ohx_host_free": ([vp], i32),
             "ohx_memcpy_h2d": ([vp, vp, vp, sz], i32),
             "ohx_memcpy_d2h": ([vp, vp, vp, sz], i32),
             "ohx_memcpy_d2d": ([vp, vp, vp, sz], i32),
@@ -170,6 +172,33 @@ def lib():
             f.restype = res
         _lib = L
     return _lib
+
+
+class HostBuffer:
+    """Page-locked, zero-filled host memory (ohx_host_alloc), the memory the
+    drop-in's complex_H planes live in; `array` views it as numpy."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = int(nbytes)
+        p = ctypes.c_void_p()
+        st = lib().ohx_host_alloc(self.nbytes, ctypes.byref(p))
+        if st != OK:
+            raise OhxError(st, "ohx_host_alloc", f"{self.nbytes} bytes")
+        self.ptr = p.value
+
+    def array(self, dtype, count: int) -> np.ndarray:
+        dtype = np.dtype(dtype)
+        if count * dtype.itemsize > self.nbytes:
+            raise ValueError("view larger than the buffer")
+        raw = (ctypes.c_char * (count * dtype.itemsize)).from_address(self.ptr)
+        return np.frombuffer(raw, dtype=dtype, count=count)
+
+    def free(self):
+        if self.ptr:
+            st = lib().ohx_host_free(self.ptr)
+            self.ptr = None
+            if st != OK:
+                raise OhxError(st, "ohx_host_free", "")
 
 
 class DeviceBuffer:

@@ -251,6 +251,30 @@ int ohx_free(ohx_ctx* c, void* p)
     return check_hip(c, hipFree(p), "hipFree");
 }
 
+int ohx_host_alloc(size_t bytes, void** p)
+{
+    if (!p) return OHX_ERR_INVALID_ARG;
+    *p = nullptr;
+    if (bytes == 0) return OHX_OK;
+    if (hipHostMalloc(p, bytes, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        *p = nullptr;
+        return OHX_ERR_DEVICE;
+    }
+    std::memset(*p, 0, bytes);
+    return OHX_OK;
+}
+
+int ohx_host_free(void* p)
+{
+    if (!p) return OHX_OK;
+    if (hipHostFree(p) != hipSuccess) {
+        (void)hipGetLastError();
+        return OHX_ERR_DEVICE;
+    }
+    return OHX_OK;
+}
+
 static int copy(ohx_ctx* c, void* d, const void* s, size_t n, hipMemcpyKind k)
 {
     if (!c || (n && (!d || !s))) return OHX_ERR_INVALID_ARG;

@@ -66,9 +66,17 @@ protected:
     bool checkExtension(const char* fname, const char* ext);
     bool mergeColor(int idx, int width, int height, uchar* src, uchar* dst);
     virtual void ophFree(void);
+    // complex_H as `planes` zeroed planes of npx pixels, page-locked when a
+    // device is present (the per-channel read-back then overlaps the next
+    // channel's kernels), else plain heap memory; frees the previous planes.
+    void allocComplexField(uint planes, uint64_t npx);
+    void freeComplexField(void);
 
     OphConfig context_;
     ImageConfig imgCfg;
     Complex<Real>** complex_H = nullptr;
+    uint complex_H_planes = 0;
+    uint64_t complex_H_pixels = 0;   // per plane
+    bool complex_H_pinned = false;
     unsigned long refCnt = 0;
 };

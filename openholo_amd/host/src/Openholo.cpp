@@ -3,14 +3,58 @@
 
 #include <cstring>
 #include <fstream>
+#include <memory>
 #include <vector>
+
+#include "openholo_amd.h"
 
 Openholo::Openholo(void) {}
 
 Openholo::~Openholo(void)
 {
+    freeComplexField();
     delete[] context_.wave_length;
     context_.wave_length = nullptr;
+}
+
+void Openholo::allocComplexField(uint planes, uint64_t npx)
+{
+    freeComplexField();
+    complex_H = new Complex<Real>*[planes]();
+    complex_H_planes = planes;
+    complex_H_pixels = npx;
+    complex_H_pinned = true;
+    for (uint i = 0; i < planes; ++i) {
+        void* p = nullptr;
+        if (ohx_host_alloc(npx * sizeof(Complex<Real>), &p) != OHX_OK) {
+            // no device (CPU-only use of the host classes): heap planes
+            for (uint j = 0; j < i; ++j) {
+                ohx_host_free(complex_H[j]);
+                complex_H[j] = nullptr;
+            }
+            complex_H_pinned = false;
+            break;
+        }
+        complex_H[i] = static_cast<Complex<Real>*>(p);
+        std::uninitialized_value_construct_n(complex_H[i], npx);
+    }
+    if (!complex_H_pinned)
+        for (uint i = 0; i < planes; ++i) complex_H[i] = new Complex<Real>[npx];
+}
+
+void Openholo::freeComplexField(void)
+{
+    if (complex_H) {
+        for (uint i = 0; i < complex_H_planes; ++i) {
+            if (complex_H_pinned) ohx_host_free(complex_H[i]);
+            else delete[] complex_H[i];
+        }
+        delete[] complex_H;
+    }
+    complex_H = nullptr;
+    complex_H_planes = 0;
+    complex_H_pixels = 0;
+    complex_H_pinned = false;
 }
 
 unsigned long Openholo::release(void)
@@ -190,6 +234,10 @@ bool Openholo::saveAsOhc(const char* fname)
     if (!checkExtension(fname, ".ohc")) path += ".ohc";
     const uint32_t nw = context_.waveNum;
     const uint64_t npx = (uint64_t)context_.pixel_number[_X] * context_.pixel_number[_Y];
+    if (nw > complex_H_planes || npx != complex_H_pixels) {
+        OPH_LOG("<FAILED> saveAsOhc: the complex field does not match the context's size.\n");
+        return false;
+    }
     OhcFieldInfo fi{};
     fi.header_size = (uint32_t)(sizeof(OhcFieldInfo) + 8ull * nw);
     fi.px_num_x = (uint32_t)context_.pixel_number[_X];
@@ -257,18 +305,13 @@ bool Openholo::loadAsOhc(const char* fname)
     f.read((char*)wl.data(), 8 * nw);
     if (!f) return false;
     f.seekg(fh.data_offset);
-    if (complex_H) {
-        for (uint i = 0; i < context_.waveNum; ++i) delete[] complex_H[i];
-        delete[] complex_H;
-        complex_H = nullptr;
-    }
+    freeComplexField();
     setWaveNum((int)nw);
     for (uint32_t w = 0; w < nw; ++w) context_.wave_length[w] = wl[w];
     context_.pixel_pitch = vec2(fi.px_pitch_x, fi.px_pitch_y);
     setPixelNumber(ivec2((int)fi.px_num_x, (int)fi.px_num_y));
     context_.k = (2 * M_PI) / context_.wave_length[0];
-    complex_H = new Complex<Real>*[nw];
-    for (uint32_t w = 0; w < nw; ++w) complex_H[w] = new Complex<Real>[npx];
+    allocComplexField(nw, npx);
     std::vector<char> buf(npx * es);
     auto value = [es](const char* p) {
         if (es == 8) {
@@ -300,9 +343,5 @@ bool Openholo::loadAsOhc(const char* fname)
 
 void Openholo::ophFree(void)
 {
-    if (complex_H) {
-        for (uint i = 0; i < context_.waveNum; ++i) delete[] complex_H[i];
-        delete[] complex_H;
-        complex_H = nullptr;
-    }
+    freeComplexField();
 }

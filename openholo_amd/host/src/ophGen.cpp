@@ -59,10 +59,6 @@ void ophGen::initialize(void)
         m_lastError = "initialize: need 1..3 wavelengths and 1..2^28 pixels";
         return;
     }
-    if (complex_H) {
-        for (uint i = 0; i < m_nOldChannel; ++i) delete[] complex_H[i];
-        delete[] complex_H;
-    }
     if (m_lpEncoded) {
         for (uint i = 0; i < m_nOldChannel; ++i) delete[] m_lpEncoded[i];
         delete[] m_lpEncoded;
@@ -71,11 +67,10 @@ void ophGen::initialize(void)
         for (uint i = 0; i < m_nOldChannel; ++i) delete[] m_lpNormalized[i];
         delete[] m_lpNormalized;
     }
-    complex_H = new Complex<Real>*[nch];
+    allocComplexField(nch, (uint64_t)n);
     m_lpEncoded = new Real*[nch];
     m_lpNormalized = new uchar*[nch];
     for (uint i = 0; i < nch; ++i) {
-        complex_H[i] = new Complex<Real>[n];
         m_lpEncoded[i] = new Real[n]();
         m_lpNormalized[i] = new uchar[n]();
     }
@@ -649,10 +644,6 @@ void ophGen::releaseBuffers(void)
         delete[] m_lpNormalized;
         m_lpNormalized = nullptr;
     }
-    if (complex_H) {
-        for (uint i = 0; i < m_nOldChannel; ++i) delete[] complex_H[i];
-        delete[] complex_H;
-        complex_H = nullptr;
-    }
+    freeComplexField();
     m_nOldChannel = 0;
 }

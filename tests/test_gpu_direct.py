@@ -426,6 +426,20 @@ def test_d2h_after_overlaps_next_launch(ctx):
         assert np.array_equal(early, f0.download(np.complex128, npx))
         want1 = capi.generate(ctx, v, pn, pp, [638e-9], 1, FP32)[0].ravel()
         assert np.array_equal(f1.download(np.complex128, npx), want1)
+        # the same into page-locked memory (the drop-in's complex_H planes)
+        host = capi.HostBuffer(16 * npx)
+        try:
+            pinned = host.array(np.complex128, npx)
+            assert not pinned.any()                          # zero-filled
+            ctx.field_zero(f1, npx)
+            ctx.diffract(dv, len(v), f1, p1, 1, FP32)
+            f0.download_after(pinned)
+            ctx.sync()
+            assert np.array_equal(pinned, early)
+            assert np.array_equal(f1.download(np.complex128, npx), want1)
+            del pinned
+        finally:
+            host.free()
     finally:
         dv.free()
         f0.free()

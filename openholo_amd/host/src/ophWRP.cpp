@@ -118,7 +118,8 @@ bool ophWRP::calculateWRP(void)
         delete[] p_wrp_;
         p_wrp_ = new Complex<Real>[npx];
     }
-    for (uint ch = 0; ch < nch; ++ch) {
+    // channel ch's field is read back while channel ch + 1 is generated
+    auto launch = [&](uint ch) {
         const Real lambda = context_.wave_length[ch];
         context_.k = (2 * M_PI) / lambda;
         ohx_wrp_params p{};
@@ -132,15 +133,21 @@ bool ophWRP::calculateWRP(void)
         p.random_phase = 0;
         p.wrp_location = wrp_config_.wrp_location;
         p.propagation_distance = wrp_config_.propagation_distance;
-        int st = ohx_wrp_generate(c, d_vertices, (int64_t)n, &p, d_field[ch], d_wrp);
-        if (!st) st = ohx_memcpy_d2h(c, complex_H[ch], d_field[ch], 16 * (size_t)d_pixels);
-        if (st) {
-            fail("WRP generation", st);
-            return false;
-        }
+        return ohx_wrp_generate(c, d_vertices, (int64_t)n, &p, d_field[ch], d_wrp);
+    };
+    int st = launch(0);
+    for (uint ch = 0; !st && ch < nch; ++ch) {
+        if (ch + 1 < nch && (st = launch(ch + 1))) break;
+        st = ohx_memcpy_d2h_after(c, complex_H[ch], d_field[ch], 16 * (size_t)d_pixels);
         m_nProgress = (ch + 1) * 100 / nch;
     }
-    const int st = ohx_memcpy_d2h(c, p_wrp_, d_wrp, 16 * (size_t)npx);
+    if (st) {
+        (void)ohx_synchronize(c);
+        fail("WRP generation", st);
+        return false;
+    }
+    // the plane of the last channel, as the reference leaves in p_wrp_
+    st = ohx_memcpy_d2h(c, p_wrp_, d_wrp, 16 * (size_t)npx);
     if (st) {
         fail("WRP download", st);
         return false;

@@ -49,3 +49,31 @@ def test_null_context_rejected():
     p = capi.params((16, 16), (8e-6, 8e-6), 532e-9)
     assert lib.ohx_pc_diffract(None, None, 0, None, ctypes.byref(p), 0, 2) == -1
     assert lib.ohx_field_zero(None, None, 0) == -1
+
+
+def test_host_alloc_contract():
+    """ohx_host_alloc: page-locked, zero-filled memory where a device is
+    present; without one OHX_ERR_DEVICE and a null pointer, the status the
+    host classes take as the cue to use heap planes.  Null and zero-size
+    arguments behave as documented either way."""
+    from openholo_amd import capi
+    lib = capi.lib()
+    p = ctypes.c_void_p(1)
+    assert lib.ohx_host_alloc(0, ctypes.byref(p)) == capi.OK and p.value is None
+    assert lib.ohx_host_alloc(64, None) == -1          # OHX_ERR_INVALID_ARG
+    assert lib.ohx_host_free(None) == capi.OK
+    try:
+        buf = capi.HostBuffer(1 << 20)
+    except capi.OhxError as e:
+        assert e.status == -3                          # OHX_ERR_DEVICE: no device here
+        q = ctypes.c_void_p(1)
+        assert lib.ohx_host_alloc(1 << 20, ctypes.byref(q)) == -3 and q.value is None
+        return
+    try:
+        a = buf.array("u1", 1 << 20)
+        assert not a.any()
+        a[:] = 7
+        assert int(buf.array("u1", 1 << 20).sum()) == 7 << 20
+        del a
+    finally:
+        buf.free()

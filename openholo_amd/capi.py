@@ -176,7 +176,8 @@ def lib():
 
 class HostBuffer:
     """Page-locked, zero-filled host memory (ohx_host_alloc), the memory the
-    drop-in's complex_H planes live in; `array` views it as numpy."""
+    drop-in's complex_H planes live in; `array` views it as numpy.  A view does
+    not keep the buffer alive: drop every view before `free`."""
 
     def __init__(self, nbytes: int):
         self.nbytes = int(nbytes)

@@ -21,6 +21,9 @@
 //                     overload (-e 7: single-side-band encodeHologram)
 //   --load-ohc F      only load the OHC file F (Openholo::loadAsOhc), print its
 //                     configuration as one JSON line, write --dump-field
+//   --resize W,H      (point cloud) after the first hologram, setResolution(W,H)
+//                     and generate again; encoding and the outputs then use it
+//   --dump-field2 F   raw complex128 fields of the second hologram
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -105,7 +108,8 @@ static int run_wrp(const std::string& config, const std::string& input, unsigned
 int main(int argc, char** argv)
 {
     std::string config, input, vtx, outdir = "Result", dump_field, dump_enc, dump_u8, ohc, dump_points, dump_wrp,
-        load_ohc;
+        load_ohc, dump_field2;
+    int resize_w = 0, resize_h = 0;
     bool no_generate = false, dump_config = false, two_arg_encoding = false;
     unsigned passband = 0;
     int alg = 0, flag = 0;
@@ -140,6 +144,13 @@ int main(int argc, char** argv)
         else if (a == "--dump-config") dump_config = true;
         else if (a == "--dump-wrp") dump_wrp = v, ++i;
         else if (a == "--load-ohc") load_ohc = v, ++i;
+        else if (a == "--resize") {
+            if (std::sscanf(v, "%d,%d", &resize_w, &resize_h) != 2 || resize_w < 1 || resize_h < 1) {
+                std::fprintf(stderr, "--resize takes W,H\n");
+                return 1;
+            }
+            ++i;
+        } else if (a == "--dump-field2") dump_field2 = v, ++i;
         else if (a == "--passband") passband = (unsigned)std::atoi(v), two_arg_encoding = true, ++i;
         else {
             std::fprintf(stderr, "unknown argument %s\n", a.c_str());
@@ -225,12 +236,30 @@ int main(int argc, char** argv)
     }
     std::printf("generateHologram: %.6f s\n", t);
     const uint nch = h->getContext().waveNum;
-    const long long npx = (long long)h->getContext().pixel_number[_X] * h->getContext().pixel_number[_Y];
+    long long npx = (long long)h->getContext().pixel_number[_X] * h->getContext().pixel_number[_Y];
     if (!ohc.empty() && !h->saveAsOhc(ohc.c_str())) return fail();
     if (!dump_field.empty()) {
         FILE* fp = std::fopen(dump_field.c_str(), "wb");
         for (uint c = 0; c < nch; ++c) std::fwrite(h->getComplexField()[c], 16, npx, fp);
         std::fclose(fp);
+    }
+    if (resize_w > 0) {
+        // a second hologram at another resolution in the same process (the
+        // host planes and device buffers are re-allocated); encoding and the
+        // outputs below then use it
+        h->setResolution(ivec2(resize_w, resize_h));
+        const Real t2 = h->generateHologram((uint)flag);
+        if (t2 <= 0.0) {
+            std::fprintf(stderr, "generateHologram after --resize failed: %s\n", h->lastError().c_str());
+            return fail();
+        }
+        std::printf("generateHologram (%dx%d): %.6f s\n", resize_w, resize_h, t2);
+        npx = (long long)resize_w * resize_h;
+        if (!dump_field2.empty()) {
+            FILE* fp = std::fopen(dump_field2.c_str(), "wb");
+            for (uint c = 0; c < nch; ++c) std::fwrite(h->getComplexField()[c], 16, npx, fp);
+            std::fclose(fp);
+        }
     }
     if (two_arg_encoding) h->encoding(encode, passband);
     else h->encoding(encode);

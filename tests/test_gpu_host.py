@@ -191,3 +191,32 @@ def test_cli_odd_width_bmp(tmp_path, merge, flip, rotate):
         for ch in range(3):
             got = _bmp_rows(d / "R" / f"{ch}_PointCloud_Fresnel_Phase.bmp", w, h, 1)
             assert np.array_equal(got[..., 0], tf(u[ch]))
+
+
+@pytest.mark.parametrize("diff", [0, 1])
+def test_cli_resize_between_holograms(tmp_path, diff):
+    """setResolution between two generateHologram calls in one process: the
+    drop-in frees and re-allocates its page-locked complex_H planes and its
+    device buffers.  Each field equals a one-shot generation at its own size
+    through the C ABI (same kernels, so to 1e-13), and the 8-bit output has
+    the second size."""
+    from openholo_amd import capi
+    d = tmp_path
+    run(os.path.join(BIN, "ophpc_cli"), "-a", "0", "-c", CFG_RGB, "-i", PLY, "-e", "0", "-m", "0", "-p", "1",
+        "-f", str(diff), "-o", str(d / "Result"), "--dump-points", str(d / "p.bin"),
+        "--dump-field", str(d / "f1.bin"), "--resize", "700,500", "--dump-field2", str(d / "f2.bin"),
+        "--dump-u8", str(d / "u.bin"))
+    v = np.fromfile(d / "p.bin", VERTEX)
+    pp, lams = (10e-6, 10e-6), [638e-9, 520e-9, 450e-9]
+    ctx = capi.Context(0)
+    try:
+        for name, pn in (("f1.bin", (1920, 1080)), ("f2.bin", (700, 500))):
+            f = np.fromfile(d / name, np.complex128).reshape(3, pn[1], pn[0])
+            want = capi.generate(ctx, v, pn, pp, lams, diff, capi.MODE_GPU | capi.MODE_FLOAT,
+                                 scale=(0.01, 0.01, 0.01), distance=0.5)
+            for ch in range(3):
+                assert np.abs(f[ch]).max() > 0, (name, ch)
+                assert rel_l2(f[ch], want[ch]) < 1e-13, (name, ch)
+    finally:
+        ctx.close()
+    assert os.path.getsize(d / "u.bin") == 3 * 700 * 500

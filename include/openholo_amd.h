@@ -105,12 +105,21 @@ int ohx_host_free(void* ptr);
 int ohx_memcpy_h2d(ohx_ctx* ctx, void* d_dst, const void* src, size_t bytes);
 int ohx_memcpy_d2h(ohx_ctx* ctx, void* dst, const void* d_src, size_t bytes);
 int ohx_memcpy_d2d(ohx_ctx* ctx, void* d_dst, const void* d_src, size_t bytes);
-/* Device-to-host copy of d_src once all work enqueued so far on ctx's stream
- * has finished, on a second stream: work enqueued on ctx's stream later (the
- * next channel) runs while the copy is in flight.  Returns when the copy is
- * complete; that later work must not write d_src.  Replaces the per-channel
- * cudaMemcpy of the field in ophgen/src/ophPointCloud_GPU.cpp:272, so that
- * generateHologram's read-back of one channel overlaps the next channel. */
+/* Read-back that overlaps later kernels.  Replaces the per-channel
+ * cudaMemcpy of the field in ophgen/src/ophPointCloud_GPU.cpp:272.
+ *
+ * ohx_memcpy_d2h_async_after enqueues a device-to-host copy of d_src on the
+ * context's second (copy) stream, ordered after the work enqueued on ctx's
+ * stream up to this call and after nothing enqueued later, and returns at
+ * once.  Work enqueued on ctx's stream afterwards (the next channel) runs
+ * while the copy is in flight; it must not write d_src, and dst must stay
+ * valid and untouched until ohx_copy_synchronize returns.  With page-locked
+ * dst (ohx_host_alloc) the copy is asynchronous; with pageable dst the
+ * runtime may complete it before returning.
+ * ohx_copy_synchronize waits for every copy enqueued so far.
+ * ohx_memcpy_d2h_after is the two in sequence for one copy (blocking). */
+int ohx_memcpy_d2h_async_after(ohx_ctx* ctx, void* dst, const void* d_src, size_t bytes);
+int ohx_copy_synchronize(ohx_ctx* ctx);
 int ohx_memcpy_d2h_after(ohx_ctx* ctx, void* dst, const void* d_src, size_t bytes);
 int ohx_synchronize(ohx_ctx* ctx);
 
@@ -142,6 +151,14 @@ int ohx_field_accumulate(ohx_ctx* ctx, double* d_dst, const double* d_src, int64
 int ohx_pc_probe_f64(ohx_ctx* ctx, const void* d_vertices, int64_t n_points,
                      const ohx_pc_params* params, uint32_t diff_flag,
                      const int64_t* d_pix, int64_t n_pix, double* d_out);
+
+/* Self-check of the FP32 sine/cosine the FP32 kernels use: d_sin[i] =
+ * sin(2 pi x[i]), d_cos[i] = cos(2 pi x[i]) for FP32 arguments in revolutions,
+ * evaluated by the same device functions (the hardware instructions, which
+ * reduce the argument themselves).  No reference counterpart; the test suite
+ * pins their accuracy on unreduced arguments up to the largest the kernels
+ * produce. */
+int ohx_sincos_rev(ohx_ctx* ctx, const float* d_x, int64_t n, float* d_sin, float* d_cos);
 
 /* ---- WRP (wavefront recording plane) generator -----------------------------
  * One channel of ophWRP::generateHologram (ophgen/src/ophWRP.cpp:437-463):

@@ -26,8 +26,8 @@ ENCODE_PHASE, ENCODE_AMPLITUDE, ENCODE_REAL, ENCODE_IMAGINARY = 0, 1, 2, 3
 EXPORTS = [
     "ohx_ctx_create", "ohx_ctx_destroy", "ohx_ctx_stream", "ohx_ctx_device", "ohx_status_string",
     "ohx_last_error", "ohx_mem_info", "ohx_malloc", "ohx_free", "ohx_host_alloc", "ohx_host_free", "ohx_memcpy_h2d", "ohx_memcpy_d2h",
-    "ohx_memcpy_d2d", "ohx_memcpy_d2h_after", "ohx_synchronize", "ohx_field_zero", "ohx_pc_diffract", "ohx_field_accumulate",
-    "ohx_pc_probe_f64", "ohx_wrp_generate", "ohx_encode", "ohx_encode_ssb", "ohx_normalize_u8", "ohx_last_launch_stats",
+    "ohx_memcpy_d2d", "ohx_memcpy_d2h_async_after", "ohx_copy_synchronize", "ohx_memcpy_d2h_after", "ohx_synchronize", "ohx_field_zero", "ohx_pc_diffract", "ohx_field_accumulate",
+    "ohx_pc_probe_f64", "ohx_sincos_rev", "ohx_wrp_generate", "ohx_encode", "ohx_encode_ssb", "ohx_normalize_u8", "ohx_last_launch_stats",
     "ohx_server_create", "ohx_server_submit", "ohx_server_wait", "ohx_server_cancel",
     "ohx_server_counters_get", "ohx_server_destroy",
 ]
@@ -147,12 +147,15 @@ def lib():
             "ohx_memcpy_h2d": ([vp, vp, vp, sz], i32),
             "ohx_memcpy_d2h": ([vp, vp, vp, sz], i32),
             "ohx_memcpy_d2d": ([vp, vp, vp, sz], i32),
+            "ohx_memcpy_d2h_async_after": ([vp, vp, vp, sz], i32),
+            "ohx_copy_synchronize": ([vp], i32),
             "ohx_memcpy_d2h_after": ([vp, vp, vp, sz], i32),
             "ohx_synchronize": ([vp], i32),
             "ohx_field_zero": ([vp, vp, i64], i32),
             "ohx_pc_diffract": ([vp, vp, i64, vp, ctypes.POINTER(PcParams), u32, u32], i32),
             "ohx_field_accumulate": ([vp, vp, vp, i64], i32),
             "ohx_pc_probe_f64": ([vp, vp, i64, ctypes.POINTER(PcParams), u32, vp, i64, vp], i32),
+            "ohx_sincos_rev": ([vp, vp, i64, vp, vp], i32),
             "ohx_wrp_generate": ([vp, vp, i64, ctypes.POINTER(WrpParams), vp, vp], i32),
             "ohx_encode": ([vp, vp, i64, vp, u32], i32),
             "ohx_encode_ssb": ([vp, vp, i32, i32, ctypes.c_double, ctypes.c_double, ctypes.c_double,
@@ -235,6 +238,15 @@ class DeviceBuffer:
                         "d2h_after")
         return out
 
+    def download_async_after(self, out: np.ndarray):
+        """ohx_memcpy_d2h_async_after into `out`: enqueued after the work enqueued
+        so far on the context's stream; `out` is valid after Context.copy_sync()
+        and must stay alive until then."""
+        assert out.flags["C_CONTIGUOUS"] and out.nbytes <= self.nbytes
+        self.ctx._check(lib().ohx_memcpy_d2h_async_after(self.ctx.h, out.ctypes.data, self.ptr, out.nbytes),
+                        "d2h_async_after")
+        return out
+
 
 class Context:
     def __init__(self, device: int = 0):
@@ -275,6 +287,10 @@ class Context:
 
     def sync(self):
         self._check(lib().ohx_synchronize(self.h), "ohx_synchronize")
+
+    def copy_sync(self):
+        """ohx_copy_synchronize: wait for every read-back enqueued so far."""
+        self._check(lib().ohx_copy_synchronize(self.h), "ohx_copy_synchronize")
 
     def field_zero(self, field: DeviceBuffer, n_px: int):
         self._check(lib().ohx_field_zero(self.h, field.ptr, int(n_px)), "ohx_field_zero")

@@ -143,6 +143,18 @@ __global__ void k_field_acc(double2* __restrict__ d, const double2* __restrict__
     }
 }
 
+__global__ void k_sincos_rev(const float* __restrict__ x, int64_t n, float* __restrict__ s,
+                             float* __restrict__ c)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) {
+        const float v = x[i];
+        s[i] = sin_rev(v);
+        c[i] = cos_rev(v);
+    }
+}
+
 static unsigned grid_for(int64_t n, int block)
 {
     int64_t g = (n + block - 1) / block;
@@ -289,20 +301,33 @@ int ohx_memcpy_h2d(ohx_ctx* c, void* d, const void* s, size_t n) { return copy(c
 int ohx_memcpy_d2h(ohx_ctx* c, void* d, const void* s, size_t n) { return copy(c, d, s, n, hipMemcpyDeviceToHost); }
 int ohx_memcpy_d2d(ohx_ctx* c, void* d, const void* s, size_t n) { return copy(c, d, s, n, hipMemcpyDeviceToDevice); }
 
-int ohx_memcpy_d2h_after(ohx_ctx* c, void* d, const void* s, size_t n)
+int ohx_memcpy_d2h_async_after(ohx_ctx* c, void* d, const void* s, size_t n)
 {
     if (!c || (n && (!d || !s))) return OHX_ERR_INVALID_ARG;
     if (!n) return OHX_OK;
-    int st;
-    {
-        std::lock_guard<std::mutex> lk(c->mu);   // one copy_ready event per context
-        (void)hipSetDevice(c->device);
-        st = check_hip(c, hipEventRecord(c->copy_ready, c->stream), "hipEventRecord");
-        if (!st) st = check_hip(c, hipStreamWaitEvent(c->copy_stream, c->copy_ready, 0), "hipStreamWaitEvent");
-        if (!st) st = check_hip(c, hipMemcpyAsync(d, s, n, hipMemcpyDeviceToHost, c->copy_stream), "hipMemcpyAsync");
-    }
-    if (!st) st = check_hip(c, hipStreamSynchronize(c->copy_stream), "hipStreamSynchronize");
+    // The event marks the end of the work enqueued on c->stream so far; the
+    // copy stream's wait binds to this record, so re-recording the event for
+    // the next copy does not move this copy's dependency.
+    std::lock_guard<std::mutex> lk(c->mu);
+    (void)hipSetDevice(c->device);
+    int st = check_hip(c, hipEventRecord(c->copy_ready, c->stream), "hipEventRecord");
+    if (!st) st = check_hip(c, hipStreamWaitEvent(c->copy_stream, c->copy_ready, 0), "hipStreamWaitEvent");
+    if (!st) st = check_hip(c, hipMemcpyAsync(d, s, n, hipMemcpyDeviceToHost, c->copy_stream), "hipMemcpyAsync");
     return st;
+}
+
+int ohx_copy_synchronize(ohx_ctx* c)
+{
+    if (!c) return OHX_ERR_INVALID_ARG;
+    (void)hipSetDevice(c->device);
+    return check_hip(c, hipStreamSynchronize(c->copy_stream), "hipStreamSynchronize(copy)");
+}
+
+int ohx_memcpy_d2h_after(ohx_ctx* c, void* d, const void* s, size_t n)
+{
+    int st = ohx_memcpy_d2h_async_after(c, d, s, n);
+    const int st2 = c ? ohx_copy_synchronize(c) : OHX_ERR_INVALID_ARG;
+    return st ? st : st2;
 }
 
 int ohx_synchronize(ohx_ctx* c)
@@ -354,6 +379,15 @@ int ohx_pc_diffract(ohx_ctx* c, const void* d_vertices, int64_t n, double* d_fie
     if (mfma && diff == OHX_DIFF_FRESNEL) return launch_fresnel_gemm(c, g, soa, d_field, mode);
     const bool fp64 = (mode & (OHX_MODE_FLOAT | OHX_MODE_FASTMATH)) == 0 && mfma == 0;
     return launch_direct(c, g, soa, d_field, fp64);
+}
+
+int ohx_sincos_rev(ohx_ctx* c, const float* d_x, int64_t n, float* d_sin, float* d_cos)
+{
+    if (!c || n < 0 || (n && (!d_x || !d_sin || !d_cos))) return OHX_ERR_INVALID_ARG;
+    if (!n) return OHX_OK;
+    (void)hipSetDevice(c->device);
+    hipLaunchKernelGGL(k_sincos_rev, dim3(grid_for(n, 256)), dim3(256), 0, c->stream, d_x, n, d_sin, d_cos);
+    return check_hip(c, hipGetLastError(), "k_sincos_rev");
 }
 
 int ohx_pc_probe_f64(ohx_ctx* c, const void* d_vertices, int64_t n, const ohx_pc_params* p,

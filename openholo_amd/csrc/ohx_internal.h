@@ -39,6 +39,14 @@ struct ohx_ctx {
 
 namespace ohx {
 
+// sin(2 pi x) and cos(2 pi x) of an FP32 argument in revolutions.
+// v_sin_f32 / v_cos_f32 take revolutions and reduce them themselves, so the
+// FP32 kernels pass unreduced phases (below 65 cycles in the direct sums,
+// below 21 in the matrix-core operand generation); ohx_sincos_rev evaluates
+// exactly these two functions so that tests can pin their accuracy there.
+__device__ inline float sin_rev(float x) { return __builtin_amdgcn_sinf(x); }
+__device__ inline float cos_rev(float x) { return __builtin_amdgcn_cosf(x); }
+
 // Points per LDS chunk of the FP32 direct kernel: its FP32 partial sums are
 // folded into FP64 at chunk ends, so a point slice that is a whole number of
 // chunks sums exactly as the one-shot launch does.

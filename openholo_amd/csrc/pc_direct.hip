@@ -173,11 +173,8 @@ __global__ void k_zero_u64(unsigned long long* p)
 }
 
 // ---------------------------------------------------------------------------
-// v_sin_f32 / v_cos_f32 take revolutions and reduce them themselves: on
-// gfx950 sin(x) and sin(fract(x)) agree to 1 ulp for |x| <= 256, so the FP32
-// phases (below MAX_CYC + 1 cycles by the classification) go in unreduced.
-__device__ inline float sin_rev(float x) { return __builtin_amdgcn_sinf(x); }  // sin(2 pi x)
-__device__ inline float cos_rev(float x) { return __builtin_amdgcn_cosf(x); }  // cos(2 pi x)
+// sin_rev / cos_rev (ohx_internal.h): the FP32 phases (below MAX_CYC + 1
+// cycles by the classification) go in unreduced.
 
 // FP64 term (1/j) amp exp(j 2pi c) of one (point, pixel) pair, added to (re, im)
 template <int DIFF>

@@ -256,10 +256,10 @@ __device__ inline float2 elem_value(const Pt& P)
 {
     if constexpr (elem_is_v(E)) {
         const float f = phase_at(elem_index(E) - NVT / 2, P.v0, P.v1, P.v2);
-        return make_float2(P.amp * __builtin_amdgcn_cosf(f), P.amp * __builtin_amdgcn_sinf(f));
+        return make_float2(P.amp * cos_rev(f), P.amp * sin_rev(f));
     } else {
         const float f = phase_at(elem_index(E) - NUT / 2, P.u0, P.u1, P.u2);
-        return make_float2(__builtin_amdgcn_cosf(f), __builtin_amdgcn_sinf(f));
+        return make_float2(cos_rev(f), sin_rev(f));
     }
 }
 template <int VAR, int E>

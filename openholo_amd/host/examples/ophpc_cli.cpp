@@ -24,6 +24,12 @@
 //   --resize W,H      (point cloud) after the first hologram, setResolution(W,H)
 //                     and generate again; encoding and the outputs then use it
 //   --dump-field2 F   raw complex128 fields of the second hologram
+//   --conj-field      conjugate complex_H through getComplexField() after
+//                     generation (and after --dump-field), before encoding
+//   --set-pixel-number W,H  call setPixelNumber(W,H) alone (no setResolution)
+//                     after loading, then generateHologram, encoding and
+//                     normalize; print each call's lastError() and exit 3
+//                     when all three refused the stale host buffers
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -109,8 +115,8 @@ int main(int argc, char** argv)
 {
     std::string config, input, vtx, outdir = "Result", dump_field, dump_enc, dump_u8, ohc, dump_points, dump_wrp,
         load_ohc, dump_field2;
-    int resize_w = 0, resize_h = 0;
-    bool no_generate = false, dump_config = false, two_arg_encoding = false;
+    int resize_w = 0, resize_h = 0, spn_w = 0, spn_h = 0;
+    bool no_generate = false, dump_config = false, two_arg_encoding = false, conj_field = false;
     unsigned passband = 0;
     int alg = 0, flag = 0;
     unsigned encode = 0, mode = 0;
@@ -150,7 +156,14 @@ int main(int argc, char** argv)
                 return 1;
             }
             ++i;
+        } else if (a == "--set-pixel-number") {
+            if (std::sscanf(v, "%d,%d", &spn_w, &spn_h) != 2 || spn_w < 1 || spn_h < 1) {
+                std::fprintf(stderr, "--set-pixel-number takes W,H\n");
+                return 1;
+            }
+            ++i;
         } else if (a == "--dump-field2") dump_field2 = v, ++i;
+        else if (a == "--conj-field") conj_field = true;
         else if (a == "--passband") passband = (unsigned)std::atoi(v), two_arg_encoding = true, ++i;
         else {
             std::fprintf(stderr, "unknown argument %s\n", a.c_str());
@@ -229,6 +242,21 @@ int main(int argc, char** argv)
         h->release();
         return 0;
     }
+    if (spn_w > 0) {
+        h->setPixelNumber(spn_w, spn_h);
+        int refused = 0;
+        const Real t = h->generateHologram((uint)flag);
+        std::printf("generateHologram: %g s, lastError: %s\n", t, h->lastError().c_str());
+        refused += t == 0.0 && h->lastError().find("host buffers do not match") != std::string::npos;
+        h->encoding(encode);
+        std::printf("encoding lastError: %s\n", h->lastError().c_str());
+        refused += h->lastError().rfind("encoding: host buffers do not match", 0) == 0;
+        h->normalize();
+        std::printf("normalize lastError: %s\n", h->lastError().c_str());
+        refused += h->lastError().rfind("normalize: host buffers do not match", 0) == 0;
+        h->release();
+        return refused == 3 ? 3 : 1;
+    }
     const Real t = h->generateHologram((uint)flag);
     if (t <= 0.0) {
         std::fprintf(stderr, "generateHologram failed: %s\n", h->lastError().c_str());
@@ -260,6 +288,11 @@ int main(int argc, char** argv)
             for (uint c = 0; c < nch; ++c) std::fwrite(h->getComplexField()[c], 16, npx, fp);
             std::fclose(fp);
         }
+    }
+    if (conj_field) {
+        Complex<Real>** H = h->getComplexField();
+        for (uint c = 0; c < nch; ++c)
+            for (long long i = 0; i < npx; ++i) H[c][i][_IM] = -H[c][i][_IM];
     }
     if (two_arg_encoding) h->encoding(encode, passband);
     else h->encoding(encode);

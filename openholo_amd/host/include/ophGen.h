@@ -88,6 +88,11 @@ protected:
     ohx_ctx* device();
     bool ensureDeviceBuffers();
     bool uploadFields();
+    // complex_H and the encode buffers exist for every configured channel and
+    // hold pixel_number[_X] * pixel_number[_Y] pixels each (setPixelNumber
+    // without setResolution/initialize leaves them at the old size); sets
+    // lastError() and returns false otherwise
+    bool hostPlanesReady(const char* what);
     void fail(const char* what, int status);
 
     ivec2 m_vecEncodeSize;
@@ -107,7 +112,5 @@ protected:
     uint8_t** d_u8 = nullptr;
     uint d_channels = 0;
     long long d_pixels = 0;
-    bool d_field_valid = false;   // device fields hold complex_H
-    bool d_enc_valid = false;     // device encoded buffers hold m_lpEncoded
     std::string m_lastError;
 };

@@ -76,7 +76,6 @@ void ophGen::initialize(void)
     }
     m_nOldChannel = nch;
     m_vecEncodeSize = context_.pixel_number;
-    d_field_valid = d_enc_valid = false;
 }
 
 void ophGen::resetBuffer()
@@ -88,7 +87,6 @@ void ophGen::resetBuffer()
         if (m_lpEncoded && m_lpEncoded[ch]) std::memset(m_lpEncoded[ch], 0, sizeof(Real) * n2);
         if (m_lpNormalized && m_lpNormalized[ch]) std::memset(m_lpNormalized[ch], 0, n2);
     }
-    d_enc_valid = false;
 }
 
 void ophGen::setResolution(ivec2 resolution)
@@ -501,13 +499,29 @@ bool ophGen::ensureDeviceBuffers()
             return false;
         }
     }
-    d_field_valid = d_enc_valid = false;
+    return true;
+}
+
+bool ophGen::hostPlanesReady(const char* what)
+{
+    const uint nch = context_.waveNum;
+    const uint64_t npx = (uint64_t)context_.pixel_number[_X] * (uint64_t)context_.pixel_number[_Y];
+    if (nch == 0 || nch > 3 || !complex_H || m_nOldChannel != nch || complex_H_planes < nch ||
+        context_.pixel_number[_X] < 1 || context_.pixel_number[_Y] < 1 || complex_H_pixels != npx) {
+        m_lastError = std::string(what) +
+                      ": host buffers do not match the configured channels and pixel number "
+                      "(call setResolution or readConfig/initialize after changing them)";
+        OPH_LOG("<FAILED> %s\n", m_lastError.c_str());
+        return false;
+    }
     return true;
 }
 
 bool ophGen::uploadFields()
 {
-    if (d_field_valid) return true;
+    // complex_H is always taken from the host: callers may have edited it
+    // through getComplexField() since it was generated (the reference encodes
+    // from the host field, ophgen/src/ophGen.cpp:816-849)
     if (!complex_H || !ensureDeviceBuffers()) return false;
     ohx_ctx* c = device();
     for (uint ch = 0; ch < context_.waveNum; ++ch) {
@@ -517,7 +531,6 @@ bool ophGen::uploadFields()
             return false;
         }
     }
-    d_field_valid = true;
     return true;
 }
 
@@ -528,6 +541,7 @@ void ophGen::encoding(unsigned int ENCODE_FLAG)
         OPH_LOG("<FAILED> WRONG PARAMETERS.\n");
         return;
     }
+    if (!hostPlanesReady("encoding")) return;
     m_vecEncodeSize = context_.pixel_number;
     if (!uploadFields()) return;
     ohx_ctx* c = device();
@@ -539,24 +553,22 @@ void ophGen::encoding(unsigned int ENCODE_FLAG)
             return;
         }
     }
-    d_enc_valid = true;
 }
 
 void ophGen::normalize(void)
 {
-    // ophgen/src/ophGen.cpp:654-687: global min/max over all channels, on the GPU
-    if (!ensureDeviceBuffers()) return;
+    // ophgen/src/ophGen.cpp:654-687: global min/max over all channels, on the GPU.
+    // The encoded buffers are taken from the host (callers may edit them
+    // through getEncodedBuffer()).
+    if (!hostPlanesReady("normalize") || !ensureDeviceBuffers()) return;
     ohx_ctx* c = device();
     const uint nch = context_.waveNum;
-    if (!d_enc_valid) {
-        for (uint ch = 0; ch < nch; ++ch) {
-            const int st = ohx_memcpy_h2d(c, d_enc[ch], m_lpEncoded[ch], 8 * (size_t)d_pixels);
-            if (st) {
-                fail("encoded buffer upload", st);
-                return;
-            }
+    for (uint ch = 0; ch < nch; ++ch) {
+        const int st = ohx_memcpy_h2d(c, d_enc[ch], m_lpEncoded[ch], 8 * (size_t)d_pixels);
+        if (st) {
+            fail("encoded buffer upload", st);
+            return;
         }
-        d_enc_valid = true;
     }
     std::vector<const double*> e(d_enc, d_enc + nch);
     std::vector<uint8_t*> o(d_u8, d_u8 + nch);
@@ -611,7 +623,6 @@ bool ophGen::loadAsOhc(const char* fname)
     }
     m_nOldChannel = nch;
     m_vecEncodeSize = context_.pixel_number;
-    d_field_valid = d_enc_valid = false;
     return true;
 }
 

@@ -91,14 +91,10 @@ bool ophPointCloud::genCghPointCloudGPU(uint diff_flag)
 {
     // replaces ophgen/src/ophPointCloud_GPU.cpp:179-412: vertices stay resident
     // on the device between calls; one diffraction launch per channel
+    if (!hostPlanesReady("generateHologram")) return false;
     ohx_ctx* c = device();
     if (!c) return false;
     const uint nch = context_.waveNum;
-    if (nch == 0 || nch > 3 || !complex_H || m_nOldChannel != nch) {
-        m_lastError = "configure wavelengths (1..3) and call readConfig/initialize first";
-        OPH_LOG("<FAILED> %s\n", m_lastError.c_str());
-        return false;
-    }
     if (!ensureDeviceBuffers()) return false;
     const ulonglong n = pc_data_.n_points;
     std::vector<Vertex> vw;
@@ -147,19 +143,21 @@ bool ophPointCloud::genCghPointCloudGPU(uint diff_flag)
         if (!st) st = ohx_pc_diffract(c, d_vertices, (int64_t)n, d_field[ch], &p, diff_flag, m_mode);
         return st;
     };
-    int st = launch(0);
+    // each channel's copy is ordered after its own launch only, so it runs
+    // while the next channel's kernels do
+    int st = OHX_OK;
     for (uint ch = 0; !st && ch < nch; ++ch) {
-        if (ch + 1 < nch && (st = launch(ch + 1))) break;
-        st = ohx_memcpy_d2h_after(c, complex_H[ch], d_field[ch], 16 * (size_t)d_pixels);
-        m_nProgress = (ch + 1) * 100 / nch;
+        st = launch(ch);
+        if (!st) st = ohx_memcpy_d2h_async_after(c, complex_H[ch], d_field[ch], 16 * (size_t)d_pixels);
     }
+    const int st_copy = ohx_copy_synchronize(c);
+    if (!st) st = st_copy;
+    m_nProgress = 100;
     if (st) {
         (void)ohx_synchronize(c);
         fail("diffraction", st);
         return false;
     }
-    d_field_valid = true;
-    d_enc_valid = false;
     return true;
 }
 
@@ -182,6 +180,7 @@ void ophPointCloud::encodeHologram(vec2 band_limit, vec2 spectrum_shift)
         OPH_LOG("<FAILED> Not found diffracted data.");
         return;
     }
+    if (!hostPlanesReady("encodeHologram")) return;
     const int pnX = context_.pixel_number[_X], pnY = context_.pixel_number[_Y];
     m_vecEncodeSize = ivec2(pnX, pnY);
     context_.ss[_X] = pnX * context_.pixel_pitch[_X];
@@ -197,7 +196,6 @@ void ophPointCloud::encodeHologram(vec2 band_limit, vec2 spectrum_shift)
             return;
         }
     }
-    d_enc_valid = true;
 }
 
 void ophPointCloud::ophFree(void)

@@ -73,11 +73,7 @@ bool ophWRP::calculateWRP(void)
     ohx_ctx* c = device();
     if (!c) return false;
     const uint nch = context_.waveNum;
-    if (nch == 0 || nch > 3 || !complex_H || m_nOldChannel != nch) {
-        m_lastError = "configure wavelengths (1..3) and call readConfig/initialize first";
-        OPH_LOG("<FAILED> %s\n", m_lastError.c_str());
-        return false;
-    }
+    if (!hostPlanesReady("WRP generateHologram")) return false;
     if (GetRandomPhase()) {
         m_lastError = "the WRP random phase is not supported";
         OPH_LOG("<FAILED> %s\n", m_lastError.c_str());
@@ -135,12 +131,14 @@ bool ophWRP::calculateWRP(void)
         p.propagation_distance = wrp_config_.propagation_distance;
         return ohx_wrp_generate(c, d_vertices, (int64_t)n, &p, d_field[ch], d_wrp);
     };
-    int st = launch(0);
+    int st = OHX_OK;
     for (uint ch = 0; !st && ch < nch; ++ch) {
-        if (ch + 1 < nch && (st = launch(ch + 1))) break;
-        st = ohx_memcpy_d2h_after(c, complex_H[ch], d_field[ch], 16 * (size_t)d_pixels);
-        m_nProgress = (ch + 1) * 100 / nch;
+        st = launch(ch);
+        if (!st) st = ohx_memcpy_d2h_async_after(c, complex_H[ch], d_field[ch], 16 * (size_t)d_pixels);
     }
+    const int st_copy = ohx_copy_synchronize(c);
+    if (!st) st = st_copy;
+    m_nProgress = 100;
     if (st) {
         (void)ohx_synchronize(c);
         fail("WRP generation", st);
@@ -152,8 +150,6 @@ bool ophWRP::calculateWRP(void)
         fail("WRP download", st);
         return false;
     }
-    d_field_valid = true;
-    d_enc_valid = false;
     return true;
 }
 

@@ -220,3 +220,27 @@ def test_cli_resize_between_holograms(tmp_path, diff):
     finally:
         ctx.close()
     assert os.path.getsize(d / "u.bin") == 3 * 700 * 500
+
+
+def test_cli_edit_of_complex_field_reaches_encoding(tmp_path, orc):
+    """A caller that edits complex_H through getComplexField() between
+    generateHologram and encoding gets the edited field encoded, as in the
+    reference, which encodes from the host field (ophGen.cpp:816-849): the
+    CLI conjugates every pixel, and the phase encoding equals the oracle's
+    encoder applied to the conjugated field."""
+    d = tmp_path
+    run(os.path.join(BIN, "ophpc_cli"), "-a", "0", "-c", CFG_RGB, "-i", PLY, "-e", "0", "-m", "0", "-p", "1",
+        "-f", "1", "-o", str(d / "Result"), "--dump-field", str(d / "f.bin"), "--conj-field",
+        "--dump-enc", str(d / "e.bin"), "--dump-u8", str(d / "u.bin"))
+    npx = 1920 * 1080
+    f = np.fromfile(d / "f.bin", np.complex128).reshape(3, npx)
+    e = np.fromfile(d / "e.bin", np.float64).reshape(3, npx)
+    u = np.fromfile(d / "u.bin", np.uint8).reshape(3, npx)
+    want_e = [orc.encode(np.conj(f[ch]), orc.ENCODE_PHASE) for ch in range(3)]
+    for ch in range(3):
+        assert np.allclose(e[ch], want_e[ch], rtol=0, atol=4e-16), ch
+        # and not the unedited field's encoding
+        assert not np.allclose(e[ch], orc.encode(f[ch], orc.ENCODE_PHASE), rtol=0, atol=1e-6)
+    want_u = orc.normalize(want_e)
+    for ch in range(3):
+        assert np.array_equal(u[ch], want_u[ch])

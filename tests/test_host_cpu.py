@@ -238,3 +238,18 @@ def test_load_ohc_rejects_inconsistent_headers(tmp_path, bad):
     p.write_bytes(_ohc_bytes(fields, (8e-6, 8e-6), (532e-9,), **bad))
     r = run_cli("--load-ohc", str(p), check=False)
     assert r.returncode == 1
+
+
+def test_set_pixel_number_without_initialize_is_refused(tmp_path):
+    """setPixelNumber alone enlarges the configured hologram but leaves the
+    host planes (complex_H, encoded, normalised) at the size initialize()
+    gave them.  generateHologram, encoding and normalize refuse to run
+    rather than write past those planes; each reports it through
+    lastError().  Runs without a device: the check comes first."""
+    r = run_cli("-c", os.path.join(GOLD, "Dice_PointCloud.xml"), "-i", os.path.join(GOLD, "pointcloud_1470.ply"),
+                "-p", "1", "-f", "0", "-o", str(tmp_path / "R"), "--set-pixel-number", "2500,1500", check=False)
+    assert r.returncode == 3, r.stdout + r.stderr
+    lines = r.stdout.splitlines()
+    assert any(l.startswith("generateHologram: 0 s") and "host buffers do not match" in l for l in lines)
+    assert any(l.startswith("encoding lastError: encoding: host buffers do not match") for l in lines)
+    assert any(l.startswith("normalize lastError: normalize: host buffers do not match") for l in lines)

@@ -1,19 +1,29 @@
 #!/usr/bin/env python3
 """Benchmark of the point-cloud hologram path on one MI355X.
 
-Default workload (one "step"): the 100,000-point cloud (x,y uniform in
-+-8 mm, z in [0.45, 0.55] m, amplitude in [0.1, 1], SplitMix64 seed
-20261202+2) rendered at 3840x2160, 3.74 um pitch, three channels
-(638/532/473 nm) with both RS and Fresnel diffraction on the FP32 direct
-sum, then phase-encoded and normalised to 8 bits -- everything resident in
-HBM before the timed region starts.
+Headline workload (one "step"): the 1,000,000-point cloud with complex
+amplitudes (x, y uniform in +-8 mm, z in [0.4, 0.6] m, |a| in [0.1, 1],
+arg a in [0, 2 pi), SplitMix64 seed 20261202+3) rendered with Fresnel
+diffraction at 3840x2160, 3.74 um pitch, 532 nm on the BF16 3-term
+matrix-core kernel (H = U diag(a) V^T), then phase-encoded and normalised to
+8 bits -- everything resident in HBM before the timed region starts.
 
 value = point-pixel pairs per second (G/s) over the whole step.
-Also reported: field relative L2 against the library's FP64 probe on sampled
-pixels, p50/p99 ms per hologram (one diffraction mode, 3 channels + encode),
-the dominant kernel's roofline (VALU issue model, HIP events on the
-context's stream) and the CPU baseline (the reference's OpenMP CPU path,
-restated in oracle/, timed on a bounded sample on this host).
+Reported beside it:
+  * field relative L2 against the library's FP64 probe on sampled pixels
+    (the probe is pinned to the extended-precision oracle by the test suite
+    at every workload geometry), per diffraction mode;
+  * p50/p99 ms per hologram, per diffraction mode;
+  * the headline kernel's roofline (matrix-core FLOPs, HIP events on the
+    context's stream; HBM traffic from rocprofv3 counters);
+  * the same workload on the FP32 direct kernel and the other matrix-core
+    variants (pairs/s, rel-L2, speed-up over the direct kernel);
+  * the 100k-point RGB RS + Fresnel workload (3840x2160, FP32 direct), the
+    10k-point RS workload (1920x1080, FP32 and FP64) and one step of the
+    2M-point RGB workload (7680x4320, RS FP32 direct + Fresnel FP16 3-term);
+  * encode/normalise HBM rates, the WRP generator, the serving stream;
+  * the CPU baseline (the reference's OpenMP CPU path, restated in oracle/,
+    timed on a bounded sample on this host).
 """
 from __future__ import annotations
 
@@ -50,13 +60,15 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", type=int, default=2, choices=[1, 2, 3, 5])
+    ap.add_argument("--workload", type=int, default=3, choices=[1, 2, 3, 5])
     ap.add_argument("--points", type=int, default=0, help="override point count (0 = workload's)")
-    ap.add_argument("--kernel", default="direct", choices=list(KERNELS))
+    ap.add_argument("--kernel", default="bf16x3", choices=list(KERNELS))
     ap.add_argument("--fresnel-kernel", default=None, choices=list(KERNELS),
                     help="kernel for the Fresnel part of the step (default: --kernel)")
     ap.add_argument("--variants", type=int, default=1,
                     help="also time the 1M-point Fresnel workload on every Fresnel kernel (0 = off)")
+    ap.add_argument("--blocks", default="rgb2160,rs1080,large4320",
+                    help="other workloads to time beside the headline (comma list, '' = none)")
     ap.add_argument("--serving", type=int, default=1,
                     help="also run the mixed Poisson request stream on the serving engine (0 = off)")
     ap.add_argument("--requests", type=int, default=120)
@@ -121,10 +133,12 @@ def cpu_baseline(w, target_s: float):
             "host": {"cpu_model": _cpu_model(), "nproc": os.cpu_count(), "OMP_NUM_THREADS": omp_env or None}}
 
 
-# HBM bytes per launch of the direct kernels at the default workload
-# (rocprofv3 --pmc FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the scale factors read
-# off a 1 GiB copy kernel); profiles/k_pc_direct_rs_fresnel_100k_2160p.md.
-TRAFFIC = {0: 0.985e9, 1: 0.880e9}
+# HBM bytes per launch of the headline kernel (k_pc_prep + k_fresnel_gemm +
+# k_gemm_reduce) at the 1M-point 3840x2160 workload: rocprofv3 --pmc
+# FETCH_SIZE and WRITE_SIZE summed over the launch's dispatches, each scaled by
+# the factor the same counter reads on a copy kernel of known size;
+# profiles/traffic_1m_2160p.md gives the counts, the factors and the commit.
+TRAFFIC = {}
 # Speed of light of the direct kernels' separable pair loops (per lane and point,
 # 8 rows: column factor + 8 pairs) on this
 # hardware (openholo_amd/bin/valu_rates); profiles/valu_mfma_rates_gfx950.md.
@@ -214,10 +228,12 @@ def fresnel_variants(ctx, capi, synth, Event, launches=2):
 
 
 def serving_stream(capi, synth, n_requests: int):
-    """Mixed request stream on the continuous-batching server.  Pass 1 submits
-    every request at once to measure capacity (jobs/s); pass 2 replays the
+    """Mixed request stream on the continuous-batching server.  Pass 1 runs
+    every request alone (submit, wait) for its alone latency; pass 2 submits
+    every request at once to measure capacity (jobs/s); pass 3 replays the
     same requests with Poisson arrivals at 70% of that rate and reports
-    per-request latency (completion - arrival) and throughput."""
+    per-request latency (completion - arrival), latency per diffraction mode,
+    slowdown against the alone latency, and throughput."""
     import threading
     reqs = synth.serving_requests(n_requests)
     clouds = [synth.random_cloud(r["seed"], r["n"], 4e-3, (0.49, 0.51)) for r in reqs]
@@ -235,6 +251,11 @@ def serving_stream(capi, synth, n_requests: int):
         # warm-up: one request of each kind
         for i in range(min(4, len(reqs))):
             srv.wait(submit(srv, i))
+        alone = np.zeros(len(reqs))
+        for i in range(len(reqs)):
+            st, stt, _, _ = srv.wait(submit(srv, i))
+            assert st == capi.OK
+            alone[i] = stt.end_ms - stt.submit_ms
         t0 = time.perf_counter()
         ids = [submit(srv, i) for i in range(len(reqs))]
         for j in ids:
@@ -256,7 +277,6 @@ def serving_stream(capi, synth, n_requests: int):
                 ids[i] = submit(srv, i)
 
         th = threading.Thread(target=producer)
-        t1 = time.perf_counter()
         th.start()
         lat, first_submit, last_end = [], None, None
         for i in range(len(reqs)):
@@ -271,15 +291,240 @@ def serving_stream(capi, synth, n_requests: int):
         c = srv.counters()
     span_s = (last_end - first_submit) / 1e3
     lat = np.array(lat)
+    slow = lat / alone
+    diff = np.array([r["diff"] for r in reqs])
+    per_mode = {}
+    for d, name in ((1, "Fresnel"), (0, "RS")):
+        m = diff == d
+        if m.any():
+            per_mode[name] = {"requests": int(m.sum()), "p50_ms": _pctl(lat[m], 50), "p99_ms": _pctl(lat[m], 99),
+                              "alone_p50_ms": _pctl(alone[m], 50), "alone_p99_ms": _pctl(alone[m], 99),
+                              "slowdown_p50": _pctl(slow[m], 50), "slowdown_p99": _pctl(slow[m], 99)}
     return {
         "requests": len(reqs), "pairs": float(pairs),
         "burst_capacity_jobs_per_s": round(capacity, 2), "poisson_rate_jobs_per_s": round(rate, 2),
         "throughput_Gpairs_per_s": round(pairs / span_s / 1e9, 1), "burst_Gpairs_per_s": round(pairs / burst_s / 1e9, 1),
-        "p50_ms": round(float(np.percentile(lat, 50)), 2), "p99_ms": round(float(np.percentile(lat, 99)), 2),
-        "mean_ms": round(float(lat.mean()), 2), "iterations": int(c.iterations), "graph_replays": int(c.graph_replays),
+        "p50_ms": _pctl(lat, 50), "p99_ms": _pctl(lat, 99), "mean_ms": round(float(lat.mean()), 2),
+        "alone_p50_ms": _pctl(alone, 50), "alone_p99_ms": _pctl(alone, 99),
+        "slowdown_p50": _pctl(slow, 50), "slowdown_p99": _pctl(slow, 99),
+        "per_mode": per_mode,
+        "iterations": int(c.iterations), "graph_replays": int(c.graph_replays),
         "peak_reserved_GB": round(c.peak_reserved_bytes / 1e9, 2),
         "kernels": "RS: FP32 direct; Fresnel: BF16 3-term matrix cores",
     }
+
+
+MODE_NAMES = {0: "RS", 1: "Fresnel"}
+
+
+def _pctl(xs, q):
+    return round(float(np.percentile(xs, q)), 3) if len(xs) else None
+
+
+def run_workload(ctx, capi, synth, Event, w, kernel_of, steps, warmup, probe_pixels, n=None, label="",
+                 accuracy_channels=None):
+    """Time `steps` steps of workload w (every diffraction mode of w, every
+    channel: zero, diffract, then phase-encode and normalise the channels),
+    resident in HBM, with HIP events on the context's stream.  kernel_of maps
+    a diffraction flag to a KERNELS name.  Returns the measurements plus the
+    device buffers' host-side handles for further legs (caller frees).
+    accuracy_channels: channels whose field is checked against the probe
+    after the timed region (default all)."""
+    n = n or w.n_points
+    npx, nch = w.n_pixels, len(w.lambdas)
+    stream = ctx.stream
+    pts = synth.points(w, n)
+    dv = ctx.to_device(pts)
+    fields = [ctx.alloc(16 * npx) for _ in range(nch)]
+    encs = [ctx.alloc(8 * npx) for _ in range(nch)]
+    u8s = [ctx.alloc(npx) for _ in range(nch)]
+    params = [capi.params(w.pn, w.pp, lam, channel=ch, use_phase=w.complex_amplitude)
+              for ch, lam in enumerate(w.lambdas)]
+    kev, holo_ev, enc_ev = {}, {}, []
+
+    def hologram(diff, timed):
+        e0 = Event()
+        e0.record(stream)
+        for ch in range(nch):
+            ctx.field_zero(fields[ch], npx)
+            a, b = Event(), Event()
+            a.record(stream)
+            ctx.diffract(dv, n, fields[ch], params[ch], diff, KERNELS[kernel_of[diff]])
+            b.record(stream)
+            if timed:
+                kev.setdefault(diff, []).append((a, b))
+        em = Event()
+        em.record(stream)
+        for ch in range(nch):
+            ctx.encode(fields[ch], npx, encs[ch], capi.ENCODE_PHASE)
+        ctx.normalize_u8(encs, npx, u8s)
+        e1 = Event()
+        e1.record(stream)
+        if timed:
+            holo_ev.setdefault(diff, []).append((e0, e1))
+            enc_ev.append((em, e1))
+
+    for i in range(warmup):
+        for diff in w.modes:
+            hologram(diff, False)
+        ctx.sync()
+        print(f"# {label}warmup step {i + 1}/{warmup} done", file=sys.stderr, flush=True)
+    ctx.sync()
+    t_start, t_stop = Event(), Event()
+    wall0 = time.perf_counter()
+    t_start.record(stream)
+    for i in range(steps):
+        for diff in w.modes:
+            hologram(diff, True)
+    t_stop.record(stream)
+    ctx.sync()
+    wall = time.perf_counter() - wall0
+    ms_step = t_start.elapsed_ms(t_stop) / steps
+    pairs_step = n * npx * nch * len(w.modes)
+    print(f"# {label}timed {steps} steps: {ms_step:.1f} ms/step (wall {wall:.2f} s)", file=sys.stderr, flush=True)
+
+    launch_ms = {d: [a.elapsed_ms(b) for a, b in ev] for d, ev in kev.items()}
+    holo_ms = {d: [a.elapsed_ms(b) for a, b in ev] for d, ev in holo_ev.items()}
+    enc_ms = float(np.mean([a.elapsed_ms(b) for a, b in enc_ev])) if enc_ev else float("nan")
+
+    # accuracy: the field vs the library's FP64 probe on sampled pixels, and the
+    # share of nominal pairs inside the points' anti-aliasing boxes
+    rng = np.random.default_rng(1234)
+    pix = rng.choice(npx, min(probe_pixels, npx), replace=False)
+    rel, in_box = {}, {}
+    for diff in w.modes:
+        num = den = 0.0
+        boxed = nominal = 0
+        for ch in (range(nch) if accuracy_channels is None else accuracy_channels):
+            ctx.field_zero(fields[ch], npx)
+            ctx.diffract(dv, n, fields[ch], params[ch], diff, KERNELS[kernel_of[diff]])
+            st = ctx.last_stats()
+            boxed += st.pairs_in_box
+            nominal += st.pairs_nominal
+            f = fields[ch].download(np.complex128, npx)[pix]
+            ref = ctx.probe_f64(dv, n, params[ch], diff, pix)
+            num += float(np.sum(np.abs(f - ref) ** 2))
+            den += float(np.sum(np.abs(ref) ** 2))
+        rel[MODE_NAMES[diff]] = float(f"{np.sqrt(num / den):.3e}")
+        in_box[MODE_NAMES[diff]] = round(boxed / nominal, 6) if nominal else None
+    per_mode = {}
+    for diff in w.modes:
+        ms_l = float(np.mean(launch_ms[diff]))
+        per_mode[MODE_NAMES[diff]] = {
+            "kernel": kernel_of[diff], "ms_per_channel": round(ms_l, 3),
+            "Gpairs_per_s": round(n * npx / (ms_l / 1e3) / 1e9, 1),
+            "p50_ms_per_hologram": _pctl(holo_ms[diff], 50), "p99_ms_per_hologram": _pctl(holo_ms[diff], 99),
+            "holograms_timed": len(holo_ms[diff]),
+            "rel_l2_vs_fp64": rel[MODE_NAMES[diff]], "pairs_in_box_fraction": in_box[MODE_NAMES[diff]]}
+    res = {"workload": w.name, "points": n, "resolution": list(w.pn), "channels": nch,
+           "modes": [MODE_NAMES[m] for m in w.modes], "steps": steps, "warmup": warmup,
+           "ms_per_step": round(ms_step, 3), "Gpairs_per_s": round(pairs_step / (ms_step / 1e3) / 1e9, 1),
+           "per_mode": per_mode, "encode_normalize_ms_per_hologram": round(enc_ms, 4),
+           "probe_pixels": int(len(pix)),
+           "accuracy_channels": list(range(nch)) if accuracy_channels is None else list(accuracy_channels)}
+    bufs = {"dv": dv, "fields": fields, "encs": encs, "u8s": u8s, "params": params, "pts": pts, "n": n,
+            "launch_ms": launch_ms, "enc_ms": enc_ms, "pairs_step": pairs_step}
+    return res, bufs
+
+
+def free_bufs(bufs):
+    for b in [bufs["dv"]] + bufs["fields"] + bufs["encs"] + bufs["u8s"]:
+        b.free()
+
+
+def direct_roofline(w, n, diff, launch_ms):
+    """Roofline of an FP32 direct kernel: FP32 FLOPs of the pair loop against the
+    published FP32 vector peak, and against the loop's own speed of light."""
+    k_ms = float(np.mean(launch_ms))
+    pps = n * w.n_pixels / (k_ms / 1e3)
+    fl = FLOP_PER_PAIR[diff]
+    achieved = pps * fl / 1e12
+    return {"bound": "valu", "achieved": round(achieved, 2), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+            "kernel": f"k_pc_direct<{MODE_NAMES[diff]},FP32>", "avg_launch_ms": round(k_ms, 3),
+            "model": f"{fl} FP32 FLOP/pair in the separable pair loop (FMA=2); the per-lane column factor "
+                     f"(v_sin/v_cos once per point, shared by 8 rows) is not counted",
+            "mix_sol_pairs_per_s": MIX_SOL_PAIRS_PER_S[diff],
+            "frac_of_mix_sol": round(pps / MIX_SOL_PAIRS_PER_S[diff], 4)}
+
+
+def host_handover(ctx, capi, w, bufs, kernel_of):
+    """PCIe-inclusive rate of the drop-in (ophPointCloud::generateHologram):
+    vertices from the host, every channel's FP64 field handed back into
+    page-locked complex_H planes, each channel's copy ordered after its own
+    launch so it overlaps the next channel; one step on the wall clock
+    (never the value)."""
+    n, npx, nch = bufs["n"], w.n_pixels, len(w.lambdas)
+    fields, params = bufs["fields"], bufs["params"]
+    host_bufs = [capi.HostBuffer(16 * npx) for _ in range(nch)]
+    host_fields = [b.array(np.complex128, npx) for b in host_bufs]
+    ctx.sync()
+    t0 = time.perf_counter()
+    dv_h = ctx.to_device(bufs["pts"])
+    for diff in w.modes:
+        for ch in range(nch):
+            ctx.field_zero(fields[ch], npx)
+            ctx.diffract(dv_h, n, fields[ch], params[ch], diff, KERNELS[kernel_of[diff]])
+            fields[ch].download_async_after(host_fields[ch])
+        ctx.copy_sync()
+    ctx.sync()
+    host_ms = (time.perf_counter() - t0) * 1e3
+    dv_h.free()
+    t0 = time.perf_counter()
+    assert capi.lib().ohx_memcpy_d2h(ctx.h, host_fields[0].ctypes.data, fields[0].ptr, 16 * npx) == capi.OK
+    d2h_s = time.perf_counter() - t0
+    del host_fields
+    for b in host_bufs:
+        b.free()
+    out = {"ms_per_step": round(host_ms, 2), "Gpairs_per_s": round(bufs["pairs_step"] / (host_ms / 1e3) / 1e9, 1),
+           "field_d2h_GB_per_s": round(16 * npx / d2h_s / 1e9, 2),
+           "what": "vertex upload + per channel and mode: zero, diffract, FP64 field to a page-locked host "
+                   "plane (copy ordered after its own channel, overlapping the next); wall clock"}
+    print(f"# host hand-over: {host_ms:.1f} ms/step, field D2H {out['field_d2h_GB_per_s']} GB/s",
+          file=sys.stderr, flush=True)
+    return out
+
+
+def encode_block(ctx, w, bufs, Event):
+    """Encode + normalise (HBM-bound) and SSB encoding of the resident fields."""
+    npx, nch = w.n_pixels, len(w.lambdas)
+    enc_ms = bufs["enc_ms"]
+    # per pixel and channel: k_encode reads the FP64 field (16 B) and writes the
+    # encoded plane (8 B); normalisation reads it twice (min/max, then u8) and
+    # writes 1 B: 41 B algorithmic
+    enc_bytes = 41 * npx * nch
+    out = {"ms_per_hologram": round(enc_ms, 4), "bytes": enc_bytes,
+           "GB_per_s": round(enc_bytes / (enc_ms / 1e3) / 1e9, 1), "peak_GB_per_s": 8000.0,
+           "frac": round(enc_bytes / (enc_ms / 1e3) / 1e9 / 8000.0, 4),
+           "what": "k_encode<PHASE> per channel + k_minmax + k_to_u8 over all channels, HIP events"}
+    ssb_ev = []
+    for rep in range(4):
+        a, b = Event(), Event()
+        a.record(ctx.stream)
+        for ch in range(nch):
+            ctx.encode_ssb(bufs["fields"][ch], w.pn, w.pp, bufs["encs"][ch])
+        b.record(ctx.stream)
+        if rep:
+            ssb_ev.append((a, b))
+    ctx.sync()
+    ssb_ms = float(np.mean([a.elapsed_ms(b) for a, b in ssb_ev]))
+    ssb_bytes = 24 * npx * nch
+    out["ssb"] = {"ms_per_hologram": round(ssb_ms, 4), "bytes": ssb_bytes,
+                  "GB_per_s": round(ssb_bytes / (ssb_ms / 1e3) / 1e9, 1),
+                  "what": "k_encode_ssb per channel (encodeHologram, default spectrum shift), HIP events"}
+    return out
+
+
+def workload_block(ctx, capi, synth, Event, wid, kernel_of, steps, warmup, probe_pixels, extra=None,
+                   accuracy_channels=None):
+    """One of the other workloads, timed as the headline is."""
+    w = synth.WORKLOADS[wid]
+    res, bufs = run_workload(ctx, capi, synth, Event, w, kernel_of, steps, warmup, probe_pixels,
+                             label=f"[{w.name}] ", accuracy_channels=accuracy_channels)
+    if extra:
+        extra(res, bufs, w)
+    free_bufs(bufs)
+    return res
 
 
 def main():
@@ -293,209 +538,57 @@ def main():
     from openholo_amd.hip_events import Event
 
     w = synth.WORKLOADS[args.workload]
-    n = args.points or w.n_points
-    mode = KERNELS[args.kernel]
-    fmode = KERNELS[args.fresnel_kernel] if args.fresnel_kernel else mode
-    mode_of = {0: mode, 1: fmode}
-    npx = w.n_pixels
-    nch = len(w.lambdas)
-    pts = synth.points(w, n)
-
+    kernel_of = {0: args.kernel if args.kernel in ("direct", "fp64") else "direct",
+                 1: args.fresnel_kernel or args.kernel}
     ctx = capi.Context(0)
-    stream = ctx.stream
-    dv = ctx.to_device(pts)
-    fields = [ctx.alloc(16 * npx) for _ in range(nch)]
-    encs = [ctx.alloc(8 * npx) for _ in range(nch)]
-    u8s = [ctx.alloc(npx) for _ in range(nch)]
-    params = [capi.params(w.pn, w.pp, lam, channel=ch, use_phase=w.complex_amplitude)
-              for ch, lam in enumerate(w.lambdas)]
-
+    res, bufs = run_workload(ctx, capi, synth, Event, w, kernel_of, args.steps, args.warmup,
+                             args.probe_pixels, n=args.points or None)
+    n = bufs["n"]
     dom = 0 if 0 in w.modes else 1          # dominant kernel: RS when present
-    kev = {}                                # per-launch events of the dominant kernel
-    holo_ev = []
-    enc_ev = []                             # encode + normalise part of each hologram
-
-    def hologram(diff, timed):
-        e0 = Event()
-        e0.record(stream)
-        for ch in range(nch):
-            ctx.field_zero(fields[ch], npx)
-            if timed:
-                a, b = Event(), Event()
-                a.record(stream)
-                ctx.diffract(dv, n, fields[ch], params[ch], diff, mode_of[diff])
-                b.record(stream)
-                kev.setdefault(diff, []).append((a, b))
-            else:
-                ctx.diffract(dv, n, fields[ch], params[ch], diff, mode_of[diff])
-        em = Event()
-        em.record(stream)
-        for ch in range(nch):
-            ctx.encode(fields[ch], npx, encs[ch], capi.ENCODE_PHASE)
-        ctx.normalize_u8(encs, npx, u8s)
-        e1 = Event()
-        e1.record(stream)
-        if timed:
-            holo_ev.append((e0, e1))
-            enc_ev.append((em, e1))
-
-    def step(timed):
-        for diff in w.modes:
-            hologram(diff, timed)
-
-    for i in range(args.warmup):
-        step(False)
-        print(f"# warmup step {i + 1}/{args.warmup} done", file=sys.stderr, flush=True)
-    ctx.sync()
-    t_start, t_stop = Event(), Event()
-    wall0 = time.perf_counter()
-    t_start.record(stream)
-    for i in range(args.steps):
-        step(True)
-    t_stop.record(stream)
-    ctx.sync()
-    wall = time.perf_counter() - wall0
-    ms_total = t_start.elapsed_ms(t_stop)
-    ms_step = ms_total / args.steps
-    print(f"# timed {args.steps} steps: {ms_step:.1f} ms/step (wall {wall:.2f} s)", file=sys.stderr, flush=True)
-
-    pairs_step = n * npx * nch * len(w.modes)
-    value = pairs_step / (ms_step / 1e3) / 1e9
-
-    # dominant kernel: average launch time from events on the launch stream
-    launches = kev.get(dom, [])
-    k_ms = float(np.mean([a.elapsed_ms(b) for a, b in launches])) if launches else float("nan")
-    pairs_launch = n * npx
-    if args.kernel == "direct":
-        fl = FLOP_PER_PAIR[dom]
-        achieved = pairs_launch / (k_ms / 1e3) * fl / 1e12
-        pps = pairs_launch / (k_ms / 1e3)
-        traffic = TRAFFIC.get(dom) if (args.workload == 2 and n == w.n_points) else None
-        roofline = {"bound": "valu", "achieved": round(achieved, 2), "peak": PEAK_FP32_TFLOPS,
-                    "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": traffic,
-                    "kernel": f"k_pc_direct<{'RS' if dom == 0 else 'Fresnel'},FP32>",
-                    "avg_launch_ms": round(k_ms, 3),
-                    "model": f"{fl} FP32 FLOP/pair in the separable pair loop (FMA=2); the per-lane "
-                             f"column factor (v_sin/v_cos once per point, shared by 8 rows) is not "
-                             f"counted; packed FP32 is not emitted (a VGPR-operand FMA stream "
-                             f"reaches 104 TFLOP/s on this part)",
-                    "mix_sol_pairs_per_s": MIX_SOL_PAIRS_PER_S[dom],
-                    "frac_of_mix_sol": round(pps / MIX_SOL_PAIRS_PER_S[dom], 4)}
-    elif args.kernel in MFMA_TERMS and dom == 1:
-        achieved = pairs_launch / (k_ms / 1e3) * 8 * MFMA_TERMS[args.kernel] / 1e12
+    launch_ms = bufs["launch_ms"][dom]
+    kname = kernel_of[dom]
+    if kname in MFMA_TERMS:
+        k_ms = float(np.mean(launch_ms))
+        achieved = n * w.n_pixels / (k_ms / 1e3) * 8 * MFMA_TERMS[kname] / 1e12
         roofline = {"bound": "mfma", "achieved": round(achieved, 2), "peak": PEAK_BF16_TFLOPS,
-                    "unit": "TFLOP/s", "frac": round(achieved / PEAK_BF16_TFLOPS, 4), "traffic": None,
-                    "kernel": f"k_fresnel_gemm<{args.kernel}>", "avg_launch_ms": round(k_ms, 3),
-                    "model": f"8 FLOP per complex MAC x {MFMA_TERMS[args.kernel]} term(s)"}
-    elif args.kernel == "fp64":
+                    "unit": "TFLOP/s", "frac": round(achieved / PEAK_BF16_TFLOPS, 4),
+                    "traffic": TRAFFIC.get(kname) if (args.workload == 3 and n == w.n_points) else None,
+                    "kernel": f"k_fresnel_gemm<{kname}>", "avg_launch_ms": round(k_ms, 3),
+                    "model": f"8 FLOP per complex MAC x {MFMA_TERMS[kname]} term(s); the events bracket "
+                             f"k_pc_prep + k_fresnel_gemm + k_gemm_reduce of each launch"}
+    elif kname == "direct":
+        roofline = direct_roofline(w, n, dom, launch_ms)
+        roofline["traffic"] = None
+    else:
+        k_ms = float(np.mean(launch_ms))
         fl = FLOP_PER_PAIR_FP64[dom]
-        achieved = pairs_launch / (k_ms / 1e3) * fl / 1e12
+        achieved = n * w.n_pixels / (k_ms / 1e3) * fl / 1e12
         roofline = {"bound": "valu", "achieved": round(achieved, 2), "peak": PEAK_FP64_TFLOPS,
                     "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": None,
-                    "kernel": f"k_pc_direct64<{'RS' if dom == 0 else 'Fresnel'}>",
-                    "avg_launch_ms": round(k_ms, 3),
-                    "model": f"{fl} FP64 FLOP/pair in the pair loop (FMA=2); the per-lane column "
-                             f"factor (one FP64 sincospi per point, shared by 8 rows) is not counted"}
-    else:
-        roofline = {"bound": "valu", "achieved": None, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                    "frac": None, "traffic": None, "avg_launch_ms": round(k_ms, 3)}
+                    "kernel": f"k_pc_direct64<{MODE_NAMES[dom]}>", "avg_launch_ms": round(k_ms, 3),
+                    "model": f"{fl} FP64 FLOP/pair in the pair loop (FMA=2); the per-lane column factor "
+                             f"(one FP64 sincospi per point, shared by 8 rows) is not counted"}
+    encode_hbm = encode_block(ctx, w, bufs, Event)
+    handover = host_handover(ctx, capi, w, bufs, kernel_of)
+    free_bufs(bufs)
 
-    per_mode_ms = {("RS" if d == 0 else "Fresnel"): round(float(np.mean([a.elapsed_ms(b) for a, b in ev])), 3)
-                   for d, ev in kev.items()}
-
-    # encode + normalise (HBM-bound): per pixel and channel k_encode reads the
-    # FP64 field (16 B) and writes the encoded plane (8 B); normalisation reads
-    # it twice (min/max, then u8) and writes 1 B: 41 B algorithmic
-    enc_ms = float(np.mean([a.elapsed_ms(b) for a, b in enc_ev])) if enc_ev else float("nan")
-    enc_bytes = 41 * npx * nch
-    encode_hbm = {"ms_per_hologram": round(enc_ms, 4), "bytes": enc_bytes,
-                  "GB_per_s": round(enc_bytes / (enc_ms / 1e3) / 1e9, 1), "peak_GB_per_s": 8000.0,
-                  "frac": round(enc_bytes / (enc_ms / 1e3) / 1e9 / 8000.0, 4),
-                  "what": "k_encode<PHASE> per channel + k_minmax + k_to_u8 over all channels, HIP events"}
-
-    # single-side-band encoding (ophPointCloud::encodeHologram) of the same
-    # fields: 16 B read + 8 B written per pixel and channel
-    ssb_ev = []
-    for rep in range(4):
-        a, b = Event(), Event()
-        a.record(stream)
-        for ch in range(nch):
-            ctx.encode_ssb(fields[ch], w.pn, w.pp, encs[ch])
-        b.record(stream)
-        if rep:
-            ssb_ev.append((a, b))
-    ctx.sync()
-    ssb_ms = float(np.mean([a.elapsed_ms(b) for a, b in ssb_ev]))
-    ssb_bytes = 24 * npx * nch
-    encode_hbm["ssb"] = {"ms_per_hologram": round(ssb_ms, 4), "bytes": ssb_bytes,
-                         "GB_per_s": round(ssb_bytes / (ssb_ms / 1e3) / 1e9, 1),
-                         "what": "k_encode_ssb per channel (encodeHologram, default spectrum shift), HIP events"}
-
-    # p50 / p99 per hologram (one mode, all channels, encode + normalise)
-    hol = sorted(a.elapsed_ms(b) for a, b in holo_ev)
-    p50 = float(np.percentile(hol, 50)) if hol else None
-    p99 = float(np.percentile(hol, 99)) if hol else None
-
-    # accuracy: FP32 field vs the library's FP64 probe on sampled pixels
-    rng = np.random.default_rng(1234)
-    pix = rng.choice(npx, min(args.probe_pixels, npx), replace=False)
-    rel, in_box = {}, {}
-    for diff in w.modes:
-        num = den = 0.0
-        boxed = nominal = 0
-        for ch in range(nch):
-            ctx.field_zero(fields[ch], npx)
-            ctx.diffract(dv, n, fields[ch], params[ch], diff, mode_of[diff])
-            st = ctx.last_stats()
-            boxed += st.pairs_in_box
-            nominal += st.pairs_nominal
-            f = fields[ch].download(np.complex128, npx)[pix]
-            ref = ctx.probe_f64(dv, n, params[ch], diff, pix)
-            num += float(np.sum(np.abs(f - ref) ** 2))
-            den += float(np.sum(np.abs(ref) ** 2))
-        rel["RS" if diff == 0 else "Fresnel"] = float(np.sqrt(num / den))
-        in_box[("RS" if diff == 0 else "Fresnel")] = round(boxed / nominal, 6) if nominal else None
-
-    # PCIe-inclusive rate of the drop-in: ophPointCloud::generateHologram takes the
-    # vertices from the host and hands every channel's FP64 field back in its
-    # complex_H planes (page-locked host memory); one step timed end to end on the
-    # wall clock (never the value).
-    host_bufs = [capi.HostBuffer(16 * npx) for _ in range(nch)]
-    host_fields = [b.array(np.complex128, npx) for b in host_bufs]
-    lib = capi.lib()
-    ctx.sync()
-    t0 = time.perf_counter()
-    dv_h = ctx.to_device(pts)
-    for diff in w.modes:
-        # as the drop-in does: channel ch is read back while channel ch + 1 computes
-        def launch(ch):
-            ctx.field_zero(fields[ch], npx)
-            ctx.diffract(dv_h, n, fields[ch], params[ch], diff, mode_of[diff])
-        launch(0)
-        for ch in range(nch):
-            if ch + 1 < nch:
-                launch(ch + 1)
-            fields[ch].download_after(host_fields[ch])
-    ctx.sync()
-    host_ms = (time.perf_counter() - t0) * 1e3
-    dv_h.free()
-    t0 = time.perf_counter()
-    assert lib.ohx_memcpy_d2h(ctx.h, host_fields[0].ctypes.data, fields[0].ptr, 16 * npx) == capi.OK
-    d2h_s = time.perf_counter() - t0
-    del host_fields
-    for b in host_bufs:
-        b.free()
-    handover = {"ms_per_step": round(host_ms, 2), "Gpairs_per_s": round(pairs_step / (host_ms / 1e3) / 1e9, 1),
-                "field_d2h_GB_per_s": round(16 * npx / d2h_s / 1e9, 2),
-                "what": "vertex upload + per channel and mode: zero, diffract, FP64 field to a page-locked "
-                        "host plane overlapped with the next channel (the drop-in's generateHologram); "
-                        "wall clock"}
-    print(f"# host hand-over: {host_ms:.1f} ms/step, field D2H {handover['field_d2h_GB_per_s']} GB/s",
-          file=sys.stderr, flush=True)
-
-    for b in [dv] + fields + encs + u8s:
-        b.free()
     var = fresnel_variants(ctx, capi, synth, Event) if args.variants else None
+    blocks = {}
+    wanted = [b for b in args.blocks.split(",") if b]
+    if "rgb2160" in wanted:
+        def rs_roofline(r, b, wl):
+            r["roofline_rs"] = direct_roofline(wl, b["n"], 0, b["launch_ms"][0])
+        blocks["rgb_2160p"] = workload_block(ctx, capi, synth, Event, 2, {0: "direct", 1: "direct"}, 3, 1,
+                                             args.probe_pixels, extra=rs_roofline)
+    if "rs1080" in wanted:
+        blocks["rs_1080p"] = workload_block(ctx, capi, synth, Event, 1, {0: "direct"}, 20, 3, args.probe_pixels)
+        blocks["rs_1080p_fp64"] = workload_block(ctx, capi, synth, Event, 1, {0: "fp64"}, 5, 1,
+                                                 args.probe_pixels)
+    if "large4320" in wanted:
+        # one step (3 channels x RS + Fresnel, ~40 s); accuracy and box share on
+        # channel 0 only, to keep the run within minutes
+        blocks["large_4320p"] = workload_block(ctx, capi, synth, Event, 5, {0: "direct", 1: "fp16x3"}, 1, 0, 256,
+                                               accuracy_channels=[0])
     wrp = wrp_block(ctx, capi, synth, Event) if args.wrp else None
     if wrp:
         print(f"# wrp: {wrp}", file=sys.stderr, flush=True)
@@ -511,31 +604,38 @@ def main():
         cpu = cpu_baseline(w, args.cpu_seconds)
         print(f"# cpu baseline: {cpu['value']:.4f} Gpairs/s on {cpu['cores']} threads", file=sys.stderr, flush=True)
 
+    pm = res["per_mode"]
     line = {
         "metric": "point-pixel pairs/s (G/s) + field rel-L2 vs FP64; p50/p99 ms per hologram",
-        "value": round(value, 3), "unit": "Gpairs/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(ms_step, 3), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64" if args.kernel == "fp64" else "f32",
+        "value": res["Gpairs_per_s"], "unit": "Gpairs/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": res["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if kernel_of[dom] == "fp64" else ("bf16" if kernel_of[dom].startswith("bf16") else
+                                                          "f16" if kernel_of[dom] == "fp16x3" else "f32"),
         "data": "synthetic (SplitMix64 point cloud)",
         "config": {"workload": w.name, "points": n, "resolution": list(w.pn), "pitch_m": list(w.pp),
-                   "lambdas_m": list(w.lambdas), "modes": ["RS" if m == 0 else "Fresnel" for m in w.modes],
-                   "kernel": args.kernel, "fresnel_kernel": args.fresnel_kernel or args.kernel,
-                   "parallelism": "single GPU"},
-        "field_rel_l2_vs_fp64": {k: float(f"{v:.3e}") for k, v in rel.items()},
-        # fraction of the nominal pairs inside the points' anti-aliasing boxes
-        # (the pairs the kernels evaluate; value counts all nominal pairs)
-        "pairs_in_box_fraction": in_box,
-        "kernel_ms_per_channel": per_mode_ms,
-        "probe_pixels": int(len(pix)),
-        "p50_ms_per_hologram": round(p50, 3) if p50 else None,
-        "p99_ms_per_hologram": round(p99, 3) if p99 else None,
+                   "lambdas_m": list(w.lambdas), "modes": res["modes"],
+                   "complex_amplitude": w.complex_amplitude,
+                   "kernel": {m: v["kernel"] for m, v in pm.items()}, "parallelism": "single GPU"},
+        "field_rel_l2_vs_fp64": {m: v["rel_l2_vs_fp64"] for m, v in pm.items()},
+        "pairs_in_box_fraction": {m: v["pairs_in_box_fraction"] for m, v in pm.items()},
+        "kernel_ms_per_channel": {m: v["ms_per_channel"] for m, v in pm.items()},
+        "latency_ms_per_hologram": {m: {"p50": v["p50_ms_per_hologram"], "p99": v["p99_ms_per_hologram"],
+                                        "holograms": v["holograms_timed"]} for m, v in pm.items()},
+        "probe_pixels": res["probe_pixels"],
         "roofline": roofline,
         "cpu_baseline": cpu,
         "host_handover": handover,
         "encode_hbm": encode_hbm,
     }
+    if len(pm) == 1:
+        only = next(iter(pm.values()))
+        line["p50_ms_per_hologram"] = only["p50_ms_per_hologram"]
+        line["p99_ms_per_hologram"] = only["p99_ms_per_hologram"]
     if var:
         line["variants"] = var
+    if blocks:
+        line["workloads"] = blocks
     if wrp:
         line["wrp"] = wrp
     if serving:

@@ -78,7 +78,9 @@ template <bool FP16>
 __device__ inline float lo16(unsigned w)
 {
     if constexpr (FP16) return (float)__builtin_bit_cast(_Float16, (unsigned short)(w & 0xffffu));
-    return __builtin_bit_cast(float, w << 16);
+    // the low half moved up by a byte permute: a shift would let the compiler
+    // re-derive it from a second conversion of the low operand alone
+    return __builtin_bit_cast(float, __builtin_amdgcn_perm(w, 0u, 0x05040c0cu));
 }
 template <bool FP16>
 __device__ inline float hi16(unsigned w)

@@ -140,6 +140,24 @@ int ohx_pc_diffract(ohx_ctx* ctx, const void* d_vertices, int64_t n_points,
                     double* d_field, const ohx_pc_params* params,
                     uint32_t diff_flag, uint32_t mode);
 
+/* Allocation-free variant of ohx_pc_diffract.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  It calls no hipMalloc, hipFree or allocating library routine:
+ * its scratch (the per-point constants and two flag words) is a static device
+ * array of about 32 MiB per GPU, held from library load until the process
+ * ends.  Same arguments and result as ohx_pc_diffract for the direct modes
+ * (MODE_DOUBLE, MODE_FLOAT): every tile's points run in one workgroup (no
+ * partial planes), so results agree with ohx_pc_diffract to FP64 rounding of
+ * the per-pixel sums.  OHX_ERR_INVALID_ARG when n_points exceeds
+ * OHX_NOALLOC_MAX_POINTS; OHX_ERR_UNSUPPORTED for the matrix-core modes.
+ * All calls on one GPU share its scratch: calls that overlap on one GPU
+ * (several contexts or host threads) race, so callers serialise them. */
+#define OHX_NOALLOC_MAX_POINTS 524288
+int ohx_pc_diffract_noalloc(ohx_ctx* ctx, const void* d_vertices, int64_t n_points,
+                            double* d_field, const ohx_pc_params* params,
+                            uint32_t diff_flag, uint32_t mode);
+
 /* d_dst += d_src over n_px complex FP64 pixels.
  * Replaces sum_Kernel (ophgen/src/ophPCKernel.cu:874-891). */
 int ohx_field_accumulate(ohx_ctx* ctx, double* d_dst, const double* d_src, int64_t n_px);

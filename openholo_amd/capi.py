@@ -26,7 +26,7 @@ ENCODE_PHASE, ENCODE_AMPLITUDE, ENCODE_REAL, ENCODE_IMAGINARY = 0, 1, 2, 3
 EXPORTS = [
     "ohx_ctx_create", "ohx_ctx_destroy", "ohx_ctx_stream", "ohx_ctx_device", "ohx_status_string",
     "ohx_last_error", "ohx_mem_info", "ohx_malloc", "ohx_free", "ohx_host_alloc", "ohx_host_free", "ohx_memcpy_h2d", "ohx_memcpy_d2h",
-    "ohx_memcpy_d2d", "ohx_memcpy_d2h_async_after", "ohx_copy_synchronize", "ohx_memcpy_d2h_after", "ohx_synchronize", "ohx_field_zero", "ohx_pc_diffract", "ohx_field_accumulate",
+    "ohx_memcpy_d2d", "ohx_memcpy_d2h_async_after", "ohx_copy_synchronize", "ohx_memcpy_d2h_after", "ohx_synchronize", "ohx_field_zero", "ohx_pc_diffract", "ohx_pc_diffract_noalloc", "ohx_field_accumulate",
     "ohx_pc_probe_f64", "ohx_sincos_rev", "ohx_wrp_generate", "ohx_encode", "ohx_encode_ssb", "ohx_normalize_u8", "ohx_last_launch_stats",
     "ohx_server_create", "ohx_server_submit", "ohx_server_wait", "ohx_server_cancel",
     "ohx_server_counters_get", "ohx_server_destroy",
@@ -153,6 +153,7 @@ def lib():
             "ohx_synchronize": ([vp], i32),
             "ohx_field_zero": ([vp, vp, i64], i32),
             "ohx_pc_diffract": ([vp, vp, i64, vp, ctypes.POINTER(PcParams), u32, u32], i32),
+            "ohx_pc_diffract_noalloc": ([vp, vp, i64, vp, ctypes.POINTER(PcParams), u32, u32], i32),
             "ohx_field_accumulate": ([vp, vp, vp, i64], i32),
             "ohx_pc_probe_f64": ([vp, vp, i64, ctypes.POINTER(PcParams), u32, vp, i64, vp], i32),
             "ohx_sincos_rev": ([vp, vp, i64, vp, vp], i32),

@@ -212,6 +212,7 @@ int ohx_ctx_create(int device, ohx_ctx** out)
         delete c;
         return OHX_ERR_DEVICE;
     }
+    noalloc_init(device);   // address of the allocation-free entry point's scratch
     *out = c;
     return OHX_OK;
 }
@@ -375,6 +376,7 @@ int ohx_pc_diffract(ohx_ctx* c, const void* d_vertices, int64_t n, double* d_fie
     if ((st = ensure_points(c, n, &soa))) return st;
     if ((st = launch_prep(c, d_vertices, g, soa, c->minmax + 4))) return st;
     c->boxsum_pending = true;
+    c->boxsum_src = c->minmax + 4;
     const uint32_t mfma = mode & (OHX_MODE_MFMA_BF16X1 | OHX_MODE_MFMA_BF16X3 | OHX_MODE_MFMA_FP16X3);
     if (mfma && diff == OHX_DIFF_FRESNEL) return launch_fresnel_gemm(c, g, soa, d_field, mode);
     const bool fp64 = (mode & (OHX_MODE_FLOAT | OHX_MODE_FASTMATH)) == 0 && mfma == 0;
@@ -438,7 +440,7 @@ int ohx_last_launch_stats(ohx_ctx* c, ohx_launch_stats* out)
     if (c->boxsum_pending) {   // the box-pixel count is read back on request only
         (void)hipSetDevice(c->device);
         unsigned long long v = 0;
-        int st = check_hip(c, hipMemcpyAsync(&v, c->minmax + 4, sizeof(v), hipMemcpyDeviceToHost, c->stream),
+        int st = check_hip(c, hipMemcpyAsync(&v, c->boxsum_src, sizeof(v), hipMemcpyDeviceToHost, c->stream),
                            "hipMemcpyAsync");
         if (!st) st = check_hip(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
         if (st) return st;

@@ -12,7 +12,7 @@
 struct ohx_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
-    // field read-back that overlaps later work on `stream` (ohx_memcpy_d2h_after)
+    // field read-back that overlaps later work on `stream` (ohx_memcpy_d2h_async_after)
     hipStream_t copy_stream = nullptr;
     hipEvent_t copy_ready = nullptr;
     std::mutex mu;                 // serialises use of the scratch buffers
@@ -34,7 +34,8 @@ struct ohx_ctx {
     void* wrp_plan_a = nullptr;
     int wrp_plan_L = 0, wrp_plan_M = 0;
     ohx_launch_stats last{};
-    bool boxsum_pending = false;   // last.pairs_in_box still on the device (minmax[4])
+    bool boxsum_pending = false;   // last.pairs_in_box still on the device (*boxsum_src)
+    const unsigned long long* boxsum_src = nullptr;   // minmax + 4, or the no-alloc scratch
 };
 
 namespace ohx {
@@ -101,6 +102,14 @@ int ensure_gemm_scratch(ohx_ctx* ctx, size_t bytes);
 int launch_prep(ohx_ctx* ctx, const void* d_vertices, const Geometry& g, const PointSoA& soa,
                 unsigned long long* boxsum = nullptr);
 int launch_direct(ohx_ctx* ctx, const Geometry& g, const PointSoA& soa, double* d_field, bool fp64);
+// flag: device word for the FP32 kernel's "EXACT pairs left" flag;
+// allow_splits = false runs every tile's points in one workgroup (no partial
+// planes, no occupancy query)
+int launch_direct_with(ohx_ctx* ctx, const Geometry& g, const PointSoA& soa, double* d_field, bool fp64,
+                       unsigned long long* flag, bool allow_splits);
+// allocation-free entry point's per-device scratch (noalloc.hip): called by
+// ohx_ctx_create with the device current
+void noalloc_init(int device);
 int launch_fresnel_gemm(ohx_ctx* ctx, const Geometry& g, const PointSoA& soa, double* d_field, uint32_t mode);
 int direct_tiles(const Geometry& g, bool fp64, int* tiles_x);
 int wrp_generate(ohx_ctx* ctx, const void* d_vertices, int64_t n, const ohx_wrp_params* p, double* d_field,

@@ -1095,6 +1095,12 @@ static int choose_splits(int64_t tiles, int64_t n, int slots, int64_t npx, int c
 
 int launch_direct(ohx_ctx* ctx, const Geometry& g, const PointSoA& soa, double* d_field, bool fp64)
 {
+    return launch_direct_with(ctx, g, soa, d_field, fp64, ctx->minmax + 5, true);
+}
+
+int launch_direct_with(ohx_ctx* ctx, const Geometry& g, const PointSoA& soa, double* d_field, bool fp64,
+                       unsigned long long* flag, bool allow_splits)
+{
     const int tw = fp64 ? TW64 : TW, th = fp64 ? TH64 : TH;
     const int tiles_x = (g.pn_x + tw - 1) / tw, tiles_y = (g.pn_y + th - 1) / th;
     const int64_t tiles = (int64_t)tiles_x * tiles_y;
@@ -1105,7 +1111,7 @@ int launch_direct(ohx_ctx* ctx, const Geometry& g, const PointSoA& soa, double* 
     const void* kern = fp64 ? (rs ? (const void*)k_pc_direct64<OHX_DIFF_RS> : (const void*)k_pc_direct64<OHX_DIFF_FRESNEL>)
                             : (rs ? (const void*)k_pc_direct<OHX_DIFF_RS> : (const void*)k_pc_direct<OHX_DIFF_FRESNEL>);
     const int chunk = fp64 ? PCD : PCH;
-    int splits = choose_splits(tiles, soa.n, resident_slots(ctx, kern), npx, chunk);
+    int splits = allow_splits ? choose_splits(tiles, soa.n, resident_slots(ctx, kern), npx, chunk) : 1;
     SplitArgs sa{(int)tiles, soa.n, nullptr};
     if (splits > 1) {
         // split boundaries on chunk boundaries: a split sums exactly as the
@@ -1121,7 +1127,6 @@ int launch_direct(ohx_ctx* ctx, const Geometry& g, const PointSoA& soa, double* 
     }
     const dim3 grid((unsigned)(tiles * splits)), block(NT);
     const dim3 tgrid((unsigned)tiles);
-    unsigned long long* flag = ctx->minmax + 5;   // EXACT pairs left by the FP32 kernel
     if (rs) {
         if (fp64) {
             hipLaunchKernelGGL((k_pc_direct64<OHX_DIFF_RS>), grid, block, 0, ctx->stream, g, soa, f, tiles_x, sa);

@@ -73,6 +73,9 @@ def parse():
                     help="also run the mixed Poisson request stream on the serving engine (0 = off)")
     ap.add_argument("--requests", type=int, default=120)
     ap.add_argument("--wrp", type=int, default=1, help="also time the WRP generator (0 = off)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run ohx_field_accumulate over two 3840x2160 FP64 fields (reads 265.4 MB, "
+                         "writes 132.7 MB), the known-size kernel that scales FETCH_SIZE / WRITE_SIZE")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--probe-pixels", type=int, default=512)
@@ -138,7 +141,11 @@ def cpu_baseline(w, target_s: float):
 # FETCH_SIZE and WRITE_SIZE summed over the launch's dispatches, each scaled by
 # the factor the same counter reads on a copy kernel of known size;
 # profiles/traffic_1m_2160p.md gives the counts, the factors and the commit.
-TRAFFIC = {}
+# k_fresnel_gemm<BF16x3> alone: 2.08 GB read (the split-K planes are written,
+# point records re-read per tile mostly hit L2) + 1.062 GB written; the whole
+# launch (prep, constants, GEMM, split-K reduce): 4.61 GB.
+TRAFFIC = {"bf16x3": 3.14e9}
+TRAFFIC_LAUNCH = {"bf16x3": 4.61e9}
 # Speed of light of the direct kernels' separable pair loops (per lane and point,
 # 8 rows: column factor + 8 pairs) on this
 # hardware (openholo_amd/bin/valu_rates); profiles/valu_mfma_rates_gfx950.md.
@@ -355,9 +362,7 @@ def run_workload(ctx, capi, synth, Event, w, kernel_of, steps, warmup, probe_pix
                 kev.setdefault(diff, []).append((a, b))
         em = Event()
         em.record(stream)
-        for ch in range(nch):
-            ctx.encode(fields[ch], npx, encs[ch], capi.ENCODE_PHASE)
-        ctx.normalize_u8(encs, npx, u8s)
+        ctx.encode_normalize_u8(fields, npx, capi.ENCODE_PHASE, encs, u8s)
         e1 = Event()
         e1.record(stream)
         if timed:
@@ -489,14 +494,30 @@ def encode_block(ctx, w, bufs, Event):
     """Encode + normalise (HBM-bound) and SSB encoding of the resident fields."""
     npx, nch = w.n_pixels, len(w.lambdas)
     enc_ms = bufs["enc_ms"]
-    # per pixel and channel: k_encode reads the FP64 field (16 B) and writes the
-    # encoded plane (8 B); normalisation reads it twice (min/max, then u8) and
-    # writes 1 B: 41 B algorithmic
-    enc_bytes = 41 * npx * nch
+    # per pixel and channel: the encode pass reads the FP64 field (16 B), writes
+    # the encoded plane (8 B) and takes the min/max on the way; the 8-bit pass
+    # reads the encoded plane (8 B) and writes 1 B: 33 B algorithmic
+    enc_bytes = 33 * npx * nch
     out = {"ms_per_hologram": round(enc_ms, 4), "bytes": enc_bytes,
            "GB_per_s": round(enc_bytes / (enc_ms / 1e3) / 1e9, 1), "peak_GB_per_s": 8000.0,
            "frac": round(enc_bytes / (enc_ms / 1e3) / 1e9 / 8000.0, 4),
-           "what": "k_encode<PHASE> per channel + k_minmax + k_to_u8 over all channels, HIP events"}
+           "what": "ohx_encode_normalize_u8 (k_encode_mm<PHASE> over all channels + k_to_u8), HIP events"}
+    # the same work as two calls (ohx_encode per channel, then ohx_normalize_u8:
+    # a separate min/max pass, 41 B)
+    two = []
+    for rep in range(4):
+        a, b = Event(), Event()
+        a.record(ctx.stream)
+        for ch in range(nch):
+            ctx.encode(bufs["fields"][ch], npx, bufs["encs"][ch], 0)
+        ctx.normalize_u8(bufs["encs"], npx, bufs["u8s"])
+        b.record(ctx.stream)
+        if rep:
+            two.append((a, b))
+    ctx.sync()
+    two_ms = float(np.mean([a.elapsed_ms(b) for a, b in two]))
+    out["two_calls"] = {"ms_per_hologram": round(two_ms, 4),
+                        "GB_per_s_of_41_B": round(41 * npx * nch / (two_ms / 1e3) / 1e9, 1)}
     ssb_ev = []
     for rep in range(4):
         a, b = Event(), Event()
@@ -541,6 +562,14 @@ def main():
     kernel_of = {0: args.kernel if args.kernel in ("direct", "fp64") else "direct",
                  1: args.fresnel_kernel or args.kernel}
     ctx = capi.Context(0)
+    if args.copy_calibration:
+        npx_c = 3840 * 2160
+        a, b = ctx.alloc(16 * npx_c), ctx.alloc(16 * npx_c)
+        for _ in range(3):
+            ctx.accumulate(a, b, npx_c)      # k_field_acc: 32 B read + 16 B written per pixel
+        ctx.sync()
+        a.free()
+        b.free()
     res, bufs = run_workload(ctx, capi, synth, Event, w, kernel_of, args.steps, args.warmup,
                              args.probe_pixels, n=args.points or None)
     n = bufs["n"]
@@ -553,6 +582,9 @@ def main():
         roofline = {"bound": "mfma", "achieved": round(achieved, 2), "peak": PEAK_BF16_TFLOPS,
                     "unit": "TFLOP/s", "frac": round(achieved / PEAK_BF16_TFLOPS, 4),
                     "traffic": TRAFFIC.get(kname) if (args.workload == 3 and n == w.n_points) else None,
+                    "traffic_launch": TRAFFIC_LAUNCH.get(kname) if (args.workload == 3 and n == w.n_points) else None,
+                    "traffic_source": "rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (factors from k_field_acc "
+                                      "of known size), profiles/traffic_1m_2160p.md",
                     "kernel": f"k_fresnel_gemm<{kname}>", "avg_launch_ms": round(k_ms, 3),
                     "model": f"8 FLOP per complex MAC x {MFMA_TERMS[kname]} term(s); the events bracket "
                              f"k_pc_prep + k_fresnel_gemm + k_gemm_reduce of each launch"}

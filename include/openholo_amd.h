@@ -233,6 +233,16 @@ int ohx_encode_ssb(ohx_ctx* ctx, const double* d_field, int32_t pn_x, int32_t pn
 int ohx_normalize_u8(ohx_ctx* ctx, const double* const* d_enc, int n_channels,
                      int64_t n_px, uint8_t* const* d_out);
 
+/* ophGen::encoding(encode_flag) of every channel followed by ophGen::normalize
+ * (ophgen/src/ophGen.cpp:816-849, then :654-687) in one call: the encode pass
+ * also takes the global min/max, so the encoded planes are read once (for the
+ * 8-bit pass) instead of twice.  d_enc and d_out receive exactly what
+ * ohx_encode per channel followed by ohx_normalize_u8 writes.  d_field, d_enc,
+ * d_out: host arrays of n_channels device pointers. */
+int ohx_encode_normalize_u8(ohx_ctx* ctx, const double* const* d_field, int n_channels,
+                            int64_t n_px, uint32_t encode_flag, double* const* d_enc,
+                            uint8_t* const* d_out);
+
 /* Shape of the last ohx_pc_diffract on ctx (for throughput accounting). */
 typedef struct ohx_launch_stats {
     int64_t pairs_nominal;   /* n_points * pn_x * pn_y                        */

@@ -27,7 +27,7 @@ EXPORTS = [
     "ohx_ctx_create", "ohx_ctx_destroy", "ohx_ctx_stream", "ohx_ctx_device", "ohx_status_string",
     "ohx_last_error", "ohx_mem_info", "ohx_malloc", "ohx_free", "ohx_host_alloc", "ohx_host_free", "ohx_memcpy_h2d", "ohx_memcpy_d2h",
     "ohx_memcpy_d2d", "ohx_memcpy_d2h_async_after", "ohx_copy_synchronize", "ohx_memcpy_d2h_after", "ohx_synchronize", "ohx_field_zero", "ohx_pc_diffract", "ohx_pc_diffract_noalloc", "ohx_field_accumulate",
-    "ohx_pc_probe_f64", "ohx_sincos_rev", "ohx_wrp_generate", "ohx_encode", "ohx_encode_ssb", "ohx_normalize_u8", "ohx_last_launch_stats",
+    "ohx_pc_probe_f64", "ohx_sincos_rev", "ohx_wrp_generate", "ohx_encode", "ohx_encode_ssb", "ohx_normalize_u8", "ohx_encode_normalize_u8", "ohx_last_launch_stats",
     "ohx_server_create", "ohx_server_submit", "ohx_server_wait", "ohx_server_cancel",
     "ohx_server_counters_get", "ohx_server_destroy",
 ]
@@ -162,6 +162,8 @@ def lib():
             "ohx_encode_ssb": ([vp, vp, i32, i32, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                 ctypes.c_double, vp], i32),
             "ohx_normalize_u8": ([vp, ctypes.POINTER(vp), i32, i64, ctypes.POINTER(vp)], i32),
+            "ohx_encode_normalize_u8": ([vp, ctypes.POINTER(vp), i32, i64, u32, ctypes.POINTER(vp),
+                                         ctypes.POINTER(vp)], i32),
             "ohx_last_launch_stats": ([vp, ctypes.POINTER(LaunchStats)], i32),
             "ohx_server_create": ([ctypes.POINTER(ServerOpts), ctypes.POINTER(vp)], i32),
             "ohx_server_submit": ([vp, ctypes.POINTER(Job), ctypes.POINTER(ctypes.c_uint64)], i32),
@@ -336,6 +338,14 @@ class Context:
         a = (ctypes.c_void_p * len(encs))(*[e.ptr for e in encs])
         b = (ctypes.c_void_p * len(outs))(*[o.ptr for o in outs])
         self._check(lib().ohx_normalize_u8(self.h, a, len(encs), int(n_px), b), "ohx_normalize_u8")
+
+    def encode_normalize_u8(self, fields, n_px: int, flag: int, encs, outs):
+        """ohx_encode_normalize_u8: encode every channel and normalise to 8 bits."""
+        f = (ctypes.c_void_p * len(fields))(*[x.ptr for x in fields])
+        e = (ctypes.c_void_p * len(encs))(*[x.ptr for x in encs])
+        o = (ctypes.c_void_p * len(outs))(*[x.ptr for x in outs])
+        self._check(lib().ohx_encode_normalize_u8(self.h, f, len(fields), int(n_px), int(flag), e, o),
+                    "ohx_encode_normalize_u8")
 
     def last_stats(self) -> LaunchStats:
         s = LaunchStats()

@@ -35,21 +35,22 @@ __device__ inline double key_value(unsigned long long k)
 }
 
 template <int FLAG>
-__global__ __launch_bounds__(EB) void k_encode(const double2* __restrict__ f, int64_t n, double* __restrict__ e)
+__device__ inline double encode_value(double2 h)
 {
     const double pi = 3.14159265358979323846;
     const double pi2 = pi * 2;
+    if (FLAG == OHX_ENCODE_PHASE) return (atan2(h.y, h.x) + pi) / pi2;
+    if (FLAG == OHX_ENCODE_AMPLITUDE) return sqrt(h.x * h.x + h.y * h.y);
+    if (FLAG == OHX_ENCODE_REAL) return h.x;
+    return h.y;
+}
+
+template <int FLAG>
+__global__ __launch_bounds__(EB) void k_encode(const double2* __restrict__ f, int64_t n, double* __restrict__ e)
+{
     int64_t i = (int64_t)blockIdx.x * EB + threadIdx.x;
     const int64_t stride = (int64_t)gridDim.x * EB;
-    for (; i < n; i += stride) {
-        const double2 h = f[i];
-        double v;
-        if (FLAG == OHX_ENCODE_PHASE) v = (atan2(h.y, h.x) + pi) / pi2;
-        else if (FLAG == OHX_ENCODE_AMPLITUDE) v = sqrt(h.x * h.x + h.y * h.y);
-        else if (FLAG == OHX_ENCODE_REAL) v = h.x;
-        else v = h.y;
-        e[i] = v;
-    }
+    for (; i < n; i += stride) e[i] = encode_value<FLAG>(f[i]);
 }
 
 // Single-side-band encoding (ophPointCloud::encodeHologram,
@@ -134,10 +135,41 @@ __device__ inline void minmax_upd(double v, unsigned long long& kmin, unsigned l
     }
 }
 
+// Block-wide min/max of the order keys (wave shuffles, then LDS), merged into
+// mm[0] / mm[1] with one 64-bit atomic each per block.
+__device__ inline void commit_minmax(unsigned long long kmin, unsigned long long kmax, unsigned long long* mm)
+{
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long a = __shfl_xor(kmin, off);
+        const unsigned long long b = __shfl_xor(kmax, off);
+        kmin = a < kmin ? a : kmin;
+        kmax = b > kmax ? b : kmax;
+    }
+    __shared__ unsigned long long smin[EB / 64], smax[EB / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) {
+        smin[wave] = kmin;
+        smax[wave] = kmax;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < EB / 64; ++w) {
+            kmin = smin[w] < kmin ? smin[w] : kmin;
+            kmax = smax[w] > kmax ? smax[w] : kmax;
+        }
+        if (kmin != ~0ull) atomicMin(&mm[0], kmin);
+        if (kmax != 0ull) atomicMax(&mm[1], kmax);
+    }
+}
+
 // Few blocks (MINMAX_BLOCKS) so that only a few hundred 64-bit atomics meet
 // on the two result words; each thread keeps four 16-byte loads in flight.
 // Channel buffers that are not 16-byte aligned take the scalar loop.
 constexpr int MINMAX_BLOCKS = 512;
+// fused encode + min/max: the FP64 atan2 per pixel needs the latency hiding of
+// k_encode's full grid (2048 blocks ran at half its rate); one atomic pair per
+// block, 16k per launch at most
+constexpr int ENCODE_MM_BLOCKS = 8192;
 
 __global__ __launch_bounds__(EB) void k_minmax(ChanPtrs cp, int nch, int64_t n, unsigned long long* mm)
 {
@@ -171,27 +203,43 @@ __global__ __launch_bounds__(EB) void k_minmax(ChanPtrs cp, int nch, int64_t n, 
         }
         for (int64_t i = done + tid; i < n; i += stride) minmax_upd(e[i], kmin, kmax);
     }
-    for (int off = 32; off > 0; off >>= 1) {
-        const unsigned long long a = __shfl_xor(kmin, off);
-        const unsigned long long b = __shfl_xor(kmax, off);
-        kmin = a < kmin ? a : kmin;
-        kmax = b > kmax ? b : kmax;
-    }
-    __shared__ unsigned long long smin[EB / 64], smax[EB / 64];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (lane == 0) {
-        smin[wave] = kmin;
-        smax[wave] = kmax;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int w = 1; w < EB / 64; ++w) {
-            kmin = smin[w] < kmin ? smin[w] : kmin;
-            kmax = smax[w] > kmax ? smax[w] : kmax;
+    commit_minmax(kmin, kmax, mm);
+}
+
+// Encoders of every channel that also take the global min/max of the encoded
+// values (the normaliser's first pass), so that ohx_encode_normalize_u8 reads
+// the encoded planes only once more, in k_to_u8.  Values and min/max equal
+// k_encode's and k_minmax's bit for bit.
+struct EncPtrs {
+    const double2* f[MAXCH];
+    double* e[MAXCH];
+};
+
+template <int FLAG>
+__global__ __launch_bounds__(EB) void k_encode_mm(EncPtrs ep, int nch, int64_t n, unsigned long long* mm)
+{
+    unsigned long long kmin = ~0ull, kmax = 0ull;
+    const int64_t tid = (int64_t)blockIdx.x * EB + threadIdx.x;
+    const int64_t stride = (int64_t)gridDim.x * EB;
+    for (int c = 0; c < nch; ++c) {
+        const double2* __restrict__ f = ep.f[c];
+        double* __restrict__ e = ep.e[c];
+        int64_t i = tid;
+        for (; i + stride < n; i += 2 * stride) {
+            const double2 h0 = f[i], h1 = f[i + stride];
+            const double v0 = encode_value<FLAG>(h0), v1 = encode_value<FLAG>(h1);
+            e[i] = v0;
+            e[i + stride] = v1;
+            minmax_upd(v0, kmin, kmax);
+            minmax_upd(v1, kmin, kmax);
         }
-        if (kmin != ~0ull) atomicMin(&mm[0], kmin);
-        if (kmax != 0ull) atomicMax(&mm[1], kmax);
+        if (i < n) {
+            const double v = encode_value<FLAG>(f[i]);
+            e[i] = v;
+            minmax_upd(v, kmin, kmax);
+        }
     }
+    commit_minmax(kmin, kmax, mm);
 }
 
 __device__ inline uint8_t to_u8(double v, double mn, double gap)
@@ -306,4 +354,46 @@ extern "C" int ohx_normalize_u8(ohx_ctx* c, const double* const* d_enc, int nch,
     hipLaunchKernelGGL(k_to_u8, dim3(egrid((n + 3) / 4)), dim3(EB), 0, c->stream, cp, nch, n,
                        (const unsigned long long*)c->minmax);
     return check_hip(c, hipGetLastError(), "k_normalize");
+}
+
+extern "C" int ohx_encode_normalize_u8(ohx_ctx* c, const double* const* d_field, int nch, int64_t n,
+                                       uint32_t flag, double* const* d_enc, uint8_t* const* d_out)
+{
+    if (!c) return OHX_ERR_INVALID_ARG;
+    if (nch < 1 || nch > MAXCH || n < 0 || !d_field || !d_enc || !d_out)
+        return set_error(c, OHX_ERR_INVALID_ARG, "bad channel count or buffers");
+    if (flag > OHX_ENCODE_IMAGINARY)
+        return set_error(c, OHX_ERR_UNSUPPORTED, "encode flag not on the point-cloud path");
+    EncPtrs ep{};
+    ChanPtrs cp{};
+    for (int i = 0; i < nch; ++i) {
+        if (n > 0 && (!d_field[i] || !d_enc[i] || !d_out[i]))
+            return set_error(c, OHX_ERR_INVALID_ARG, "null channel buffer");
+        ep.f[i] = (const double2*)d_field[i];
+        ep.e[i] = d_enc[i];
+        cp.p[i] = d_enc[i];
+        cp.o[i] = d_out[i];
+    }
+    if (n == 0) return OHX_OK;
+    std::lock_guard<std::mutex> lk(c->mu);
+    (void)hipSetDevice(c->device);
+    hipLaunchKernelGGL(k_minmax_init, dim3(1), dim3(64), 0, c->stream, c->minmax);
+    const dim3 g(std::min<unsigned>(egrid(n), ENCODE_MM_BLOCKS)), b(EB);
+    switch (flag) {
+    case OHX_ENCODE_PHASE:
+        hipLaunchKernelGGL(k_encode_mm<OHX_ENCODE_PHASE>, g, b, 0, c->stream, ep, nch, n, c->minmax);
+        break;
+    case OHX_ENCODE_AMPLITUDE:
+        hipLaunchKernelGGL(k_encode_mm<OHX_ENCODE_AMPLITUDE>, g, b, 0, c->stream, ep, nch, n, c->minmax);
+        break;
+    case OHX_ENCODE_REAL:
+        hipLaunchKernelGGL(k_encode_mm<OHX_ENCODE_REAL>, g, b, 0, c->stream, ep, nch, n, c->minmax);
+        break;
+    default:
+        hipLaunchKernelGGL(k_encode_mm<OHX_ENCODE_IMAGINARY>, g, b, 0, c->stream, ep, nch, n, c->minmax);
+        break;
+    }
+    hipLaunchKernelGGL(k_to_u8, dim3(egrid((n + 3) / 4)), dim3(EB), 0, c->stream, cp, nch, n,
+                       (const unsigned long long*)c->minmax);
+    return check_hip(c, hipGetLastError(), "k_encode_normalize");
 }

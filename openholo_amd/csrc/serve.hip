@@ -172,10 +172,10 @@ static int finish_job(ohx_server* sv, SJob& j)
     const int64_t npx = (int64_t)j.g.pn_x * j.g.pn_y;
     int st = OHX_OK;
     if (j.out_u8) {
-        st = ohx_encode(c, j.field, npx, j.enc, OHX_ENCODE_PHASE);
-        const double* e = j.enc;
+        const double* f = j.field;
+        double* e = j.enc;
         uint8_t* o = j.u8;
-        if (!st) st = ohx_normalize_u8(c, &e, 1, npx, &o);
+        st = ohx_encode_normalize_u8(c, &f, 1, npx, OHX_ENCODE_PHASE, &e, &o);
         if (!st) st = check_hip(c, hipMemcpyAsync(j.out_u8, j.u8, npx, hipMemcpyDeviceToHost, c->stream), "d2h u8");
     }
     if (!st && j.out_field)

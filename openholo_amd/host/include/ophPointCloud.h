@@ -44,7 +44,10 @@ public:
     Vertex* getPointCloudModel() { return pc_data_.vertices; }
     ulonglong getNumberOfPoints() { return pc_data_.n_points; }
 
-    bool isCPU() { return (m_mode & MODE_GPU) ? false : true; }
+    // Whether generation runs on the host CPU (ophgen/src/ophPointCloud.h:615-623
+    // reports the MODE_GPU bit).  This library has no CPU path: generateHologram
+    // runs on the GPU whatever the mode, so this is always false.
+    bool isCPU() { return false; }
     int loadPointCloud(const char* pc_file);
     bool readConfig(const char* cfg_file);
     Real generateHologram(uint diff_flag = PC_DIFF_RS);

@@ -285,6 +285,44 @@ def test_encode_and_normalize_match_reference(ctx, orc):
             assert (gu == wu).mean() > 0.9999
 
 
+@pytest.mark.parametrize("flag", [0, 1, 2, 3])
+@pytest.mark.parametrize("npx,nch", [(1, 1), (7, 2), (70_001, 3), (1 << 20, 3)])
+def test_fused_encode_normalize_equals_two_calls(ctx, orc, flag, npx, nch):
+    """ohx_encode_normalize_u8 (encode pass that also takes the min/max) writes
+    exactly what ohx_encode per channel followed by ohx_normalize_u8 writes:
+    encoded planes and 8-bit planes bit for bit, NaN pixels skipped by the
+    min/max as the reference's comparisons skip them; the 8-bit planes also
+    equal the oracle's normaliser on the same encoded planes."""
+    rng = np.random.default_rng(npx + 10 * flag)
+    fields = [(rng.normal(size=npx) + 1j * rng.normal(size=npx)) * (ch + 1) for ch in range(nch)]
+    if npx > 10:
+        fields[0][3] = complex(float("nan"), 1.0)
+    dfs = [ctx.to_device(f.astype(np.complex128)) for f in fields]
+    e1 = [ctx.alloc(8 * npx) for _ in fields]
+    u1 = [ctx.alloc(npx) for _ in fields]
+    e2 = [ctx.alloc(8 * npx) for _ in fields]
+    u2 = [ctx.alloc(npx) for _ in fields]
+    try:
+        for df, de in zip(dfs, e1):
+            ctx.encode(df, npx, de, flag)
+        ctx.normalize_u8(e1, npx, u1)
+        ctx.encode_normalize_u8(dfs, npx, flag, e2, u2)
+        for a, b in zip(e1, e2):
+            x, y = a.download(np.float64, npx), b.download(np.float64, npx)
+            assert np.array_equal(x.view(np.uint64), y.view(np.uint64))
+        got = [b.download(np.uint8, npx) for b in u2]
+        for a, g in zip(u1, got):
+            assert np.array_equal(a.download(np.uint8, npx), g)
+        encs = [b.download(np.float64, npx) for b in e2]
+        want = orc.normalize(encs)
+        for g, w_, e in zip(got, want, encs):   # the oracle's uint8 cast of NaN is undefined
+            ok = np.isfinite(e)
+            assert np.array_equal(g[ok], w_[ok])
+    finally:
+        for b in dfs + e1 + u1 + e2 + u2:
+            b.free()
+
+
 class _View:
     """A device pointer offset into a DeviceBuffer (for alignment cases)."""
     def __init__(self, buf, off):

@@ -285,13 +285,15 @@ def serving_stream(capi, synth, n_requests: int):
 
         th = threading.Thread(target=producer)
         th.start()
-        lat, first_submit, last_end = [], None, None
+        lat, first_submit, last_end, phases = [], None, None, []
         for i in range(len(reqs)):
             while ids[i] is None:
                 time.sleep(0.0005)
             st, stt, _, _ = srv.wait(ids[i])
             assert st == capi.OK
             lat.append(stt.end_ms - stt.submit_ms)
+            phases.append((stt.admit_ms - stt.submit_ms, stt.start_ms - stt.admit_ms, stt.end_ms - stt.start_ms,
+                           stt.iterations))
             first_submit = stt.submit_ms if first_submit is None else min(first_submit, stt.submit_ms)
             last_end = stt.end_ms if last_end is None else max(last_end, stt.end_ms)
         th.join()
@@ -307,8 +309,14 @@ def serving_stream(capi, synth, n_requests: int):
             per_mode[name] = {"requests": int(m.sum()), "p50_ms": _pctl(lat[m], 50), "p99_ms": _pctl(lat[m], 99),
                               "alone_p50_ms": _pctl(alone[m], 50), "alone_p99_ms": _pctl(alone[m], 99),
                               "slowdown_p50": _pctl(slow[m], 50), "slowdown_p99": _pctl(slow[m], 99)}
+    worst = [{"mode": "Fresnel" if reqs[i]["diff"] == 1 else "RS", "points": reqs[i]["n"],
+              "resolution": list(reqs[i]["pn"]), "alone_ms": round(float(alone[i]), 2),
+              "latency_ms": round(float(lat[i]), 2), "queued_ms": round(phases[i][0], 2),
+              "admitted_to_first_slice_ms": round(phases[i][1], 2), "slices_ms": round(phases[i][2], 2),
+              "iterations": int(phases[i][3])}
+             for i in np.argsort(-slow)[:3]]
     return {
-        "requests": len(reqs), "pairs": float(pairs),
+        "requests": len(reqs), "pairs": float(pairs), "worst_slowdown_jobs": worst,
         "burst_capacity_jobs_per_s": round(capacity, 2), "poisson_rate_jobs_per_s": round(rate, 2),
         "throughput_Gpairs_per_s": round(pairs / span_s / 1e9, 1), "burst_Gpairs_per_s": round(pairs / burst_s / 1e9, 1),
         "p50_ms": _pctl(lat, 50), "p99_ms": _pctl(lat, 99), "mean_ms": round(float(lat.mean()), 2),

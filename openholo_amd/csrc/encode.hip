@@ -356,6 +356,42 @@ extern "C" int ohx_normalize_u8(ohx_ctx* c, const double* const* d_enc, int nch,
     return check_hip(c, hipGetLastError(), "k_normalize");
 }
 
+namespace ohx {
+
+int launch_encode_normalize_u8(hipStream_t stream, unsigned long long* mm, const double* const* d_field, int nch,
+                               int64_t n, uint32_t flag, double* const* d_enc, uint8_t* const* d_out)
+{
+    EncPtrs ep{};
+    ChanPtrs cp{};
+    for (int i = 0; i < nch; ++i) {
+        ep.f[i] = (const double2*)d_field[i];
+        ep.e[i] = d_enc[i];
+        cp.p[i] = d_enc[i];
+        cp.o[i] = d_out[i];
+    }
+    hipLaunchKernelGGL(k_minmax_init, dim3(1), dim3(64), 0, stream, mm);
+    const dim3 g(std::min<unsigned>(egrid(n), ENCODE_MM_BLOCKS)), b(EB);
+    switch (flag) {
+    case OHX_ENCODE_PHASE:
+        hipLaunchKernelGGL(k_encode_mm<OHX_ENCODE_PHASE>, g, b, 0, stream, ep, nch, n, mm);
+        break;
+    case OHX_ENCODE_AMPLITUDE:
+        hipLaunchKernelGGL(k_encode_mm<OHX_ENCODE_AMPLITUDE>, g, b, 0, stream, ep, nch, n, mm);
+        break;
+    case OHX_ENCODE_REAL:
+        hipLaunchKernelGGL(k_encode_mm<OHX_ENCODE_REAL>, g, b, 0, stream, ep, nch, n, mm);
+        break;
+    default:
+        hipLaunchKernelGGL(k_encode_mm<OHX_ENCODE_IMAGINARY>, g, b, 0, stream, ep, nch, n, mm);
+        break;
+    }
+    hipLaunchKernelGGL(k_to_u8, dim3(egrid((n + 3) / 4)), dim3(EB), 0, stream, cp, nch, n,
+                       (const unsigned long long*)mm);
+    return hipGetLastError() == hipSuccess ? OHX_OK : OHX_ERR_DEVICE;
+}
+
+}  // namespace ohx
+
 extern "C" int ohx_encode_normalize_u8(ohx_ctx* c, const double* const* d_field, int nch, int64_t n,
                                        uint32_t flag, double* const* d_enc, uint8_t* const* d_out)
 {
@@ -364,36 +400,13 @@ extern "C" int ohx_encode_normalize_u8(ohx_ctx* c, const double* const* d_field,
         return set_error(c, OHX_ERR_INVALID_ARG, "bad channel count or buffers");
     if (flag > OHX_ENCODE_IMAGINARY)
         return set_error(c, OHX_ERR_UNSUPPORTED, "encode flag not on the point-cloud path");
-    EncPtrs ep{};
-    ChanPtrs cp{};
-    for (int i = 0; i < nch; ++i) {
+    for (int i = 0; i < nch; ++i)
         if (n > 0 && (!d_field[i] || !d_enc[i] || !d_out[i]))
             return set_error(c, OHX_ERR_INVALID_ARG, "null channel buffer");
-        ep.f[i] = (const double2*)d_field[i];
-        ep.e[i] = d_enc[i];
-        cp.p[i] = d_enc[i];
-        cp.o[i] = d_out[i];
-    }
     if (n == 0) return OHX_OK;
     std::lock_guard<std::mutex> lk(c->mu);
     (void)hipSetDevice(c->device);
-    hipLaunchKernelGGL(k_minmax_init, dim3(1), dim3(64), 0, c->stream, c->minmax);
-    const dim3 g(std::min<unsigned>(egrid(n), ENCODE_MM_BLOCKS)), b(EB);
-    switch (flag) {
-    case OHX_ENCODE_PHASE:
-        hipLaunchKernelGGL(k_encode_mm<OHX_ENCODE_PHASE>, g, b, 0, c->stream, ep, nch, n, c->minmax);
-        break;
-    case OHX_ENCODE_AMPLITUDE:
-        hipLaunchKernelGGL(k_encode_mm<OHX_ENCODE_AMPLITUDE>, g, b, 0, c->stream, ep, nch, n, c->minmax);
-        break;
-    case OHX_ENCODE_REAL:
-        hipLaunchKernelGGL(k_encode_mm<OHX_ENCODE_REAL>, g, b, 0, c->stream, ep, nch, n, c->minmax);
-        break;
-    default:
-        hipLaunchKernelGGL(k_encode_mm<OHX_ENCODE_IMAGINARY>, g, b, 0, c->stream, ep, nch, n, c->minmax);
-        break;
-    }
-    hipLaunchKernelGGL(k_to_u8, dim3(egrid((n + 3) / 4)), dim3(EB), 0, c->stream, cp, nch, n,
-                       (const unsigned long long*)c->minmax);
-    return check_hip(c, hipGetLastError(), "k_encode_normalize");
+    if (launch_encode_normalize_u8(c->stream, c->minmax, d_field, nch, n, flag, d_enc, d_out) != OHX_OK)
+        return check_hip(c, hipGetLastError(), "k_encode_normalize");
+    return OHX_OK;
 }

@@ -163,6 +163,13 @@ static unsigned grid_for(int64_t n, int block)
     return (unsigned)g;
 }
 
+int field_zero_on(hipStream_t stream, double* f, int64_t n)
+{
+    if (n <= 0) return OHX_OK;
+    hipLaunchKernelGGL(k_field_zero, dim3(grid_for(n, 256)), dim3(256), 0, stream, (double2*)f, n);
+    return hipGetLastError() == hipSuccess ? OHX_OK : OHX_ERR_DEVICE;
+}
+
 }  // namespace ohx
 
 using namespace ohx;

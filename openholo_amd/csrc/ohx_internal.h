@@ -101,6 +101,14 @@ int ensure_gemm_scratch(ohx_ctx* ctx, size_t bytes);
 // launchers (pc_direct.hip)
 int launch_prep(ohx_ctx* ctx, const void* d_vertices, const Geometry& g, const PointSoA& soa,
                 unsigned long long* boxsum = nullptr);
+int launch_prep_on(hipStream_t stream, const void* d_vertices, const Geometry& g, const PointSoA& soa,
+                   unsigned long long* boxsum = nullptr);
+// fill kernel on a given stream (ohx_core.hip)
+int field_zero_on(hipStream_t stream, double* f, int64_t n_px);
+// encode every channel, min/max in the same pass, then 8-bit (encode.hip);
+// mm: two device words of scratch
+int launch_encode_normalize_u8(hipStream_t stream, unsigned long long* mm, const double* const* d_field, int nch,
+                               int64_t n, uint32_t flag, double* const* d_enc, uint8_t* const* d_out);
 int launch_direct(ohx_ctx* ctx, const Geometry& g, const PointSoA& soa, double* d_field, bool fp64);
 // flag: device word for the FP32 kernel's "EXACT pairs left" flag;
 // allow_splits = false runs every tile's points in one workgroup (no partial

@@ -1052,14 +1052,21 @@ __global__ __launch_bounds__(256) void k_pc_probe(Geometry g, PointSoA s, const 
 }
 
 // ---------------------------------------------------------------------------
+int launch_prep_on(hipStream_t stream, const void* d_vertices, const Geometry& g, const PointSoA& soa,
+                   unsigned long long* boxsum)
+{
+    const unsigned grid = (unsigned)((soa.n + 255) / 256);
+    if (boxsum) hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, stream, boxsum);
+    hipLaunchKernelGGL(k_pc_prep, dim3(grid), dim3(256), 0, stream, (const double*)d_vertices, g, soa, boxsum);
+    return hipGetLastError() == hipSuccess ? OHX_OK : OHX_ERR_DEVICE;
+}
+
 int launch_prep(ohx_ctx* ctx, const void* d_vertices, const Geometry& g, const PointSoA& soa,
                 unsigned long long* boxsum)
 {
-    const unsigned grid = (unsigned)((soa.n + 255) / 256);
-    if (boxsum) hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, ctx->stream, boxsum);
-    hipLaunchKernelGGL(k_pc_prep, dim3(grid), dim3(256), 0, ctx->stream,
-                       (const double*)d_vertices, g, soa, boxsum);
-    return check_hip(ctx, hipGetLastError(), "k_pc_prep");
+    if (launch_prep_on(ctx->stream, d_vertices, g, soa, boxsum) != OHX_OK)
+        return check_hip(ctx, hipErrorLaunchFailure, "k_pc_prep");
+    return OHX_OK;
 }
 
 // Workgroups the device holds at once for kernel k (occupancy x CUs).

@@ -201,3 +201,28 @@ def test_dice_example_end_to_end(tmp_path, orc, diff, prec):
     assert err < (1e-5 if prec == 0 else 1e-9)
     assert np.allclose(e, orc.encode(f, orc.ENCODE_PHASE), rtol=0, atol=4e-16)
     assert np.array_equal(u, orc.normalize([e])[0])
+
+
+@pytest.mark.parametrize("wid,diff", [(1, 0), (3, 1), (5, 0), (5, 1)])
+def test_fp64_probe_vs_oracle_at_workload_geometry(ctx, orc, wid, diff):
+    """The library's FP64 probe (ohx_pc_probe_f64, the reference behind the
+    benchmark's rel-L2 figures; it shares k_pc_prep with the kernels) against
+    the extended-precision oracle at the 10k-point 1080p, the 1M-point
+    complex-amplitude and the 2M-point 4320p workload geometries (first
+    3,000 points, last channel, 1,000 pixels; partial masks at 4320p):
+    relative L2 <= 1e-12."""
+    w = synth.WORKLOADS[wid]
+    v = synth.points(w, 3000)
+    ch = len(w.lambdas) - 1
+    g = orc.make_geom(w.pn, w.pp)
+    pix = np.random.default_rng(700 + wid).choice(w.n_pixels, 1000, replace=False)
+    dv = ctx.to_device(v)
+    try:
+        got = ctx.probe_f64(dv, len(v), capi.params(w.pn, w.pp, w.lambdas[ch], channel=ch,
+                                                     use_phase=w.complex_amplitude), diff, pix)
+    finally:
+        dv.free()
+    want = orc.field_exact(g, v, w.lambdas[ch], ch, diff, pix, use_phase=w.complex_amplitude)
+    err = rel_l2(got, want)
+    print(f"workload {wid} diff={diff}: probe rel-L2 {err:.2e}")
+    assert err < 1e-12

@@ -44,8 +44,12 @@ enum { OHX_DIFF_RS = 0, OHX_DIFF_FRESNEL = 1 };
  * plus matrix-core variants of the Fresnel sum (no reference counterpart).
  *   0 (MODE_DOUBLE)         : FP64 direct sum (every term in FP64)
  *   MODE_FLOAT [|FASTMATH]   : FP32 direct sum with FP64 per-point phase constants
- *   MFMA_*  (Fresnel only)   : H = U diag(a) V^T on matrix cores; RS ignores them
- *                              and runs the FP32 direct sum. */
+ *   MFMA_*                   : Fresnel: H = U diag(a) V^T on matrix cores.
+ *                              RS with BF16X1 / BF16X3 (and not FP16X3): points
+ *                              whose mask is the whole hologram and whose
+ *                              rank-2 tile expansion holds run on matrix cores,
+ *                              the others (and RS with FP16X3) run the FP32
+ *                              direct sum. */
 enum {
     OHX_MODE_GPU = 1,
     OHX_MODE_FLOAT = 2,
@@ -248,7 +252,9 @@ typedef struct ohx_launch_stats {
     int64_t pairs_nominal;   /* n_points * pn_x * pn_y                        */
     int64_t workgroups;      /* workgroups of the diffraction kernel          */
     int32_t kernel_kind;     /* 0 FP32 direct, 1 FP64 direct, 2 MFMA GEMM, 3 WRP */
-    int32_t reserved;
+    int32_t gemm_points;     /* RS with an MFMA mode: points that ran on the
+                                matrix cores (the others ran the FP32 direct
+                                sum); 0 otherwise                            */
     int64_t pairs_in_box;    /* point-pixel pairs inside the points' anti-
                                 aliasing boxes (ophGen.cpp:384-407,467-490):
                                 the pairs the kernels evaluate; the RS range

@@ -71,7 +71,7 @@ def wrp_params(pn, pp, lam, wrp_location, distance, channel=0, scale=(1.0, 1.0, 
 
 class LaunchStats(ctypes.Structure):
     _fields_ = [("pairs_nominal", ctypes.c_int64), ("workgroups", ctypes.c_int64),
-                ("kernel_kind", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("kernel_kind", ctypes.c_int32), ("gemm_points", ctypes.c_int32),
                 ("pairs_in_box", ctypes.c_int64)]
 
 

@@ -1,0 +1,227 @@
+// Shared pieces of the matrix-core kernels (Fresnel and RS): tile shape, the
+// 16-bit operand packing and split, the MFMA wrapper, the LDS operand layout
+// and the pixel grid of the reference (ophgen/src/ophGen.cpp:413,422).
+#pragma once
+
+#include "ohx_internal.h"
+
+namespace ohx {
+namespace gemm {
+
+constexpr int BM = 128;           // tile rows (y)
+constexpr int BN = 128;           // tile columns (x), complex; wave w owns 32 of them
+constexpr int NT = 256;           // 4 waves; two workgroups per CU (80 KB LDS each)
+constexpr int KP = 16;            // points per stage
+constexpr int KS = 2 * KP;        // realified K per stage (32)
+constexpr int LDK = KS + 8;       // padded row length (bf16/f16 elements): 80 B rows
+constexpr int ROWW = LDK / 2;     // row length in dwords (20)
+constexpr int NGRP = NT / KP;     // thread groups of 16 lanes, one point per lane (16)
+constexpr int NVT = BM / NGRP;    // V rows per thread and stage (8): grp + NGRP i
+constexpr int NUT = BN / NGRP;    // U columns per thread and stage (8): grp + NGRP j
+constexpr int NE = NVT + NUT;     // elements per thread and stage
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+enum { V_BF16X1 = 0, V_BF16X3 = 1, V_FP16X3 = 2 };
+
+template <int VAR>
+struct Traits {
+    static constexpr bool three = VAR != V_BF16X1;
+    static constexpr bool fp16 = VAR == V_FP16X3;
+};
+
+// pack two floats into one dword of two 16-bit values (BF16: round to nearest
+// even, v_cvt_pk_bf16_f32; FP16: round toward zero, v_cvt_pkrtz_f16_f32 -- the
+// 3-term split keeps the residual either way)
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+template <bool FP16>
+__device__ inline unsigned pack2(float a, float b)
+{
+    if constexpr (FP16) {
+        return __builtin_bit_cast(unsigned, __builtin_amdgcn_cvt_pkrtz(a, b));
+    } else {
+        const bf16x2 v = {(__bf16)a, (__bf16)b};
+        return __builtin_bit_cast(unsigned, v);
+    }
+}
+template <bool FP16>
+__device__ inline float lo16(unsigned w)
+{
+    if constexpr (FP16) return (float)__builtin_bit_cast(_Float16, (unsigned short)(w & 0xffffu));
+    // the low half moved up by a byte permute: a shift would let the compiler
+    // re-derive it from a second conversion of the low operand alone
+    return __builtin_bit_cast(float, __builtin_amdgcn_perm(w, 0u, 0x05040c0cu));
+}
+template <bool FP16>
+__device__ inline float hi16(unsigned w)
+{
+    if constexpr (FP16) return (float)__builtin_bit_cast(_Float16, (unsigned short)(w >> 16));
+    return __builtin_bit_cast(float, w & 0xffff0000u);
+}
+
+template <int VAR>
+__device__ inline f32x16 mfma(u32x4 a, u32x4 b, f32x16 c)
+{
+    if constexpr (Traits<VAR>::fp16)
+        return __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(f16x8, a), __builtin_bit_cast(f16x8, b),
+                                                      c, 0, 0, 0);
+    else
+        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a),
+                                                       __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
+}
+
+// LDS: V[y][p] and U[x][p] as packed (Re, Im) 16-bit pairs, hi and (3-term) lo
+// parts, double buffered; rows padded to 80 bytes (conflict-free ds_read_b128).
+template <bool THREE>
+struct Smem {
+    static constexpr int NH = THREE ? 2 : 1;
+    unsigned v[2][NH][BM][ROWW];
+    unsigned u[2][NH][BN][ROWW];
+};
+
+__device__ inline double gpix_x(const Geometry& g, int col)
+{
+#pragma clang fp contract(off)
+    return g.x0 + (double)(col - 1 + g.off_x) * g.pp_x;
+}
+__device__ inline double gpix_y(const Geometry& g, int row)
+{
+#pragma clang fp contract(off)
+    return g.y0 + (double)(g.pn_y - row + g.off_y) * g.pp_y;
+}
+
+// x - floor(x) in one v_fract_f64 (clamped below 1.0)
+__device__ inline double frac64(double x) { return __builtin_amdgcn_fract(x); }
+
+// LDS plane layout: plane h (hi / lo part) of one buffer's V block is BM rows
+// of ROWW dwords (U block: BN rows); element (row, p) at h*PLANE + row*ROWW + p.
+constexpr int PLANE_V = BM * ROWW;
+constexpr int PLANE_U = BN * ROWW;
+
+// B' fragments from one U fragment: Re block (Re U, -Im U), Im block (Im U, Re U)
+__device__ inline u32x4 to_re(u32x4 w) { return w ^ 0x80000000u; }
+__device__ inline u32x4 to_im(u32x4 w)
+{
+    u32x4 r;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) r[i] = __builtin_amdgcn_alignbit(w[i], w[i], 16);
+    return r;
+}
+
+// ---- MFMA side of one pipelined stage ----------------------------------------
+// C holds the stage's operand registers: acc[4][2] (Re / Im accumulator blocks
+// of the wave's 128 x 32 output), A[8][2] (V fragments, hi / lo), Ub[2][2]
+// (U fragments), B[2][4] (B' fragments), and the lane's fragment coordinates
+// lrow, koff, ucol.
+// MFMA order per (ks, mb) group: 3-term (ah,Re) (ah,Im) (ah,lo Re) (ah,lo Im)
+// (al,Re) (al,Im); 1-term (ah,Re) (ah,Im).  Wave w owns output columns
+// 32w .. 32w+31 of the tile (Re and Im blocks) and all 128 rows (4 blocks).
+template <int VAR, class C>
+__device__ inline void load_a(const unsigned* rv, C& c, int g)
+{
+    const unsigned* a = rv + ((g & 3) * 32 + c.lrow) * ROWW + (g >> 2) * 8 + c.koff;
+    c.A[g][0] = *reinterpret_cast<const u32x4*>(a);
+    if constexpr (Traits<VAR>::three) c.A[g][1] = *reinterpret_cast<const u32x4*>(a + PLANE_V);
+}
+template <int VAR, class C>
+__device__ inline void load_b(const unsigned* ru, C& c, int ks)
+{
+    const unsigned* b = ru + c.ucol * ROWW + ks * 8 + c.koff;
+    c.Ub[ks][0] = *reinterpret_cast<const u32x4*>(b);
+    if constexpr (Traits<VAR>::three) c.Ub[ks][1] = *reinterpret_cast<const u32x4*>(b + PLANE_U);
+}
+template <class C>
+__device__ inline void xform_b(C& c, int ks, int h)
+{
+    c.B[ks][2 * h] = to_re(c.Ub[ks][h]);
+    c.B[ks][2 * h + 1] = to_im(c.Ub[ks][h]);
+}
+
+// MFMAs per stage and wave
+template <int VAR>
+constexpr int mfma_per_stage()
+{
+    return Traits<VAR>::three ? 48 : 16;
+}
+
+// Operands of a stage's first MFMAs, loaded before slot 0.
+template <int VAR, class C>
+__device__ inline void stage_operands(const unsigned* rv, const unsigned* ru, C& c)
+{
+    load_b<VAR>(ru, c, 0);
+    load_a<VAR>(rv, c, 0);
+    if constexpr (!Traits<VAR>::three) load_a<VAR>(rv, c, 1);
+    xform_b(c, 0, 0);
+    if constexpr (Traits<VAR>::three) xform_b(c, 0, 1);
+}
+
+// Slot S of a stage: MFMA S and the operand loads / transforms due there.
+template <int VAR, int S, class C>
+__device__ inline void mfma_slot(const unsigned* rv, const unsigned* ru, C& c)
+{
+    constexpr bool THREE = Traits<VAR>::three;
+    constexpr int NTG = THREE ? 6 : 2;       // MFMAs per (ks, mb) group
+    constexpr int NM = 8 * NTG;
+    constexpr int g = S / NTG, t = S % NTG, ks = g >> 2, mb = g & 3;
+    c.acc[mb][t & 1] = mfma<VAR>(c.A[g][t < 4 ? 0 : 1], c.B[ks][t < 4 ? t : t - 4], c.acc[mb][t & 1]);
+    if constexpr (t == 0 && THREE && g + 1 < 8) load_a<VAR>(rv, c, g + 1);
+    if constexpr (t == 0 && !THREE && g + 2 < 8) load_a<VAR>(rv, c, g + 2);
+    if constexpr (S == NM / 4) load_b<VAR>(ru, c, 1);
+    if constexpr (S == NM * 3 / 8) xform_b(c, 1, 0);
+    if constexpr (THREE && S == NM * 3 / 8 + 2) xform_b(c, 1, 1);
+}
+
+// FP32 partial sums of one split, interleaved (Re, Im) per pixel, into plane
+// `split` of part (pn_y x pn_x float2 per plane).
+__device__ inline void store_partials(const Geometry& g, const f32x16 (&acc)[4][2], float2* __restrict__ part,
+                                      int split, int x0, int y0, int ucol, int lane)
+{
+    const int col = x0 + ucol;
+    if (col >= g.pn_x) return;
+    float2* out = part + (int64_t)split * g.pn_y * (int64_t)g.pn_x;
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb)
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+            const int row = y0 + mb * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+            if (row < g.pn_y) out[(int64_t)row * g.pn_x + col] = make_float2(acc[mb][0][v], acc[mb][1][v]);
+        }
+}
+
+}  // namespace gemm
+
+// ---- host side: tiles and split-K plan ---------------------------------------
+struct GemmPlan {
+    int tiles_x, tiles;
+    int64_t splits, kslice, npx;
+};
+// kq: K granularity of a split in points (one stage)
+inline GemmPlan plan_gemm(const Geometry& g, int64_t npoints, int kq)
+{
+    using namespace gemm;
+    GemmPlan p;
+    p.tiles_x = (g.pn_x + BN - 1) / BN;
+    p.tiles = p.tiles_x * ((g.pn_y + BM - 1) / BM);
+    p.npx = (int64_t)g.pn_x * g.pn_y;
+    // Split K so that (a) every CU gets several workgroups (two reside per CU)
+    // and (b) no FP32 accumulator sums more than MAX_SLICE points: FP32
+    // rounding of the running sums grows like sqrt(points per slice).
+    constexpr int64_t MAX_SLICE = 65536;
+    const int64_t n = npoints > 1 ? npoints : 1;
+    int64_t splits = (1024 + p.tiles - 1) / p.tiles;
+    if ((n + MAX_SLICE - 1) / MAX_SLICE > splits) splits = (n + MAX_SLICE - 1) / MAX_SLICE;
+    int64_t max_by_k = (n + 4 * kq - 1) / (4 * kq);   // >= 4 stages per split
+    if (max_by_k < 1) max_by_k = 1;
+    if (splits > max_by_k) splits = max_by_k;
+    int64_t kslice = (n + splits - 1) / splits;
+    kslice = (kslice + kq - 1) / kq * kq;
+    p.splits = (n + kslice - 1) / kslice;
+    if (p.splits < 1) p.splits = 1;
+    p.kslice = kslice;
+    return p;
+}
+
+}  // namespace ohx

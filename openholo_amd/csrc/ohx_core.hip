@@ -232,6 +232,7 @@ int ohx_ctx_destroy(ohx_ctx* c)
     if (c->pts) (void)hipFree(c->pts);
     if (c->minmax) (void)hipFree(c->minmax);
     if (c->gemm_scratch) (void)hipFree(c->gemm_scratch);
+    if (c->rs_scratch) (void)hipFree(c->rs_scratch);
     wrp_release(c);
     (void)hipStreamSynchronize(c->copy_stream);
     (void)hipEventDestroy(c->copy_ready);
@@ -386,6 +387,7 @@ int ohx_pc_diffract(ohx_ctx* c, const void* d_vertices, int64_t n, double* d_fie
     c->boxsum_src = c->minmax + 4;
     const uint32_t mfma = mode & (OHX_MODE_MFMA_BF16X1 | OHX_MODE_MFMA_BF16X3 | OHX_MODE_MFMA_FP16X3);
     if (mfma && diff == OHX_DIFF_FRESNEL) return launch_fresnel_gemm(c, g, soa, d_field, mode);
+    if (mfma && !(mode & OHX_MODE_MFMA_FP16X3)) return launch_rs_gemm(c, g, soa, d_field, mode);
     const bool fp64 = (mode & (OHX_MODE_FLOAT | OHX_MODE_FASTMATH)) == 0 && mfma == 0;
     return launch_direct(c, g, soa, d_field, fp64);
 }

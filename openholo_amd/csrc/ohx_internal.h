@@ -27,6 +27,10 @@ struct ohx_ctx {
     // partial planes of the direct kernels (grown on demand)
     void* gemm_scratch = nullptr;
     size_t gemm_scratch_bytes = 0;
+    // RS matrix-core path: point classification, partition and the two point
+    // subsets' constants (grown on demand)
+    void* rs_scratch = nullptr;
+    size_t rs_scratch_bytes = 0;
     // hipFFT plans of the WRP propagation (hipfftHandle) for an L x M grid:
     // length-L transforms of ceil(M/4) contiguous rows of the transposed
     // grid, length-M transforms of all L rows
@@ -127,6 +131,11 @@ int launch_direct_jobs(hipStream_t stream, const DirectJob* d_jobs, const int* d
 int launch_fresnel_gemm_range(hipStream_t stream, void* scratch, size_t scratch_bytes, const Geometry& g,
                               const PointSoA& soa, int64_t p0, int64_t p1, double* d_field, uint32_t mode);
 size_t fresnel_gemm_scratch_bytes(const Geometry& g, int64_t npoints);
+// field += -j * sum of `splits` FP32 partial planes (fixed order), pc_gemm.hip
+int launch_gemm_reduce(hipStream_t stream, const void* part, int64_t splits, int64_t npx, double* d_field);
+// RS on matrix cores (rs_gemm.hip): points whose separable rank-2 form holds on
+// every tile go to the GEMM, the others to the FP32 direct kernel
+int launch_rs_gemm(ohx_ctx* ctx, const Geometry& g, const PointSoA& soa, double* d_field, uint32_t mode);
 int launch_probe(ohx_ctx* ctx, const Geometry& g, const PointSoA& soa, const int64_t* d_pix,
                  int64_t n_pix, double* d_out);
 

@@ -29,96 +29,11 @@
 // order (deterministic) and adds -j S into the FP64 field.
 #include <algorithm>
 
-#include "ohx_internal.h"
+#include "gemm_common.h"
 
 namespace ohx {
 
 namespace gemm {
-
-constexpr int BM = 128;           // tile rows (y)
-constexpr int BN = 128;           // tile columns (x), complex; wave w owns 32 of them
-constexpr int NT = 256;           // 4 waves; two workgroups per CU (80 KB LDS each)
-constexpr int KP = 16;            // points per stage
-constexpr int KS = 2 * KP;        // realified K per stage (32)
-constexpr int LDK = KS + 8;       // padded row length (bf16/f16 elements): 80 B rows
-constexpr int ROWW = LDK / 2;     // row length in dwords (20)
-constexpr int NGRP = NT / KP;     // thread groups of 16 lanes, one point per lane (16)
-constexpr int NVT = BM / NGRP;    // V rows per thread and stage (8): grp + NGRP i
-constexpr int NUT = BN / NGRP;    // U columns per thread and stage (8): grp + NGRP j
-constexpr int NE = NVT + NUT;     // elements per thread and stage
-
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
-typedef float f32x16 __attribute__((ext_vector_type(16)));
-typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
-
-enum { V_BF16X1 = 0, V_BF16X3 = 1, V_FP16X3 = 2 };
-
-template <int VAR>
-struct Traits {
-    static constexpr bool three = VAR != V_BF16X1;
-    static constexpr bool fp16 = VAR == V_FP16X3;
-};
-
-// pack two floats into one dword of two 16-bit values (BF16: round to nearest
-// even, v_cvt_pk_bf16_f32; FP16: round toward zero, v_cvt_pkrtz_f16_f32 -- the
-// 3-term split keeps the residual either way)
-typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
-template <bool FP16>
-__device__ inline unsigned pack2(float a, float b)
-{
-    if constexpr (FP16) {
-        return __builtin_bit_cast(unsigned, __builtin_amdgcn_cvt_pkrtz(a, b));
-    } else {
-        const bf16x2 v = {(__bf16)a, (__bf16)b};
-        return __builtin_bit_cast(unsigned, v);
-    }
-}
-template <bool FP16>
-__device__ inline float lo16(unsigned w)
-{
-    if constexpr (FP16) return (float)__builtin_bit_cast(_Float16, (unsigned short)(w & 0xffffu));
-    // the low half moved up by a byte permute: a shift would let the compiler
-    // re-derive it from a second conversion of the low operand alone
-    return __builtin_bit_cast(float, __builtin_amdgcn_perm(w, 0u, 0x05040c0cu));
-}
-template <bool FP16>
-__device__ inline float hi16(unsigned w)
-{
-    if constexpr (FP16) return (float)__builtin_bit_cast(_Float16, (unsigned short)(w >> 16));
-    return __builtin_bit_cast(float, w & 0xffff0000u);
-}
-
-template <int VAR>
-__device__ inline f32x16 mfma(u32x4 a, u32x4 b, f32x16 c)
-{
-    if constexpr (Traits<VAR>::fp16)
-        return __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(f16x8, a), __builtin_bit_cast(f16x8, b),
-                                                      c, 0, 0, 0);
-    else
-        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a),
-                                                       __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
-}
-
-// LDS: V[y][p] and U[x][p] as packed (Re, Im) 16-bit pairs, hi and (3-term) lo
-// parts, double buffered; rows padded to 80 bytes (conflict-free ds_read_b128).
-template <bool THREE>
-struct Smem {
-    static constexpr int NH = THREE ? 2 : 1;
-    unsigned v[2][NH][BM][ROWW];
-    unsigned u[2][NH][BN][ROWW];
-};
-
-__device__ inline double gpix_x(const Geometry& g, int col)
-{
-#pragma clang fp contract(off)
-    return g.x0 + (double)(col - 1 + g.off_x) * g.pp_x;
-}
-__device__ inline double gpix_y(const Geometry& g, int row)
-{
-#pragma clang fp contract(off)
-    return g.y0 + (double)(g.pn_y - row + g.off_y) * g.pp_y;
-}
 
 // Point data of one stage as read from HBM: thread t holds point t % 16.
 // pc = (v2, u2, amplitude, 0) from k_gemm_consts.
@@ -157,8 +72,6 @@ struct Pt {
     int rlo, rhi, clo, chi;   // box relative to the tile origin
 };
 
-// x - floor(x) in one v_fract_f64 (clamped below 1.0)
-__device__ inline double frac64(double x) { return __builtin_amdgcn_fract(x); }
 
 // ythr / xthr: y of row grp + NGRP*(NVT/2) and x of column grp + NGRP*(NUT/2)
 // of the tile (the reference element k = 0); dy = -NGRP pp_y and
@@ -241,10 +154,6 @@ __device__ inline void put(unsigned* hi_slot, unsigned* lo_slot, float re, float
     if constexpr (Traits<VAR>::three) *lo_slot = pack2<FP16>(re - lo16<FP16>(h), im - hi16<FP16>(h));
 }
 
-// LDS plane layout: plane h (hi / lo part) of one buffer's V block is BM rows
-// of ROWW dwords (U block: BN rows); element (row, p) at h*PLANE + row*ROWW + p.
-constexpr int PLANE_V = BM * ROWW;
-constexpr int PLANE_U = BN * ROWW;
 
 // A thread's NE elements per stage, V rows spread evenly among U columns:
 // element E is a V row when it raises the running V count.
@@ -313,16 +222,6 @@ __device__ inline void mask_stage(Smem<Traits<VAR>::three>& sm, const Pt& P, int
     }
 }
 
-// B' fragments from one U fragment: Re block (Re U, -Im U), Im block (Im U, Re U)
-__device__ inline u32x4 to_re(u32x4 w) { return w ^ 0x80000000u; }
-__device__ inline u32x4 to_im(u32x4 w)
-{
-    u32x4 r;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) r[i] = __builtin_amdgcn_alignbit(w[i], w[i], 16);
-    return r;
-}
-
 // State of one pipelined stage (see stage_body).
 template <int VAR>
 struct StageCtx {
@@ -341,29 +240,6 @@ struct StageCtx {
     float2 cs[NE];
 };
 
-// MFMA order per (ks, mb) group: 3-term (ah,Re) (ah,Im) (ah,lo Re) (ah,lo Im)
-// (al,Re) (al,Im); 1-term (ah,Re) (ah,Im).  Wave w owns output columns
-// 32w .. 32w+31 of the tile (Re and Im blocks) and all 128 rows (4 blocks).
-template <int VAR>
-__device__ inline void load_a(const unsigned* rv, StageCtx<VAR>& c, int g)
-{
-    const unsigned* a = rv + ((g & 3) * 32 + c.lrow) * ROWW + (g >> 2) * 8 + c.koff;
-    c.A[g][0] = *reinterpret_cast<const u32x4*>(a);
-    if constexpr (Traits<VAR>::three) c.A[g][1] = *reinterpret_cast<const u32x4*>(a + PLANE_V);
-}
-template <int VAR>
-__device__ inline void load_b(const unsigned* ru, StageCtx<VAR>& c, int ks)
-{
-    const unsigned* b = ru + c.ucol * ROWW + ks * 8 + c.koff;
-    c.Ub[ks][0] = *reinterpret_cast<const u32x4*>(b);
-    if constexpr (Traits<VAR>::three) c.Ub[ks][1] = *reinterpret_cast<const u32x4*>(b + PLANE_U);
-}
-template <int VAR>
-__device__ inline void xform_b(StageCtx<VAR>& c, int ks, int h)
-{
-    c.B[ks][2 * h] = to_re(c.Ub[ks][h]);
-    c.B[ks][2 * h + 1] = to_im(c.Ub[ks][h]);
-}
 // The generation schedule: 3-term, half-element h (even: value of element
 // h/2, odd: its store) at slot H0 + (46 - H0) h / (2 NE); 1-term, element E
 // at slot E0 + (16 - E0) E / NE (two elements share a slot when needed).
@@ -411,16 +287,9 @@ __device__ inline void slot(const unsigned* __restrict__ rv, const unsigned* __r
                             unsigned* __restrict__ wv, unsigned* __restrict__ wu, StageCtx<VAR>& c)
 {
     constexpr bool THREE = Traits<VAR>::three;
-    constexpr int NTG = THREE ? 6 : 2;       // MFMAs per (ks, mb) group
-    constexpr int NM = 8 * NTG;              // MFMAs per stage and wave
+    constexpr int NM = mfma_per_stage<VAR>();
     if constexpr (S < NM) {
-        constexpr int g = S / NTG, t = S % NTG, ks = g >> 2, mb = g & 3;
-        c.acc[mb][t & 1] = mfma<VAR>(c.A[g][t < 4 ? 0 : 1], c.B[ks][t < 4 ? t : t - 4], c.acc[mb][t & 1]);
-        if constexpr (t == 0 && THREE && g + 1 < 8) load_a(rv, c, g + 1);
-        if constexpr (t == 0 && !THREE && g + 2 < 8) load_a(rv, c, g + 2);
-        if constexpr (S == NM / 4) load_b(ru, c, 1);
-        if constexpr (S == NM * 3 / 8) xform_b(c, 1, 0);
-        if constexpr (THREE && S == NM * 3 / 8 + 2) xform_b(c, 1, 1);
+        mfma_slot<VAR, S>(rv, ru, c);
         if constexpr (THREE && S < NPIECE) prep_piece<S>(c.P, c.t, c.R, c.ythr, c.xthr, c.dy, c.dx);
         if constexpr (!THREE && S < NPIECE / 2) {
             prep_piece<2 * S>(c.P, c.t, c.R, c.ythr, c.xthr, c.dy, c.dx);
@@ -459,14 +328,9 @@ __device__ inline bool stage_body(const unsigned* __restrict__ rv, const unsigne
                                   int64_t pnext, int64_t ke, double ythr, double xthr, double dy, double dx, int x0,
                                   int y0, int p, int grp, int lrow, int koff, int ucol)
 {
-    constexpr bool THREE = Traits<VAR>::three;
     StageCtx<VAR> c{acc, R, s, pconst, kfirst, pnext, ke, ythr, xthr, dy, dx, x0, y0, p, grp, lrow, koff, ucol};
     c.partial = false;
-    load_b(ru, c, 0);
-    load_a(rv, c, 0);
-    if constexpr (!THREE) load_a(rv, c, 1);
-    xform_b(c, 0, 0);
-    if constexpr (THREE) xform_b(c, 0, 1);
+    stage_operands<VAR>(rv, ru, c);
     slot<VAR, 0>(rv, ru, wv, wu, c);
     P = c.P;
     return c.partial;
@@ -539,18 +403,7 @@ __global__ __launch_bounds__(NT, 2) void k_fresnel_gemm(Geometry g, PointSoA s, 
                      ucolx);
     }
 
-    // FP32 partial sums S of this split, interleaved (Re, Im) per pixel
-    const int col = x0 + ucolx;
-    if (col < g.pn_x) {
-        float2* out = part + (int64_t)split * g.pn_y * (int64_t)g.pn_x;
-#pragma unroll
-        for (int mb = 0; mb < 4; ++mb)
-#pragma unroll
-            for (int v = 0; v < 16; ++v) {
-                const int row = y0 + mb * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
-                if (row < g.pn_y) out[(int64_t)row * g.pn_x + col] = make_float2(acc[mb][0][v], acc[mb][1][v]);
-            }
-    }
+    store_partials(g, acc, part, split, x0, y0, ucolx, lane);
 }
 
 // field += -j * amax * sum_split S   (FP64, fixed summation order)
@@ -610,33 +463,6 @@ __global__ __launch_bounds__(256) void k_gemm_consts(Geometry g, PointSoA s, int
 
 }  // namespace gemm
 
-namespace {
-struct GemmPlan {
-    int tiles_x, tiles;
-    int64_t splits, kslice, npx;
-};
-GemmPlan plan_gemm(const Geometry& g, int64_t npoints)
-{
-    using namespace gemm;
-    GemmPlan p;
-    p.tiles_x = (g.pn_x + BN - 1) / BN;
-    p.tiles = p.tiles_x * ((g.pn_y + BM - 1) / BM);
-    p.npx = (int64_t)g.pn_x * g.pn_y;
-    // Split K so that (a) every CU gets several workgroups (two reside per CU)
-    // and (b) no FP32 accumulator sums more than MAX_SLICE points: FP32
-    // rounding of the running sums grows like sqrt(points per slice).
-    constexpr int64_t MAX_SLICE = 65536;
-    const int64_t n = std::max<int64_t>(npoints, 1);
-    int64_t splits = std::max<int64_t>((1024 + p.tiles - 1) / p.tiles, (n + MAX_SLICE - 1) / MAX_SLICE);
-    const int64_t max_by_k = std::max<int64_t>(1, (n + 4 * KP - 1) / (4 * KP));  // >= 4 stages per split
-    splits = std::min(splits, max_by_k);
-    int64_t kslice = (n + splits - 1) / splits;
-    kslice = (kslice + KP - 1) / KP * KP;
-    p.splits = std::max<int64_t>(1, (n + kslice - 1) / kslice);
-    p.kslice = kslice;
-    return p;
-}
-}  // namespace
 
 // scratch: [amax, 256 B] [per-point float4 constants, padded to 256 B]
 // [split-K partial planes]
@@ -644,7 +470,7 @@ static size_t consts_bytes(int64_t npoints) { return ((size_t)npoints * sizeof(f
 
 size_t fresnel_gemm_scratch_bytes(const Geometry& g, int64_t npoints)
 {
-    const GemmPlan p = plan_gemm(g, npoints);
+    const GemmPlan p = plan_gemm(g, npoints, gemm::KP);
     return 256 + consts_bytes(npoints) + (size_t)p.splits * p.npx * sizeof(float2);
 }
 
@@ -654,7 +480,7 @@ int launch_fresnel_gemm_range(hipStream_t stream, void* scratch, size_t scratch_
     using namespace gemm;
     if (p1 <= p0) return OHX_OK;
     const int var = (mode & OHX_MODE_MFMA_FP16X3) ? V_FP16X3 : (mode & OHX_MODE_MFMA_BF16X3) ? V_BF16X3 : V_BF16X1;
-    const GemmPlan pl = plan_gemm(g, p1 - p0);
+    const GemmPlan pl = plan_gemm(g, p1 - p0, KP);
     if (fresnel_gemm_scratch_bytes(g, p1 - p0) > scratch_bytes) return OHX_ERR_INVALID_ARG;
     float* amax = (float*)scratch;
     float4* pconst = (float4*)((char*)scratch + 256);
@@ -686,6 +512,16 @@ int launch_fresnel_gemm_range(hipStream_t stream, void* scratch, size_t scratch_
     return hipGetLastError() == hipSuccess ? OHX_OK : OHX_ERR_DEVICE;
 }
 
+int launch_gemm_reduce(hipStream_t stream, const void* part, int64_t splits, int64_t npx, double* d_field)
+{
+    using namespace gemm;
+    int64_t rg = (npx + 255) / 256;
+    if (rg > 65536) rg = 65536;
+    hipLaunchKernelGGL(k_gemm_reduce, dim3((unsigned)rg), dim3(256), 0, stream, (const float2*)part, (int)splits, npx,
+                       (double2*)d_field, (const float*)nullptr, 0);
+    return hipGetLastError() == hipSuccess ? OHX_OK : OHX_ERR_DEVICE;
+}
+
 // The split-K partial planes grow with points x pixels (16 x 66 MB for 1M points
 // at 3840x2160); above MAX_GEMM_SCRATCH the points are processed in chunks, each
 // reduced into the field before the next (chunk boundaries depend only on the
@@ -705,7 +541,7 @@ int launch_fresnel_gemm(ohx_ctx* ctx, const Geometry& g, const PointSoA& soa, do
         const int64_t p1 = std::min<int64_t>(soa.n, p0 + chunk);
         st = launch_fresnel_gemm_range(ctx->stream, ctx->gemm_scratch, ctx->gemm_scratch_bytes, g, soa, p0, p1,
                                        d_field, mode);
-        const GemmPlan pl = plan_gemm(g, p1 - p0);
+        const GemmPlan pl = plan_gemm(g, p1 - p0, gemm::KP);
         wgs += (int64_t)pl.tiles * pl.splits;
     }
     ctx->last.workgroups = wgs;

@@ -1,0 +1,510 @@
+// RS point-cloud sum as a rank-2 complex GEMM on gfx950 matrix cores.
+//
+// ophGen::RS_Diffraction (ophgen/src/ophGen.cpp:361-445) adds, per point p and
+// pixel (x, y) inside the anti-aliasing mask,
+//   a z / (lambda r^2) (sin 2pi c - j cos 2pi c),  c = r/lambda (+ phase/2pi),
+//   r = sqrt((x - x_p)^2 + (y - y_p)^2 + z^2),
+// i.e. -j A e^{j 2pi c}.  Unlike the Fresnel term this does not factor into a
+// row part times a column part, but on a 128 x 128 tile it is a rank-2 product
+// to the accuracy of the FP32 direct kernel's separable form.  Take the tile's
+// reference pixel (x_c, y_c) = pixel (x0 + 64, y0 + 64), dx_c = x_c - x_p,
+// dy_c = y_c - y_p, r_c^2 = dx_c^2 + dy_c^2 + z^2 and u = x - x_c, v = y - y_c:
+//   r^2 - r_c^2 = n(u) + N(v),   n = u (2 dx_c + u),   N = v (2 dy_c + v)
+//   (r - r_c)/lambda = h (n + N) + k (n + N)^2 + O(e^3 r_c/lambda)
+//   z/r^2 = (z/r_c^2) (1 + mu n)(1 + mu N) + O(e^2)
+// with e = |n + N|/r_c^2, h = 1/(2 r_c lambda), k = -1/(8 r_c^3 lambda) and
+// mu = -1/r_c^2.  Writing X(n) = h n + k n^2, Y(N) = h N + k N^2, the cross
+// term 2 k n N of the phase is theta = 4 pi k n N radians, and e^{j theta} is
+// 1 + j theta to within theta^2/2, so the term A e^{j 2pi c} is
+//   U1[x] V1[y] + U2[x] V2[y]
+//   V1 = (1 + mu N) e^{j 2pi Y(N)},                 V2 = N V1
+//   U1 = A_c (1 + mu n) e^{j 2pi (phi_c + X(n))},  U2 = j 4 pi k n U1
+// with A_c = a z/(lambda r_c^2) and phi_c = frac(c_p + (r_c - z)/lambda): two
+// K entries of the Fresnel kernel's complex GEMM per point (S = sum U V^T,
+// field += -j S), 8 points per 16-entry stage.
+//
+// Which points: a point goes to the matrix cores when, over the whole
+// hologram, (a) its mask is every pixel -- the box of ophGen.cpp:384-407 is
+// the hologram and the per-pixel range test of ophGen.cpp:415-429 holds on
+// every pixel with a 1e-12 margin -- and (b) on every tile the neglected terms
+// are small: e^3 r_c/(16 lambda) <= 2e-7 cycles and e^2 <= 1e-7 (the FP32
+// direct kernel's bounds), and theta <= 5e-3 rad for BF16x3 (theta^2/2 <=
+// 1.25e-5 of the term, the size of the 3-term split's own error), 2e-2 rad
+// for BF16x1 (2e-4, a tenth of its BF16 rounding).  k_rs_classify bounds all
+// three with the largest |dx_c|, |dy_c| over the tiles and r_c >= z.  A
+// stable partition puts these points first; the rest run through the FP32
+// direct kernels (launch_direct_with) after the GEMM.
+//
+// Operand generation, per stage and thread (point t % 8, rows and columns
+// grp + 32 i, grp = t / 8): the per-(point, tile) constants in FP64 (r_c from
+// an FP32 v_rsq seed and one Newton step, relative error ~2e-14), then per row
+// N and Y in FP64, Y reduced mod 1 before rounding to FP32, one v_sin/v_cos
+// pair for both V1 and V2 (and for U1, U2).  Precision: V1 and U1 in the
+// 3-term split (hi hi + hi lo + lo hi), V2 and U2 hi only (lo part zero): the
+// rank-2 correction is below theta of the term, so its 2^-9 BF16 rounding
+// costs below 1e-5 of the term (theta <= 5e-3).  BF16 only: A_c reaches ~1e7,
+// beyond the FP16 range.
+#include <algorithm>
+#include <climits>
+
+#include <hipcub/hipcub.hpp>
+
+#include "gemm_common.h"
+
+namespace ohx {
+
+namespace rsgemm {
+using namespace gemm;
+
+constexpr int RP = KP / 2;       // points per stage (two K entries each)
+constexpr int RG = NT / RP;      // thread groups: thread t -> point t % RP, group t / RP
+constexpr int RR = BM / RG;      // V rows per thread and stage: grp + RG i
+constexpr int RC = BN / RG;      // U columns per thread and stage: grp + RG j
+constexpr int RU = RR + RC;      // work units (a row or a column) per thread and stage
+static_assert(RR == 4 && RC == 4 && PLANE_V == PLANE_U, "unit schedule assumes 4 rows and 4 columns");
+
+constexpr double FOUR_PI = 12.566370614359172953850573533118;
+// neglected terms on the matrix-core path (see the header comment)
+constexpr double MAX_PHASE3 = 2e-7;   // cycles
+constexpr double MAX_AMP2 = 1e-7;
+constexpr double MAX_THETA_X3 = 5e-3; // rad, BF16x3
+constexpr double MAX_THETA_X1 = 2e-2; // rad, BF16x1
+constexpr double RANGE_TOL = 1e-12;   // m, margin of the range test
+
+// Point data of one stage: thread t holds point t % RP.  Lanes past the slice
+// end read the slice's last point and are marked invalid (zero amplitude).
+struct Raw {
+    double px, py, pz, cyc0, kamp;
+    bool valid;
+};
+__device__ inline void load_raw(Raw& r, const PointSoA& s, int64_t pi, int64_t p1)
+{
+    r.valid = pi < p1;
+    const int64_t j = r.valid ? pi : p1 - 1;
+    r.px = s.px[j];
+    r.py = s.py[j];
+    r.pz = s.pz[j];
+    r.cyc0 = s.cyc0[j];
+    r.kamp = s.kamp[j];
+}
+
+// Per-(point, tile) constants of a stage
+struct Pt {
+    double dy2, dx2;          // 2 dy_c, 2 dx_c
+    double h, k;              // 1/(2 r_c lambda), -1/(8 r_c^3 lambda)
+    double phic;              // frac(c_p + (r_c - z)/lambda)
+    float mu, ac, acmu, g;    // -1/r_c^2, A_c (0 for invalid lanes), A_c mu, 4 pi k
+};
+struct Tmp {
+    double dyc, dxc, zz, rc2, y, irc2;
+};
+// Tile frame of a thread: reference pixel, its rows' and columns' offsets
+struct Frame {
+    double yc, xc;            // reference pixel (x0 + 64, y0 + 64)
+    double vb, dv, ub, du;    // v of row grp + RG i = vb + i dv; u of column grp + RG j = ub + j du
+    double il, hl;            // 1/lambda, 1/(2 lambda)
+};
+
+// The constants in NPR pieces of a few dependent FP64 operations each, spread
+// by the pipelined stage over its first MFMA slots.
+constexpr int NPR = 5;
+template <int K>
+__device__ inline void prep_piece(Pt& P, Tmp& t, const Raw& r, const Frame& f)
+{
+    if constexpr (K == 0) {
+        t.dyc = f.yc - r.py;
+        t.dxc = f.xc - r.px;
+        t.zz = r.pz * r.pz;
+    } else if constexpr (K == 1) {
+        t.rc2 = fma(t.dxc, t.dxc, fma(t.dyc, t.dyc, t.zz));
+        t.y = (double)__builtin_amdgcn_rsqf((float)t.rc2);
+    } else if constexpr (K == 2) {
+        // one Newton step on 1/sqrt(r_c^2): ~1e-7 -> ~2e-14 relative
+        const double e = fma(-t.rc2 * t.y, t.y, 1.0);
+        t.y = fma(0.5 * t.y, e, t.y);
+        t.irc2 = t.y * t.y;
+    } else if constexpr (K == 3) {
+        P.phic = frac64(fma(fma(t.rc2, t.y, -r.pz), f.il, r.cyc0));   // r_c = r_c^2 / r_c
+        P.h = t.y * f.hl;
+        P.k = (P.h * t.irc2) * -0.25;
+    } else if constexpr (K == 4) {
+        P.mu = (float)(-t.irc2);
+        P.ac = r.valid ? (float)(r.kamp * t.irc2) : 0.f;
+        P.acmu = P.ac * P.mu;
+        P.g = (float)(FOUR_PI * P.k);
+        P.dy2 = 2.0 * t.dyc;
+        P.dx2 = 2.0 * t.dxc;
+    }
+}
+
+// Unit E of a thread's stage: even E -> V row E/2, odd E -> U column E/2.
+// Value: (Re, Im) of the rank-1 and the rank-2 entry.
+template <int E>
+__device__ inline float4 unit_value(const Pt& P, const Frame& f)
+{
+    constexpr int I = E >> 1;
+    if constexpr ((E & 1) == 0) {
+        const double v = fma((double)I, f.dv, f.vb);
+        const double N = v * (P.dy2 + v);
+        const float ph = (float)frac64(N * fma(P.k, N, P.h));
+        const float Nf = (float)N;
+        const float a = fmaf(P.mu, Nf, 1.f);
+        const float c = a * cos_rev(ph), s = a * sin_rev(ph);
+        return make_float4(c, s, Nf * c, Nf * s);
+    } else {
+        const double u = fma((double)I, f.du, f.ub);
+        const double n = u * (P.dx2 + u);
+        const float ph = (float)frac64(fma(n, fma(P.k, n, P.h), P.phic));
+        const float nf = (float)n;
+        const float a = fmaf(P.acmu, nf, P.ac);
+        const float c = a * cos_rev(ph), s = a * sin_rev(ph);
+        const float gn = P.g * nf;
+        return make_float4(c, s, -gn * s, gn * c);
+    }
+}
+// Store: K entries 2p (rank 1) and 2p+1 (rank 2) of the unit's row / column,
+// hi plane and (3-term) lo plane, one 8-byte LDS write each.
+template <int VAR, int E>
+__device__ inline void unit_store(unsigned* wv, unsigned* wu, float4 w, int p, int grp)
+{
+    constexpr int I = E >> 1;
+    unsigned* hi = ((E & 1) == 0 ? wv : wu) + (grp + RG * I) * ROWW + 2 * p;
+    const unsigned h1 = pack2<false>(w.x, w.y);
+    const unsigned h2 = pack2<false>(w.z, w.w);
+    *reinterpret_cast<uint2*>(hi) = make_uint2(h1, h2);
+    if constexpr (Traits<VAR>::three) {
+        const unsigned l1 = pack2<false>(w.x - lo16<false>(h1), w.y - hi16<false>(h1));
+        *reinterpret_cast<uint2*>(hi + PLANE_V) = make_uint2(l1, 0u);
+    }
+}
+
+// Generation schedule.  3-term (48 MFMA slots): prep piece K at slot K, the
+// next stage's point loads at slot NPR, half-unit h (even: value of unit h/2,
+// odd: its store) at slot H0 + (46 - H0) h / (2 RU).  1-term (16 slots): prep
+// pieces two per slot, loads at slot (NPR + 1) / 2, unit E at slot
+// E0 + (16 - E0) E / RU.
+constexpr int H0 = NPR + 1, E0 = (NPR + 1) / 2 + 1;
+constexpr int half_at(int S)
+{
+    for (int h = 0; h < 2 * RU; ++h)
+        if (H0 + ((46 - H0) * h) / (2 * RU) == S) return h;
+    return -1;
+}
+constexpr int unit_at(int S)
+{
+    for (int e = 0; e < RU; ++e)
+        if (E0 + ((16 - E0) * e) / RU == S) return e;
+    return -1;
+}
+constexpr bool schedule_complete()
+{
+    for (int e = 0; e < RU; ++e) {
+        int n = 0;
+        for (int S = 0; S < 16; ++S) n += unit_at(S) == e;
+        if (n != 1) return false;
+    }
+    for (int h = 0; h < 2 * RU; ++h) {
+        int n = 0;
+        for (int S = 0; S < 48; ++S) n += half_at(S) == h;
+        if (n != 1) return false;
+    }
+    return half_at(NPR) < 0 && unit_at((NPR + 1) / 2) < 0 && E0 > (NPR + 1) / 2;
+}
+static_assert(schedule_complete(), "generation schedule must cover every unit once, after the prep");
+
+template <int VAR>
+struct StageCtx {
+    f32x16 (&acc)[4][2];
+    Raw& R;
+    const PointSoA& s;
+    const Frame& f;
+    int64_t pnext, ke;
+    int p, grp, lrow, koff, ucol;
+    Pt P;
+    Tmp t;
+    u32x4 A[8][2], Ub[2][2], B[2][4];
+    float4 cs[RU];
+};
+
+template <int VAR, int S>
+__device__ inline void slot(const unsigned* __restrict__ rv, const unsigned* __restrict__ ru,
+                           unsigned* __restrict__ wv, unsigned* __restrict__ wu, StageCtx<VAR>& c)
+{
+    constexpr bool THREE = Traits<VAR>::three;
+    constexpr int NM = mfma_per_stage<VAR>();
+    if constexpr (S < NM) {
+        mfma_slot<VAR, S>(rv, ru, c);
+        if constexpr (THREE && S < NPR) prep_piece<S>(c.P, c.t, c.R, c.f);
+        if constexpr (!THREE && 2 * S < NPR) prep_piece<2 * S>(c.P, c.t, c.R, c.f);
+        if constexpr (!THREE && 2 * S + 1 < NPR) prep_piece<2 * S + 1>(c.P, c.t, c.R, c.f);
+        if constexpr (S == (THREE ? NPR : (NPR + 1) / 2)) load_raw(c.R, c.s, c.pnext, c.ke);
+        if constexpr (THREE) {
+            constexpr int h = half_at(S);
+            if constexpr (h >= 0 && (h & 1) == 0) c.cs[h >> 1] = unit_value<(h >> 1)>(c.P, c.f);
+            if constexpr (h >= 0 && (h & 1) == 1) unit_store<VAR, (h >> 1)>(wv, wu, c.cs[h >> 1], c.p, c.grp);
+        } else {
+            constexpr int e = unit_at(S);
+            if constexpr (e >= 0) unit_store<VAR, e>(wv, wu, unit_value<e>(c.P, c.f), c.p, c.grp);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        slot<VAR, S + 1>(rv, ru, wv, wu, c);
+    }
+}
+
+// Whole-stage generation without MFMAs (the prologue's stage 0).
+template <int VAR, int E>
+__device__ inline void gen_units(unsigned* wv, unsigned* wu, const Pt& P, const Frame& f, int p, int grp)
+{
+    if constexpr (E < RU) {
+        unit_store<VAR, E>(wv, wu, unit_value<E>(P, f), p, grp);
+        gen_units<VAR, E + 1>(wv, wu, P, f, p, grp);
+    }
+}
+__device__ inline void prep_all(Pt& P, const Raw& r, const Frame& f)
+{
+    Tmp t;
+    prep_piece<0>(P, t, r, f);
+    prep_piece<1>(P, t, r, f);
+    prep_piece<2>(P, t, r, f);
+    prep_piece<3>(P, t, r, f);
+    prep_piece<4>(P, t, r, f);
+}
+
+// One pipeline step: MFMAs of stage st (buffer RB), generation of stage st+1
+// (buffer RB^1) from R, then R <- stage st+2's point data.  Past the last
+// stage the generation runs on clamped, zero-amplitude data into the unused
+// buffer, which keeps the step branch-free.
+template <int VAR, int RB>
+__device__ inline void step(Smem<Traits<VAR>::three>& sm, f32x16 (&acc)[4][2], Raw& R, const PointSoA& s,
+                            const Frame& f, int64_t pnext, int64_t ke, int p, int grp, int lrow, int koff, int ucol)
+{
+    const unsigned* __restrict__ rv = &sm.v[RB][0][0][0];
+    const unsigned* __restrict__ ru = &sm.u[RB][0][0][0];
+    StageCtx<VAR> c{acc, R, s, f, pnext, ke, p, grp, lrow, koff, ucol};
+    stage_operands<VAR>(rv, ru, c);
+    slot<VAR, 0>(rv, ru, &sm.v[RB ^ 1][0][0][0], &sm.u[RB ^ 1][0][0][0], c);
+    __syncthreads();
+}
+
+// S = sum_p U V^T over points kfirst .. klast-1 of s (all matrix-core
+// eligible), split-K: workgroup (tile, split) sums kslice points into plane
+// `split` of part.
+template <int VAR>
+__global__ __launch_bounds__(NT, 2) void k_rs_gemm(Geometry g, PointSoA s, float2* __restrict__ part, int tiles_x,
+                                                   int tiles, int64_t kfirst, int64_t klast, int64_t kslice)
+{
+    constexpr bool THREE = Traits<VAR>::three;
+    __shared__ __align__(16) Smem<THREE> sm;
+
+    const int tile = blockIdx.x % tiles;
+    const int split = blockIdx.x / tiles;
+    const int x0 = (tile % tiles_x) * BN, y0 = (tile / tiles_x) * BM;
+    const int64_t kb = kfirst + (int64_t)split * kslice;
+    const int64_t ke = min(klast, kb + kslice);
+    const int nst = (int)((ke - kb + RP - 1) / RP);
+    const int p = threadIdx.x & (RP - 1), grp = threadIdx.x / RP;
+    Frame f;
+    f.yc = gpix_y(g, y0 + BM / 2);
+    f.xc = gpix_x(g, x0 + BN / 2);
+    f.vb = -(double)(grp - BM / 2) * g.pp_y;   // row index grows toward -y
+    f.dv = -(double)RG * g.pp_y;
+    f.ub = (double)(grp - BN / 2) * g.pp_x;
+    f.du = (double)RG * g.pp_x;
+    f.il = g.inv_lambda;
+    f.hl = 0.5 * g.inv_lambda;
+
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lrow = lane & 31, koff = (lane >> 5) * 4;
+    const int ucol = wave * 32 + lrow;
+
+    Raw R;
+    if (nst > 0) {
+        load_raw(R, s, kb + p, ke);
+        Pt P;
+        prep_all(P, R, f);
+        gen_units<VAR, 0>(&sm.v[0][0][0][0], &sm.u[0][0][0][0], P, f, p, grp);
+        load_raw(R, s, kb + RP + p, ke);
+    }
+    __syncthreads();
+    f32x16 acc[4][2];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) acc[i][0] = acc[i][1] = f32x16{};
+    for (int st = 0; st < nst; st += 2) {
+        step<VAR, 0>(sm, acc, R, s, f, kb + (int64_t)(st + 2) * RP + p, ke, p, grp, lrow, koff, ucol);
+        if (st + 1 >= nst) break;
+        step<VAR, 1>(sm, acc, R, s, f, kb + (int64_t)(st + 3) * RP + p, ke, p, grp, lrow, koff, ucol);
+    }
+    store_partials(g, acc, part, split, x0, y0, ucol, lane);
+}
+
+// flag[i] = 1 when point i meets the matrix-core conditions of the header on
+// every tile of the hologram.
+__global__ __launch_bounds__(256) void k_rs_classify(Geometry g, PointSoA s, int tiles_x, int tiles_y,
+                                                     double max_theta, unsigned* __restrict__ flag)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= s.n) return;
+    const double px = s.px[i], py = s.py[i], z = s.pz[i];
+    const int4 b = s.box[i];
+    bool ok = z > 0.0 && b.x == 0 && b.y == g.pn_x && b.z == 0 && b.w == g.pn_y;
+    // the reference's range test on every pixel, with margin
+    const double xl = gpix_x(g, 0), xh = gpix_x(g, g.pn_x - 1);
+    const double yh = gpix_y(g, 0), yl = gpix_y(g, g.pn_y - 1);
+    const double dxmax = fmax(fabs(xl - px), fabs(xh - px));
+    const double dymax = fmax(fabs(yl - py), fabs(yh - py));
+    const double dxmin = (px >= xl && px <= xh) ? 0.0 : fmin(fabs(xl - px), fabs(xh - px));
+    const double dymin = (py >= yl && py <= yh) ? 0.0 : fmin(fabs(yl - py), fabs(yh - py));
+    const double zz = z * z;
+    ok = ok && fabs(g.hx_coef) * sqrt(dymin * dymin + zz) - dxmax > RANGE_TOL &&
+         fabs(g.hy_coef) * sqrt(dxmin * dxmin + zz) - dymax > RANGE_TOL;
+    // neglected terms on every tile: reference pixels at tile offset (64, 64),
+    // |u|, |v| <= 64 pitches, r_c >= z
+    const double um = (BN / 2) * g.pp_x, vm = (BM / 2) * g.pp_y;
+    const double xc0 = gpix_x(g, BN / 2), xc1 = gpix_x(g, (tiles_x - 1) * BN + BN / 2);
+    const double yc0 = gpix_y(g, BM / 2), yc1 = gpix_y(g, (tiles_y - 1) * BM + BM / 2);
+    const double dxc = fmax(fabs(xc0 - px), fabs(xc1 - px));
+    const double dyc = fmax(fabs(yc0 - py), fabs(yc1 - py));
+    const double nx = um * (2.0 * dxc + um), ny = vm * (2.0 * dyc + vm);
+    const double e = (nx + ny) / zz;
+    const double kmax = 0.125 * g.inv_lambda / (zz * z);
+    ok = ok && e * e * e * z * g.inv_lambda / 16.0 <= MAX_PHASE3 && e * e <= MAX_AMP2 &&
+         FOUR_PI * kmax * nx * ny <= max_theta;
+    flag[i] = ok ? 1u : 0u;
+}
+
+// Stable partition: eligible points to o[0, ne) and the others to o[ne, n),
+// each in point order; pos = exclusive prefix sum of flag.
+__global__ __launch_bounds__(256) void k_rs_gather(PointSoA s, const unsigned* __restrict__ flag,
+                                                   const unsigned* __restrict__ pos, PointSoA o,
+                                                   unsigned long long* __restrict__ count)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t n = s.n;
+    if (i >= n) return;
+    const int64_t ne = (int64_t)pos[n - 1] + flag[n - 1];
+    const int64_t j = flag[i] ? (int64_t)pos[i] : ne + (i - (int64_t)pos[i]);
+    o.px[j] = s.px[i];
+    o.py[j] = s.py[i];
+    o.pz[j] = s.pz[i];
+    o.cyc0[j] = s.cyc0[i];
+    o.kamp[j] = s.kamp[i];
+    o.q[j] = s.q[i];
+    o.box[j] = s.box[i];
+    if (i == 0) *count = (unsigned long long)ne;
+}
+
+}  // namespace rsgemm
+
+namespace {
+size_t al256(size_t b) { return (b + 255) & ~size_t(255); }
+
+PointSoA soa_tail(const PointSoA& s, int64_t off)
+{
+    PointSoA t = s;
+    t.px += off;
+    t.py += off;
+    t.pz += off;
+    t.cyc0 += off;
+    t.kamp += off;
+    t.q += off;
+    t.box += off;
+    t.n = s.n - off;
+    return t;
+}
+
+// split-K partial planes of one chunk of points
+size_t rs_part_bytes(const Geometry& g, int64_t npoints)
+{
+    const GemmPlan p = plan_gemm(g, npoints, rsgemm::RP);
+    return (size_t)p.splits * p.npx * sizeof(float2);
+}
+constexpr size_t MAX_RS_PART = size_t(4) << 30;
+}  // namespace
+
+int launch_rs_gemm(ohx_ctx* ctx, const Geometry& g, const PointSoA& soa, double* d_field, uint32_t mode)
+{
+    using namespace rsgemm;
+    const int64_t n = soa.n;
+    if (n <= 0) return OHX_OK;
+    if (n > INT_MAX) return set_error(ctx, OHX_ERR_INVALID_ARG, "too many points for the RS matrix-core path");
+    hipStream_t st = ctx->stream;
+    size_t cub_bytes = 0;
+    if (hipcub::DeviceScan::ExclusiveSum(nullptr, cub_bytes, (const unsigned*)nullptr, (unsigned*)nullptr, (int)n,
+                                         st) != hipSuccess)
+        return set_error(ctx, OHX_ERR_DEVICE, "hipcub scan size query failed");
+    // [count, 256 B] [flag] [pos] [scan temp] [partitioned point constants]
+    const size_t need = 256 + 2 * al256((size_t)n * sizeof(unsigned)) + al256(cub_bytes) + points_bytes(n);
+    if (need > ctx->rs_scratch_bytes) {
+        if (ctx->rs_scratch) {
+            int e = check_hip(ctx, hipStreamSynchronize(st), "hipStreamSynchronize");
+            if (e) return e;
+            (void)hipFree(ctx->rs_scratch);
+            ctx->rs_scratch = nullptr;
+            ctx->rs_scratch_bytes = 0;
+        }
+        int e = check_hip(ctx, hipMalloc(&ctx->rs_scratch, need), "hipMalloc(RS scratch)");
+        if (e) return e;
+        ctx->rs_scratch_bytes = need;
+    }
+    char* b = (char*)ctx->rs_scratch;
+    unsigned long long* d_count = (unsigned long long*)b;
+    unsigned* flag = (unsigned*)(b + 256);
+    unsigned* pos = (unsigned*)(b + 256 + al256((size_t)n * sizeof(unsigned)));
+    void* tmp = b + 256 + 2 * al256((size_t)n * sizeof(unsigned));
+    PointSoA part;
+    carve_points(b + 256 + 2 * al256((size_t)n * sizeof(unsigned)) + al256(cub_bytes), n, &part);
+
+    const int tiles_x = (g.pn_x + BN - 1) / BN, tiles_y = (g.pn_y + BM - 1) / BM;
+    const unsigned pgrid = (unsigned)((n + 255) / 256);
+    const int var = (mode & OHX_MODE_MFMA_BF16X3) ? V_BF16X3 : V_BF16X1;
+    hipLaunchKernelGGL(k_rs_classify, dim3(pgrid), dim3(256), 0, st, g, soa, tiles_x, tiles_y,
+                       var == V_BF16X3 ? MAX_THETA_X3 : MAX_THETA_X1, flag);
+    if (hipcub::DeviceScan::ExclusiveSum(tmp, cub_bytes, (const unsigned*)flag, pos, (int)n, st) != hipSuccess)
+        return set_error(ctx, OHX_ERR_DEVICE, "hipcub scan failed");
+    hipLaunchKernelGGL(k_rs_gather, dim3(pgrid), dim3(256), 0, st, soa, (const unsigned*)flag,
+                       (const unsigned*)pos, part, d_count);
+    int e = check_hip(ctx, hipGetLastError(), "k_rs_gather");
+    if (e) return e;
+    unsigned long long ne_dev = 0;
+    e = check_hip(ctx, hipMemcpyAsync(&ne_dev, d_count, sizeof(ne_dev), hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
+    if (!e) e = check_hip(ctx, hipStreamSynchronize(st), "hipStreamSynchronize");
+    if (e) return e;
+    const int64_t ne = (int64_t)ne_dev;
+
+    int64_t wgs = 0;
+    if (ne > 0) {
+        int64_t chunk = ne;
+        while (chunk > RP && rs_part_bytes(g, chunk) > MAX_RS_PART) chunk = (chunk + 1) / 2;
+        // the last, shorter chunk may plan more split planes than a full one
+        const size_t pbytes = std::max(rs_part_bytes(g, chunk), rs_part_bytes(g, ne - (ne - 1) / chunk * chunk));
+        if ((e = ensure_gemm_scratch(ctx, pbytes))) return e;
+        for (int64_t p0 = 0; p0 < ne; p0 += chunk) {
+            const int64_t p1 = std::min<int64_t>(ne, p0 + chunk);
+            const GemmPlan pl = plan_gemm(g, p1 - p0, RP);
+            if ((size_t)pl.splits * pl.npx * sizeof(float2) > ctx->gemm_scratch_bytes)
+                return set_error(ctx, OHX_ERR_INVALID_ARG, "RS matrix-core scratch too small");
+            const dim3 grid((unsigned)((int64_t)pl.tiles * pl.splits)), block(NT);
+            float2* pp = (float2*)ctx->gemm_scratch;
+            if (var == V_BF16X3)
+                hipLaunchKernelGGL(k_rs_gemm<V_BF16X3>, grid, block, 0, st, g, part, pp, pl.tiles_x, pl.tiles, p0, p1,
+                                   pl.kslice);
+            else
+                hipLaunchKernelGGL(k_rs_gemm<V_BF16X1>, grid, block, 0, st, g, part, pp, pl.tiles_x, pl.tiles, p0, p1,
+                                   pl.kslice);
+            if ((e = check_hip(ctx, hipGetLastError(), "k_rs_gemm"))) return e;
+            if ((e = launch_gemm_reduce(st, pp, pl.splits, pl.npx, d_field)))
+                return set_error(ctx, e, "k_gemm_reduce launch failed");
+            wgs += (int64_t)pl.tiles * pl.splits;
+        }
+    }
+    if (ne < n) {
+        if ((e = launch_direct_with(ctx, g, soa_tail(part, ne), d_field, false, ctx->minmax + 5, true))) return e;
+        wgs += ctx->last.workgroups;
+    }
+    ctx->last.workgroups = wgs;
+    ctx->last.kernel_kind = ne > 0 ? 2 : 0;
+    ctx->last.gemm_points = (int32_t)ne;
+    return OHX_OK;
+}
+
+}  // namespace ohx

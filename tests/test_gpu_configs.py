@@ -164,6 +164,43 @@ def test_4320p_partial_masks_vs_oracle(ctx, orc, diff):
             assert err_edge < tol, (ch, err_edge)
 
 
+def test_4320p_rs_matrix_core_vs_oracle(ctx, orc):
+    """2M-point workload geometry, first 2,000 points, all three channels, RS
+    with MODE_MFMA_BF16X3: points whose mask is the whole hologram and whose
+    tile expansion holds (most of them) run on the matrix cores, the rest
+    (partial masks, z near 0.3 m far off axis) on the FP32 direct kernel.
+    Same pixels as the direct-kernel case (random plus box edges)."""
+    w = synth.WORKLOADS[5]
+    v = synth.points(w, 2_000)
+    g = orc.make_geom(w.pn, w.pp)
+    mode, tol = MFMA["bf16x3"]
+    rng = np.random.default_rng(500)
+    for ch, lam in enumerate(w.lambdas):
+        edge, n_partial = _edge_pixels(orc, g, v, lam, 0, w.pn, rng)
+        pix = np.unique(np.concatenate([rng.choice(w.n_pixels, 2500, replace=False), edge]))
+        want = orc.field_exact(g, v, lam, ch, 0, pix)
+        npx = w.n_pixels
+        dv = ctx.to_device(v)
+        fld = ctx.alloc(16 * npx)
+        try:
+            ctx.field_zero(fld, npx)
+            ctx.diffract(dv, len(v), fld, capi.params(w.pn, w.pp, lam, channel=ch), 0, mode)
+            st = ctx.last_stats()
+            got = fld.download(np.complex128, npx)[pix]
+        finally:
+            dv.free()
+            fld.free()
+        err = rel_l2(got, want)
+        sel = np.isin(pix, edge)
+        err_edge = rel_l2(got[sel], want[sel]) if sel.any() else 0.0
+        print(f"ch={ch}: rel-L2 {err:.3e} (box-edge pixels {err_edge:.3e}), "
+              f"{st.gemm_points} of {len(v)} points on matrix cores")
+        assert 0.75 * len(v) < st.gemm_points < len(v) - n_partial + 1
+        assert err < tol, (ch, err)
+        if sel.any():
+            assert err_edge < tol, (ch, err_edge)
+
+
 @pytest.mark.parametrize("diff", [0, 1])
 @pytest.mark.parametrize("prec", [0, 1])
 def test_dice_example_end_to_end(tmp_path, orc, diff, prec):

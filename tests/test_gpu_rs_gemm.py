@@ -1,0 +1,131 @@
+"""GPU parity of the RS matrix-core path (rank-2 tile expansion on MFMA, the
+remaining points on the FP32 direct kernel), through the C ABI.
+
+Field relative-L2 tolerances against the FP64 oracle, as for the Fresnel
+matrix-core variants:
+  BF16 1-term : 1e-2
+  BF16 3-term : 1e-4
+ohx_last_launch_stats().gemm_points names how many points ran on the matrix
+cores; the tests check it against the eligibility each geometry implies
+(points whose mask is the whole hologram and whose tile expansion holds).
+"""
+import numpy as np
+import pytest
+
+from openholo_amd import capi, synth
+
+pytestmark = pytest.mark.gpu
+
+FP32 = capi.MODE_GPU | capi.MODE_FLOAT
+RS_MFMA = {
+    "bf16x1": (FP32 | capi.MODE_MFMA_BF16X1, 1e-2),
+    "bf16x3": (FP32 | capi.MODE_MFMA_BF16X3, 1e-4),
+}
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = capi.Context(0)
+    yield c
+    c.close()
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def run(ctx, v, pn, pp, lam, mode, channel=0, pix=None):
+    """One RS channel through ohx_pc_diffract: (field or field[pix], stats)."""
+    npx = pn[0] * pn[1]
+    dv = ctx.to_device(v)
+    fld = ctx.alloc(16 * npx)
+    try:
+        ctx.field_zero(fld, npx)
+        ctx.diffract(dv, len(v), fld, capi.params(pn, pp, lam, channel=channel), 0, mode)
+        st = ctx.last_stats()
+        f = fld.download(np.complex128, npx)
+        return (f if pix is None else f[pix]), st
+    finally:
+        dv.free()
+        fld.free()
+
+
+@pytest.mark.parametrize("name", list(RS_MFMA))
+def test_rs_gemm_2160p_rgb_vs_oracle(ctx, orc, name):
+    """100k-point workload geometry (3840x2160 at 3.74 um, x, y in +-8 mm, z in
+    [0.45, 0.55] m), first 10,000 points, each of 638/532/473 nm: every point's
+    mask is the whole hologram, so all of them run on the matrix cores; field
+    vs the oracle on 3,000 pixels."""
+    mode, tol = RS_MFMA[name]
+    w = synth.WORKLOADS[2]
+    v = synth.points(w, 10_000)
+    g = orc.make_geom(w.pn, w.pp)
+    pix = np.random.default_rng(500).choice(w.n_pixels, 3000, replace=False)
+    for ch, lam in enumerate(w.lambdas):
+        want = orc.field_exact(g, v, lam, ch, 0, pix)
+        got, st = run(ctx, v, w.pn, w.pp, lam, mode, ch, pix)
+        err = rel_l2(got, want)
+        print(f"{name} ch={ch}: rel-L2 {err:.3e}, {st.gemm_points} of {len(v)} points on matrix cores")
+        assert st.gemm_points == len(v) and st.kernel_kind == 2
+        assert err < tol, (ch, err)
+
+
+@pytest.mark.parametrize("name", list(RS_MFMA))
+@pytest.mark.parametrize("n", [1, 9, 1999])
+def test_rs_gemm_mixed_eligibility_vs_oracle(ctx, orc, name, n):
+    """300x170 at 8 um (not tile multiples), z in [0.15, 0.3] m: the nearest
+    points' tile expansion is out of bounds and they take the direct kernel,
+    the others the matrix cores; n = 1 and 9 leave partial 8-point stages."""
+    mode, tol = RS_MFMA[name]
+    pn, pp, lam = (300, 170), (8e-6, 8e-6), 532e-9
+    v = synth.random_cloud(31, 1999, 1.5e-3, (0.15, 0.3))[:n]
+    g = orc.make_geom(pn, pp)
+    pix = np.random.default_rng(2).choice(pn[0] * pn[1], 3000, replace=False)
+    want = orc.field_exact(g, v, lam, 0, 0, pix)
+    got, st = run(ctx, v, pn, pp, lam, mode, pix=pix)
+    err = rel_l2(got, want)
+    print(f"{name} n={n}: rel-L2 {err:.3e}, {st.gemm_points} on matrix cores")
+    assert err < tol
+    if n == 1999:
+        assert 0 < st.gemm_points < n
+
+
+@pytest.mark.parametrize("name", list(RS_MFMA))
+def test_rs_gemm_partial_masks_go_direct(ctx, orc, name):
+    """Points a few mm from the hologram have boxes a few pixels wide: they run
+    on the direct kernel (exact masks), the far points on the matrix cores.
+    Whole field vs the reference restatement."""
+    mode, tol = RS_MFMA[name]
+    pn, pp, lam = (150, 90), (10e-6, 10e-6), 633e-9
+    g = orc.make_geom(pn, pp, offset=(2, -3))
+    near = synth.random_cloud(41, 40, 500e-6, (1.5e-3, 4e-3))
+    far = synth.random_cloud(42, 300, 500e-6, (0.2, 0.3))
+    v = np.concatenate([near[:20], far, near[20:]])
+    want = orc.gen_restate(g, v, [lam], 0)[0]
+    got = capi.generate(ctx, v, pn, pp, [lam], 0, mode, offset=(2, -3))[0]
+    st = ctx.last_stats()
+    err = rel_l2(got, want)
+    print(f"{name}: rel-L2 {err:.3e}, {st.gemm_points} on matrix cores")
+    assert st.gemm_points == len(far)
+    assert err < tol
+    # the near points alone: no point on the matrix cores, bit-identical to FP32
+    a = capi.generate(ctx, near, pn, pp, [lam], 0, mode, offset=(2, -3))[0]
+    assert ctx.last_stats().gemm_points == 0
+    b = capi.generate(ctx, near, pn, pp, [lam], 0, FP32, offset=(2, -3))[0]
+    assert np.array_equal(a, b)
+
+
+def test_rs_gemm_deterministic_and_close_to_direct(ctx):
+    """Same input twice: bit-identical fields; and within 3e-5 of the FP32
+    direct kernel on the whole 1920x1080 field (10k-point RS workload)."""
+    w = synth.WORKLOADS[1]
+    v = synth.points(w, 4000)
+    mode = RS_MFMA["bf16x3"][0]
+    a, sa = run(ctx, v, w.pn, w.pp, w.lambdas[0], mode)
+    b, _ = run(ctx, v, w.pn, w.pp, w.lambdas[0], mode)
+    assert np.array_equal(a, b)
+    d, _ = run(ctx, v, w.pn, w.pp, w.lambdas[0], FP32)
+    err = rel_l2(a, d)
+    print(f"vs FP32 direct: rel-L2 {err:.3e}, {sa.gemm_points} of {len(v)} on matrix cores")
+    assert sa.gemm_points > 0
+    assert err < 3e-5

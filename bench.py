@@ -21,6 +21,8 @@ Reported beside it:
   * the 100k-point RGB RS + Fresnel workload (3840x2160, FP32 direct), the
     10k-point RS workload (1920x1080, FP32 and FP64) and one step of the
     2M-point RGB workload (7680x4320, RS FP32 direct + Fresnel FP16 3-term);
+  * RS on the matrix cores (BF16 1- and 3-term) beside the FP32 direct kernel
+    on the 100k-point and the 2M-point RGB workloads;
   * encode/normalise HBM rates, the WRP generator, the serving stream;
   * the CPU baseline (the reference's OpenMP CPU path, restated in oracle/,
     timed on a bounded sample on this host).
@@ -65,13 +67,18 @@ def parse():
     ap.add_argument("--kernel", default="bf16x3", choices=list(KERNELS))
     ap.add_argument("--fresnel-kernel", default=None, choices=list(KERNELS),
                     help="kernel for the Fresnel part of the step (default: --kernel)")
+    ap.add_argument("--rs-kernel", default=None, choices=list(KERNELS),
+                    help="kernel for the RS part of the step (default: --kernel if direct / fp64, else direct)")
     ap.add_argument("--variants", type=int, default=1,
                     help="also time the 1M-point Fresnel workload on every Fresnel kernel (0 = off)")
-    ap.add_argument("--blocks", default="rgb2160,rs1080,large4320",
+    ap.add_argument("--blocks", default="rgb2160,rs2160,rs1080,large4320,rs4320",
                     help="other workloads to time beside the headline (comma list, '' = none)")
     ap.add_argument("--serving", type=int, default=1,
                     help="also run the mixed Poisson request stream on the serving engine (0 = off)")
     ap.add_argument("--requests", type=int, default=120)
+    ap.add_argument("--serving-rs-kernel", default="bf16x3", choices=["direct", "bf16x1", "bf16x3"],
+                    help="kernel of the RS requests in the serving stream (bf16x3: matrix cores for the points "
+                         "whose tile expansion holds, FP32 direct for the rest)")
     ap.add_argument("--wrp", type=int, default=1, help="also time the WRP generator (0 = off)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run ohx_field_accumulate over two 3840x2160 FP64 fields (reads 265.4 MB, "
@@ -234,7 +241,66 @@ def fresnel_variants(ctx, capi, synth, Event, launches=2):
     return out
 
 
-def serving_stream(capi, synth, n_requests: int):
+def rs_variants(ctx, capi, synth, Event, wid, kernels, launches=2, direct_ms=None):
+    """RS on the matrix cores (rank-2 tile expansion of the RS term; points
+    outside its bounds run the FP32 direct kernel in the same launch) beside
+    the FP32 direct kernel: every channel of workload wid, pairs/s from HIP
+    events on the launch stream, field rel-L2 against the FP64 probe, and the
+    share of points that ran on the matrix cores.  direct_ms: the direct
+    kernel's ms per channel from another block, when "direct" is not timed
+    here."""
+    w = synth.WORKLOADS[wid]
+    n, npx, nch = w.n_points, w.n_pixels, len(w.lambdas)
+    dv = ctx.to_device(synth.points(w, n))
+    fld = ctx.alloc(16 * npx)
+    params = [capi.params(w.pn, w.pp, lam, channel=ch) for ch, lam in enumerate(w.lambdas)]
+    pix = np.random.default_rng(98).choice(npx, 256, replace=False)
+    refs = [ctx.probe_f64(dv, n, params[ch], 0, pix) for ch in range(nch)]
+    out = {"workload": w.name, "points": n, "resolution": list(w.pn), "channels": nch, "probe_pixels": len(pix)}
+    for name in kernels:
+        num = den = 0.0
+        on_mfma = 0
+        ts = []
+        for ch in range(nch):
+            ctx.field_zero(fld, npx)
+            ctx.diffract(dv, n, fld, params[ch], 0, KERNELS[name])    # warm-up, also the accuracy sample
+            ctx.sync()
+            on_mfma += ctx.last_stats().gemm_points
+            f = fld.download(np.complex128, npx)[pix]
+            num += float(np.sum(np.abs(f - refs[ch]) ** 2))
+            den += float(np.sum(np.abs(refs[ch]) ** 2))
+            for _ in range(launches):
+                a, b = Event(), Event()
+                a.record(ctx.stream)
+                ctx.diffract(dv, n, fld, params[ch], 0, KERNELS[name])
+                b.record(ctx.stream)
+                ts.append((a, b))
+            ctx.sync()
+        ms = float(np.mean([a.elapsed_ms(b) for a, b in ts]))
+        entry = {"ms_per_channel": round(ms, 2), "Gpairs_per_s": round(n * npx / (ms / 1e3) / 1e9, 1),
+                 "rel_l2_vs_fp64": float(f"{np.sqrt(num / den):.3e}")}
+        if name in MFMA_TERMS:
+            # two K entries per matrix-core point
+            tf = on_mfma / nch * npx * 16 * MFMA_TERMS[name] / (ms / 1e3) / 1e12
+            entry["matrix_core_point_fraction"] = round(on_mfma / (n * nch), 4)
+            entry["mfma_tflops"] = round(tf, 1)
+            entry["mfma_frac"] = round(tf / PEAK_BF16_TFLOPS, 4)
+        out[name] = entry
+        print(f"# rs {w.name} {name}: {ms:.1f} ms/ch, {entry['Gpairs_per_s']:.0f} Gpairs/s, "
+              f"rel-L2 {entry['rel_l2_vs_fp64']:.2e}", file=sys.stderr, flush=True)
+    d_ms = out["direct"]["ms_per_channel"] if "direct" in out else direct_ms
+    if d_ms:
+        if "direct" not in out:
+            out["direct_ms_per_channel"] = d_ms
+        for name in kernels:
+            if name in MFMA_TERMS:
+                out[name]["speedup_vs_direct"] = round(d_ms / out[name]["ms_per_channel"], 2)
+    dv.free()
+    fld.free()
+    return out
+
+
+def serving_stream(capi, synth, n_requests: int, rs_kernel: str = "bf16x3"):
     """Mixed request stream on the continuous-batching server.  Pass 1 runs
     every request alone (submit, wait) for its alone latency; pass 2 submits
     every request at once to measure capacity (jobs/s); pass 3 replays the
@@ -245,7 +311,7 @@ def serving_stream(capi, synth, n_requests: int):
     reqs = synth.serving_requests(n_requests)
     clouds = [synth.random_cloud(r["seed"], r["n"], 4e-3, (0.49, 0.51)) for r in reqs]
     fresnel_mode = KERNELS["bf16x3"]
-    rs_mode = KERNELS["direct"]
+    rs_mode = KERNELS[rs_kernel]
 
     def submit(srv, i):
         r = reqs[i]
@@ -325,7 +391,7 @@ def serving_stream(capi, synth, n_requests: int):
         "per_mode": per_mode,
         "iterations": int(c.iterations), "graph_replays": int(c.graph_replays),
         "peak_reserved_GB": round(c.peak_reserved_bytes / 1e9, 2),
-        "kernels": "RS: FP32 direct; Fresnel: BF16 3-term matrix cores",
+        "kernels": {"RS": rs_kernel, "Fresnel": "bf16x3"},
     }
 
 
@@ -567,7 +633,7 @@ def main():
     from openholo_amd.hip_events import Event
 
     w = synth.WORKLOADS[args.workload]
-    kernel_of = {0: args.kernel if args.kernel in ("direct", "fp64") else "direct",
+    kernel_of = {0: args.rs_kernel or (args.kernel if args.kernel in ("direct", "fp64") else "direct"),
                  1: args.fresnel_kernel or args.kernel}
     ctx = capi.Context(0)
     if args.copy_calibration:
@@ -620,6 +686,8 @@ def main():
             r["roofline_rs"] = direct_roofline(wl, b["n"], 0, b["launch_ms"][0])
         blocks["rgb_2160p"] = workload_block(ctx, capi, synth, Event, 2, {0: "direct", 1: "direct"}, 3, 1,
                                              args.probe_pixels, extra=rs_roofline)
+    if "rs2160" in wanted:
+        blocks["rs_2160p_matrix_core"] = rs_variants(ctx, capi, synth, Event, 2, ("direct", "bf16x1", "bf16x3"))
     if "rs1080" in wanted:
         blocks["rs_1080p"] = workload_block(ctx, capi, synth, Event, 1, {0: "direct"}, 20, 3, args.probe_pixels)
         blocks["rs_1080p_fp64"] = workload_block(ctx, capi, synth, Event, 1, {0: "fp64"}, 5, 1,
@@ -629,13 +697,17 @@ def main():
         # channel 0 only, to keep the run within minutes
         blocks["large_4320p"] = workload_block(ctx, capi, synth, Event, 5, {0: "direct", 1: "fp16x3"}, 1, 0, 256,
                                                accuracy_channels=[0])
+    if "rs4320" in wanted:
+        d_ms = blocks["large_4320p"]["per_mode"]["RS"]["ms_per_channel"] if "large_4320p" in blocks else None
+        blocks["rs_4320p_matrix_core"] = rs_variants(ctx, capi, synth, Event, 5, ("bf16x1", "bf16x3"), launches=1,
+                                                     direct_ms=d_ms)
     wrp = wrp_block(ctx, capi, synth, Event) if args.wrp else None
     if wrp:
         print(f"# wrp: {wrp}", file=sys.stderr, flush=True)
     ctx.close()
     serving = None
     if args.serving:
-        serving = serving_stream(capi, synth, args.requests)
+        serving = serving_stream(capi, synth, args.requests, args.serving_rs_kernel)
         print(f"# serving: p50 {serving['p50_ms']} ms, p99 {serving['p99_ms']} ms, "
               f"{serving['throughput_Gpairs_per_s']} Gpairs/s", file=sys.stderr, flush=True)
 

@@ -136,6 +136,15 @@ int launch_gemm_reduce(hipStream_t stream, const void* part, int64_t splits, int
 // RS on matrix cores (rs_gemm.hip): points whose separable rank-2 form holds on
 // every tile go to the GEMM, the others to the FP32 direct kernel
 int launch_rs_gemm(ohx_ctx* ctx, const Geometry& g, const PointSoA& soa, double* d_field, uint32_t mode);
+// its pieces, for callers that slice the work: the device-side partition of
+// soa into part (eligible points first, in order; *d_count = their number,
+// a device word inside scratch) and the GEMM over part[p0, p1) into d_field
+size_t rs_partition_scratch_bytes(int64_t n);
+int launch_rs_partition(hipStream_t st, void* scratch, size_t scratch_bytes, const Geometry& g, const PointSoA& soa,
+                        const PointSoA& part, uint32_t mode, unsigned long long** d_count);
+size_t rs_gemm_scratch_bytes(const Geometry& g, int64_t npoints);
+int launch_rs_gemm_range(hipStream_t st, void* scratch, size_t scratch_bytes, const Geometry& g,
+                         const PointSoA& part, int64_t p0, int64_t p1, double* d_field, uint32_t mode);
 int launch_probe(ohx_ctx* ctx, const Geometry& g, const PointSoA& soa, const int64_t* d_pix,
                  int64_t n_pix, double* d_out);
 

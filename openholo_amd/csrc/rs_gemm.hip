@@ -412,14 +412,71 @@ PointSoA soa_tail(const PointSoA& s, int64_t off)
     return t;
 }
 
-// split-K partial planes of one chunk of points
-size_t rs_part_bytes(const Geometry& g, int64_t npoints)
+size_t scan_temp_bytes(int64_t n)
+{
+    size_t b = 0;
+    if (hipcub::DeviceScan::ExclusiveSum(nullptr, b, (const unsigned*)nullptr, (unsigned*)nullptr, (int)n,
+                                         (hipStream_t)0) != hipSuccess)
+        return 0;
+    return b;
+}
+constexpr size_t MAX_RS_PART = size_t(4) << 30;
+}  // namespace
+
+// [count word, 256 B] [flag] [pos] [scan temp]
+size_t rs_partition_scratch_bytes(int64_t n)
+{
+    return 256 + 2 * al256((size_t)n * sizeof(unsigned)) + al256(scan_temp_bytes(n)) + 256;
+}
+
+int launch_rs_partition(hipStream_t st, void* scratch, size_t scratch_bytes, const Geometry& g, const PointSoA& soa,
+                        const PointSoA& part, uint32_t mode, unsigned long long** d_count)
+{
+    using namespace rsgemm;
+    const int64_t n = soa.n;
+    if (n <= 0 || n > INT_MAX || part.n != n) return OHX_ERR_INVALID_ARG;
+    size_t cub_bytes = scan_temp_bytes(n);
+    if (!cub_bytes || rs_partition_scratch_bytes(n) > scratch_bytes) return OHX_ERR_INVALID_ARG;
+    char* b = (char*)scratch;
+    *d_count = (unsigned long long*)b;
+    unsigned* flag = (unsigned*)(b + 256);
+    unsigned* pos = (unsigned*)(b + 256 + al256((size_t)n * sizeof(unsigned)));
+    void* tmp = b + 256 + 2 * al256((size_t)n * sizeof(unsigned));
+    const int tiles_x = (g.pn_x + BN - 1) / BN, tiles_y = (g.pn_y + BM - 1) / BM;
+    const unsigned pgrid = (unsigned)((n + 255) / 256);
+    const double max_theta = (mode & OHX_MODE_MFMA_BF16X3) ? MAX_THETA_X3 : MAX_THETA_X1;
+    hipLaunchKernelGGL(k_rs_classify, dim3(pgrid), dim3(256), 0, st, g, soa, tiles_x, tiles_y, max_theta, flag);
+    if (hipcub::DeviceScan::ExclusiveSum(tmp, cub_bytes, (const unsigned*)flag, pos, (int)n, st) != hipSuccess)
+        return OHX_ERR_DEVICE;
+    hipLaunchKernelGGL(k_rs_gather, dim3(pgrid), dim3(256), 0, st, soa, (const unsigned*)flag,
+                       (const unsigned*)pos, part, *d_count);
+    return hipGetLastError() == hipSuccess ? OHX_OK : OHX_ERR_DEVICE;
+}
+
+size_t rs_gemm_scratch_bytes(const Geometry& g, int64_t npoints)
 {
     const GemmPlan p = plan_gemm(g, npoints, rsgemm::RP);
     return (size_t)p.splits * p.npx * sizeof(float2);
 }
-constexpr size_t MAX_RS_PART = size_t(4) << 30;
-}  // namespace
+
+int launch_rs_gemm_range(hipStream_t st, void* scratch, size_t scratch_bytes, const Geometry& g,
+                         const PointSoA& part, int64_t p0, int64_t p1, double* d_field, uint32_t mode)
+{
+    using namespace rsgemm;
+    if (p1 <= p0) return OHX_OK;
+    const GemmPlan pl = plan_gemm(g, p1 - p0, RP);
+    if ((size_t)pl.splits * pl.npx * sizeof(float2) > scratch_bytes) return OHX_ERR_INVALID_ARG;
+    const dim3 grid((unsigned)((int64_t)pl.tiles * pl.splits)), block(NT);
+    float2* pp = (float2*)scratch;
+    if (mode & OHX_MODE_MFMA_BF16X3)
+        hipLaunchKernelGGL(k_rs_gemm<V_BF16X3>, grid, block, 0, st, g, part, pp, pl.tiles_x, pl.tiles, p0, p1,
+                           pl.kslice);
+    else
+        hipLaunchKernelGGL(k_rs_gemm<V_BF16X1>, grid, block, 0, st, g, part, pp, pl.tiles_x, pl.tiles, p0, p1,
+                           pl.kslice);
+    if (hipGetLastError() != hipSuccess) return OHX_ERR_DEVICE;
+    return launch_gemm_reduce(st, pp, pl.splits, pl.npx, d_field);
+}
 
 int launch_rs_gemm(ohx_ctx* ctx, const Geometry& g, const PointSoA& soa, double* d_field, uint32_t mode)
 {
@@ -428,12 +485,9 @@ int launch_rs_gemm(ohx_ctx* ctx, const Geometry& g, const PointSoA& soa, double*
     if (n <= 0) return OHX_OK;
     if (n > INT_MAX) return set_error(ctx, OHX_ERR_INVALID_ARG, "too many points for the RS matrix-core path");
     hipStream_t st = ctx->stream;
-    size_t cub_bytes = 0;
-    if (hipcub::DeviceScan::ExclusiveSum(nullptr, cub_bytes, (const unsigned*)nullptr, (unsigned*)nullptr, (int)n,
-                                         st) != hipSuccess)
-        return set_error(ctx, OHX_ERR_DEVICE, "hipcub scan size query failed");
-    // [count, 256 B] [flag] [pos] [scan temp] [partitioned point constants]
-    const size_t need = 256 + 2 * al256((size_t)n * sizeof(unsigned)) + al256(cub_bytes) + points_bytes(n);
+    // [partition scratch] [partitioned point constants]
+    const size_t pbytes = al256(rs_partition_scratch_bytes(n));
+    const size_t need = pbytes + points_bytes(n);
     if (need > ctx->rs_scratch_bytes) {
         if (ctx->rs_scratch) {
             int e = check_hip(ctx, hipStreamSynchronize(st), "hipStreamSynchronize");
@@ -446,25 +500,11 @@ int launch_rs_gemm(ohx_ctx* ctx, const Geometry& g, const PointSoA& soa, double*
         if (e) return e;
         ctx->rs_scratch_bytes = need;
     }
-    char* b = (char*)ctx->rs_scratch;
-    unsigned long long* d_count = (unsigned long long*)b;
-    unsigned* flag = (unsigned*)(b + 256);
-    unsigned* pos = (unsigned*)(b + 256 + al256((size_t)n * sizeof(unsigned)));
-    void* tmp = b + 256 + 2 * al256((size_t)n * sizeof(unsigned));
     PointSoA part;
-    carve_points(b + 256 + 2 * al256((size_t)n * sizeof(unsigned)) + al256(cub_bytes), n, &part);
-
-    const int tiles_x = (g.pn_x + BN - 1) / BN, tiles_y = (g.pn_y + BM - 1) / BM;
-    const unsigned pgrid = (unsigned)((n + 255) / 256);
-    const int var = (mode & OHX_MODE_MFMA_BF16X3) ? V_BF16X3 : V_BF16X1;
-    hipLaunchKernelGGL(k_rs_classify, dim3(pgrid), dim3(256), 0, st, g, soa, tiles_x, tiles_y,
-                       var == V_BF16X3 ? MAX_THETA_X3 : MAX_THETA_X1, flag);
-    if (hipcub::DeviceScan::ExclusiveSum(tmp, cub_bytes, (const unsigned*)flag, pos, (int)n, st) != hipSuccess)
-        return set_error(ctx, OHX_ERR_DEVICE, "hipcub scan failed");
-    hipLaunchKernelGGL(k_rs_gather, dim3(pgrid), dim3(256), 0, st, soa, (const unsigned*)flag,
-                       (const unsigned*)pos, part, d_count);
-    int e = check_hip(ctx, hipGetLastError(), "k_rs_gather");
-    if (e) return e;
+    carve_points((char*)ctx->rs_scratch + pbytes, n, &part);
+    unsigned long long* d_count = nullptr;
+    int e = launch_rs_partition(st, ctx->rs_scratch, pbytes, g, soa, part, mode, &d_count);
+    if (e) return set_error(ctx, e, "RS matrix-core partition failed");
     unsigned long long ne_dev = 0;
     e = check_hip(ctx, hipMemcpyAsync(&ne_dev, d_count, sizeof(ne_dev), hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
     if (!e) e = check_hip(ctx, hipStreamSynchronize(st), "hipStreamSynchronize");
@@ -474,26 +514,16 @@ int launch_rs_gemm(ohx_ctx* ctx, const Geometry& g, const PointSoA& soa, double*
     int64_t wgs = 0;
     if (ne > 0) {
         int64_t chunk = ne;
-        while (chunk > RP && rs_part_bytes(g, chunk) > MAX_RS_PART) chunk = (chunk + 1) / 2;
+        while (chunk > RP && rs_gemm_scratch_bytes(g, chunk) > MAX_RS_PART) chunk = (chunk + 1) / 2;
         // the last, shorter chunk may plan more split planes than a full one
-        const size_t pbytes = std::max(rs_part_bytes(g, chunk), rs_part_bytes(g, ne - (ne - 1) / chunk * chunk));
-        if ((e = ensure_gemm_scratch(ctx, pbytes))) return e;
+        const size_t gbytes =
+            std::max(rs_gemm_scratch_bytes(g, chunk), rs_gemm_scratch_bytes(g, ne - (ne - 1) / chunk * chunk));
+        if ((e = ensure_gemm_scratch(ctx, gbytes))) return e;
         for (int64_t p0 = 0; p0 < ne; p0 += chunk) {
             const int64_t p1 = std::min<int64_t>(ne, p0 + chunk);
+            e = launch_rs_gemm_range(st, ctx->gemm_scratch, ctx->gemm_scratch_bytes, g, part, p0, p1, d_field, mode);
+            if (e) return set_error(ctx, e, "RS matrix-core launch failed");
             const GemmPlan pl = plan_gemm(g, p1 - p0, RP);
-            if ((size_t)pl.splits * pl.npx * sizeof(float2) > ctx->gemm_scratch_bytes)
-                return set_error(ctx, OHX_ERR_INVALID_ARG, "RS matrix-core scratch too small");
-            const dim3 grid((unsigned)((int64_t)pl.tiles * pl.splits)), block(NT);
-            float2* pp = (float2*)ctx->gemm_scratch;
-            if (var == V_BF16X3)
-                hipLaunchKernelGGL(k_rs_gemm<V_BF16X3>, grid, block, 0, st, g, part, pp, pl.tiles_x, pl.tiles, p0, p1,
-                                   pl.kslice);
-            else
-                hipLaunchKernelGGL(k_rs_gemm<V_BF16X1>, grid, block, 0, st, g, part, pp, pl.tiles_x, pl.tiles, p0, p1,
-                                   pl.kslice);
-            if ((e = check_hip(ctx, hipGetLastError(), "k_rs_gemm"))) return e;
-            if ((e = launch_gemm_reduce(st, pp, pl.splits, pl.npx, d_field)))
-                return set_error(ctx, e, "k_gemm_reduce launch failed");
             wgs += (int64_t)pl.tiles * pl.splits;
         }
     }

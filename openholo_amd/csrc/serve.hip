@@ -151,6 +151,14 @@ struct SJob {
     uint32_t diff = 0, mode = 0;
     Geometry g{};
     bool mfma = false, fp64 = false;
+    // RS on the matrix cores: the setup partitions the points (eligible ones
+    // first, in order); GEMM slices cover part[0, ne), direct slices the rest
+    bool rsm = false;
+    PointSoA part{};
+    void* rs_tmp = nullptr;
+    size_t rs_tmp_bytes = 0;
+    int64_t ne = -1;                   // -1 until the setup has completed
+    int64_t slice_d = 0;               // direct slice of an RS matrix-core job
     double* out_field = nullptr;
     uint8_t* out_u8 = nullptr;
     // device: one block from the pool
@@ -239,19 +247,27 @@ static size_t job_bytes(const SJob& j)
     b += points_bytes(j.n) + 256;
     b += (size_t)npx * (16 + 8 + 1) + 3 * 256;
     if (j.mfma) b += fresnel_gemm_scratch_bytes(j.g, j.slice) + 256;
+    if (j.rsm)
+        b += rs_partition_scratch_bytes(j.n) + points_bytes(j.n) + rs_gemm_scratch_bytes(j.g, j.slice) + 3 * 256;
     return b;
 }
 
 static size_t stage_bytes(const SJob& j)
 {
     const size_t npx = (size_t)j.g.pn_x * j.g.pn_y;
-    return (j.out_field ? npx * 16 : 0) + (j.out_u8 ? (npx + 255) / 256 * 256 : 0);
+    // RS matrix-core jobs: one more word, the eligible-point count read back by the setup
+    return (j.out_field ? npx * 16 : 0) + (j.out_u8 ? (npx + 255) / 256 * 256 : 0) + (j.rsm ? 256 : 0);
+}
+static unsigned long long* rs_count(const SJob& j)
+{
+    return (unsigned long long*)(j.stage + j.stage_bytes - 256);
 }
 
-static int64_t slice_points(const SJob& j, const ohx_server_opts& o)
+// points per slice for `pairs` point-pixel pairs; RS on the matrix cores
+// takes half the matrix-core pairs per slice (two K entries per point)
+static int64_t slice_points(const SJob& j, double pairs)
 {
     const double npx = (double)j.g.pn_x * j.g.pn_y;
-    const double pairs = j.mfma ? o.slice_pairs_mfma : o.slice_pairs;
     int64_t s = (int64_t)(pairs / npx);
     s = std::max<int64_t>(s, 256);
     s = (s + DIRECT_CHUNK - 1) / DIRECT_CHUNK * DIRECT_CHUNK;   // whole LDS chunks of the direct kernel
@@ -277,12 +293,26 @@ static int setup_job(ohx_server* sv, SJob& j)
         j.gemm_bytes = fresnel_gemm_scratch_bytes(j.g, j.slice);
         j.gemm = take(j.gemm_bytes);
     }
+    if (j.rsm) {
+        j.gemm_bytes = rs_gemm_scratch_bytes(j.g, j.slice);
+        j.gemm = take(j.gemm_bytes);
+        j.rs_tmp_bytes = rs_partition_scratch_bytes(j.n);
+        j.rs_tmp = take(j.rs_tmp_bytes);
+        carve_points(take(points_bytes(j.n)), j.n, &j.part);
+    }
     int st = OHX_OK;
     if (j.n > 0) {
         if (hipMemcpyAsync(j.d_vtx, j.vtx.data(), (size_t)j.n * 56, hipMemcpyHostToDevice, sv->s_setup) !=
             hipSuccess)
             st = OHX_ERR_DEVICE;
         if (!st) st = launch_prep_on(sv->s_setup, j.d_vtx, j.g, j.soa);
+        if (!st && j.rsm) {
+            unsigned long long* dc = nullptr;
+            st = launch_rs_partition(sv->s_setup, j.rs_tmp, j.rs_tmp_bytes, j.g, j.soa, j.part, j.mode, &dc);
+            if (!st && hipMemcpyAsync(rs_count(j), dc, sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                                      sv->s_setup) != hipSuccess)
+                st = OHX_ERR_DEVICE;
+        }
     }
     if (!st) st = field_zero_on(sv->s_setup, j.field, npx);
     if (!st && !(j.ev_ready = get_event(sv))) st = OHX_ERR_DEVICE;
@@ -356,12 +386,27 @@ static void retire(ohx_server* sv, SJob* j, int st)
     sv->cv_done.notify_all();
 }
 
+// The job's next slice is a matrix-core slice
+static bool in_gemm(const SJob& j) { return j.mfma || (j.rsm && j.next < j.ne); }
+// End (exclusive) of the job's next slice
+static int64_t slice_end(const SJob& j)
+{
+    if (j.rsm) return j.next < j.ne ? std::min(j.ne, j.next + j.slice) : std::min(j.n, j.next + j.slice_d);
+    return std::min(j.n, j.next + j.slice);
+}
+static int64_t slices_left(const SJob& j)
+{
+    if (!j.rsm) return (j.n - j.next + j.slice - 1) / j.slice;
+    const int64_t g = j.next < j.ne ? (j.ne - j.next + j.slice - 1) / j.slice : 0;
+    return g + (j.n - std::max(j.next, j.ne) + j.slice_d - 1) / j.slice_d;
+}
 // Estimated GPU time of one slice (ms) from nominal rates of the kernels:
-// FP32 direct ~5e9 pairs/ms, FP64 direct ~1e9, matrix cores ~5e10.
+// FP32 direct ~5e9 pairs/ms, FP64 direct ~1e9, Fresnel matrix cores ~5e10,
+// RS matrix cores ~2.5e10.
 static double slice_ms(const SJob& j)
 {
-    const double pairs = (double)std::min(j.slice, j.n - j.next) * j.g.pn_x * j.g.pn_y;
-    return pairs / (j.mfma ? 5e10 : j.fp64 ? 1e9 : 5e9);
+    const double pairs = (double)(slice_end(j) - j.next) * j.g.pn_x * j.g.pn_y;
+    return pairs / (j.mfma ? 5e10 : in_gemm(j) ? 2.5e10 : j.fp64 ? 1e9 : 5e9);
 }
 
 // Enqueue one iteration over the jobs in act; true when something was
@@ -377,10 +422,16 @@ static bool enqueue_iteration(ohx_server* sv, const std::vector<SJob*>& act, int
     ohx_ctx* c = sv->ctx;
     hipStream_t s = c->stream;
     std::vector<SJob*> cand;
-    for (SJob* j : act)
-        if (j->state == J_ACTIVE && !j->cancel && j->next < j->n) cand.push_back(j);
+    for (SJob* j : act) {
+        if (j->state != J_ACTIVE || j->cancel || j->next >= j->n) continue;
+        if (j->rsm && j->ne < 0) {   // its split into GEMM and direct points is known once the setup is done
+            if (!j->ev_ready || hipEventQuery(j->ev_ready) != hipSuccess) continue;
+            j->ne = (int64_t)*rs_count(*j);
+        }
+        cand.push_back(j);
+    }
     if (cand.empty()) return false;
-    auto left = [](const SJob* j) { return (j->n - j->next + j->slice - 1) / j->slice; };
+    auto left = [](const SJob* j) { return slices_left(*j); };
     std::stable_sort(cand.begin(), cand.end(), [&](const SJob* a, const SJob* b) {
         return left(a) != left(b) ? left(a) < left(b) : a->id < b->id;
     });
@@ -390,9 +441,9 @@ static bool enqueue_iteration(ohx_server* sv, const std::vector<SJob*>& act, int
         const double t = slice_ms(*j);
         if (!gemm_jobs.empty() || !direct_jobs.empty())
             if (ms + t > ITER_BUDGET_MS) continue;
-        if (!j->mfma && (int)direct_jobs.size() == MAX_TABLE) continue;
+        if (!in_gemm(*j) && (int)direct_jobs.size() == MAX_TABLE) continue;
         ms += t;
-        (j->mfma ? gemm_jobs : direct_jobs).push_back(j);
+        (in_gemm(*j) ? gemm_jobs : direct_jobs).push_back(j);
     }
     auto wait_ready = [&](SJob* j) {
         if (!j->ev_ready) return true;
@@ -405,7 +456,7 @@ static bool enqueue_iteration(ohx_server* sv, const std::vector<SJob*>& act, int
         for (SJob* j : part) {
             if (j->st.iterations == 0) j->st.start_ms = t;
             ++j->st.iterations;
-            j->next = std::min(j->n, j->next + j->slice);
+            j->next = slice_end(*j);
             if (j->next >= j->n) {
                 j->state = J_FINISHING;
                 if (int st = enqueue_finish(sv, *j, ev)) return st;
@@ -421,8 +472,9 @@ static bool enqueue_iteration(ohx_server* sv, const std::vector<SJob*>& act, int
     for (SJob* j : gemm_jobs) {
         if (!wait_ready(j)) st = OHX_ERR_DEVICE;
         if (!st) {
-            const int64_t p0 = j->next, p1 = std::min(j->n, p0 + j->slice);
-            st = launch_fresnel_gemm_range(s, j->gemm, j->gemm_bytes, j->g, j->soa, p0, p1, j->field, j->mode);
+            const int64_t p0 = j->next, p1 = slice_end(*j);
+            st = j->mfma ? launch_fresnel_gemm_range(s, j->gemm, j->gemm_bytes, j->g, j->soa, p0, p1, j->field, j->mode)
+                         : launch_rs_gemm_range(s, j->gemm, j->gemm_bytes, j->g, j->part, p0, p1, j->field, j->mode);
         }
         if (st) break;
     }
@@ -440,10 +492,10 @@ static bool enqueue_iteration(ohx_server* sv, const std::vector<SJob*>& act, int
             }
             DirectJob& d = sv->h_table[set][nd++];
             d.g = j->g;
-            d.s = j->soa;
+            d.s = j->rsm ? j->part : j->soa;
             d.field = (double2*)j->field;
             d.p0 = j->next;
-            d.p1 = std::min(j->n, j->next + j->slice);
+            d.p1 = slice_end(*j);
             d.tiles = direct_tiles(j->g, j->fp64, &d.tiles_x);
             d.item_base = items;
             d.fp64 = j->fp64 ? 1 : 0;
@@ -669,10 +721,14 @@ int ohx_server_submit(ohx_server* sv, const ohx_job* job, uint64_t* id)
     j->mode = job->mode;
     const uint32_t mf = job->mode & (OHX_MODE_MFMA_BF16X1 | OHX_MODE_MFMA_BF16X3 | OHX_MODE_MFMA_FP16X3);
     j->mfma = mf && job->diff_flag == OHX_DIFF_FRESNEL;
+    // RS with a BF16 matrix-core mode, as ohx_pc_diffract runs it
+    j->rsm = mf && job->diff_flag == OHX_DIFF_RS && !(job->mode & OHX_MODE_MFMA_FP16X3) && j->n > 0;
     j->fp64 = !j->mfma && (job->mode & (OHX_MODE_FLOAT | OHX_MODE_FASTMATH)) == 0 && mf == 0;
     j->out_field = job->out_field;
     j->out_u8 = job->out_u8;
-    j->slice = slice_points(*j, sv->opts);
+    j->slice = slice_points(*j, j->mfma ? sv->opts.slice_pairs_mfma
+                                        : j->rsm ? 0.5 * sv->opts.slice_pairs_mfma : sv->opts.slice_pairs);
+    j->slice_d = slice_points(*j, sv->opts.slice_pairs);
     j->bytes = job_bytes(*j);
     j->stage_bytes = stage_bytes(*j);
     j->st.device_bytes = j->bytes;

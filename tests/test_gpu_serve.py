@@ -150,3 +150,25 @@ def test_cancel_pending_job():
         assert srv.wait(b, 60)[0] == capi.ERR_CANCELLED
         assert srv.wait(a, 300)[0] == capi.OK
         assert srv.cancel(12345) == capi.ERR_NOT_FOUND
+
+
+def test_rs_matrix_core_jobs():
+    """RS jobs with MODE_MFMA_BF16X3: the setup partitions the points on the
+    device (points with partial masks -- here the near ones -- to the direct
+    kernel); GEMM slices cover the others, then direct slices the rest.  The
+    field is within 2e-5 of the one-shot launch (which splits the same points
+    into different K slices) and bit-identical alone and in a mix."""
+    near = synth.random_cloud(71, 500, 2e-3, (2e-3, 4e-3))
+    far = synth.random_cloud(72, 6000, 4e-3, (0.49, 0.51))
+    v = np.concatenate([far[:3000], near, far[3000:]])
+    pn, pp, lam = (1920, 1080), (8e-6, 8e-6), 532e-9
+    p = capi.params(pn, pp, lam)
+    with capi.Server(slice_pairs=2e9, slice_pairs_mfma=5e9) as srv:
+        st, stats, alone, _ = srv.wait(srv.submit(v, p, 0, MFMA3, want_field=True), 120)
+        assert st == capi.OK and stats.iterations >= 3
+        ref = oneshot(v, p, 0, MFMA3)
+        assert rel_l2(alone, ref) < 2e-5
+        ids = [submit(srv, JOBS[0]), srv.submit(v, p, 0, MFMA3, want_field=True), submit(srv, JOBS[2])]
+        res = [srv.wait(j, 300) for j in ids]
+        assert all(r[0] == capi.OK for r in res)
+        assert np.array_equal(res[1][2], alone)

@@ -76,6 +76,8 @@ def parse():
     ap.add_argument("--serving", type=int, default=1,
                     help="also run the mixed Poisson request stream on the serving engine (0 = off)")
     ap.add_argument("--requests", type=int, default=120)
+    ap.add_argument("--only-serving-burst", action="store_true",
+                    help="run only the serving stream's burst pass and print its figures (for kernel traces)")
     ap.add_argument("--serving-rs-kernel", default="bf16x3", choices=["direct", "bf16x1", "bf16x3"],
                     help="kernel of the RS requests in the serving stream (bf16x3: matrix cores for the points "
                          "whose tile expansion holds, FP32 direct for the rest)")
@@ -298,6 +300,35 @@ def rs_variants(ctx, capi, synth, Event, wid, kernels, launches=2, direct_ms=Non
     dv.free()
     fld.free()
     return out
+
+
+def serving_burst(capi, synth, n_requests: int, rs_kernel: str = "bf16x3"):
+    """Only the serving stream's burst pass (every request submitted at once,
+    after one warm-up request of each kind), for a kernel trace of the
+    server's stream: the GPU's busy fraction while work is queued."""
+    reqs = synth.serving_requests(n_requests)
+    clouds = [synth.random_cloud(r["seed"], r["n"], 4e-3, (0.49, 0.51)) for r in reqs]
+    modes = {1: KERNELS["bf16x3"], 0: KERNELS[rs_kernel]}
+
+    def submit(srv, i):
+        r = reqs[i]
+        return srv.submit(clouds[i], capi.params(r["pn"], r["pp"], r["lambda"]), r["diff"], modes[r["diff"]],
+                          want_field=False, want_u8=True)
+
+    pairs = sum(r["n"] * r["pn"][0] * r["pn"][1] for r in reqs)
+    with capi.Server() as srv:
+        for i in range(min(4, len(reqs))):
+            srv.wait(submit(srv, i))
+        t0 = time.perf_counter()
+        ids = [submit(srv, i) for i in range(len(reqs))]
+        for j in ids:
+            st, *_ = srv.wait(j)
+            assert st == capi.OK
+        burst_s = time.perf_counter() - t0
+        c = srv.counters()
+    return {"serving_burst": {"requests": len(reqs), "seconds": round(burst_s, 4),
+                              "Gpairs_per_s": round(pairs / burst_s / 1e9, 1),
+                              "iterations": int(c.iterations), "kernels": {"RS": rs_kernel, "Fresnel": "bf16x3"}}}
 
 
 def serving_stream(capi, synth, n_requests: int, rs_kernel: str = "bf16x3"):
@@ -631,6 +662,9 @@ def main():
 
     from openholo_amd import capi, synth
     from openholo_amd.hip_events import Event
+    if args.only_serving_burst:
+        print(json.dumps(serving_burst(capi, synth, args.requests, args.serving_rs_kernel)), flush=True)
+        return
 
     w = synth.WORKLOADS[args.workload]
     kernel_of = {0: args.rs_kernel or (args.kernel if args.kernel in ("direct", "fp64") else "direct"),

@@ -166,10 +166,11 @@ __device__ inline void commit_minmax(unsigned long long kmin, unsigned long long
 // on the two result words; each thread keeps four 16-byte loads in flight.
 // Channel buffers that are not 16-byte aligned take the scalar loop.
 constexpr int MINMAX_BLOCKS = 512;
-// fused encode + min/max: the FP64 atan2 per pixel needs the latency hiding of
-// k_encode's full grid (2048 blocks ran at half its rate); one atomic pair per
-// block, 16k per launch at most
-constexpr int ENCODE_MM_BLOCKS = 8192;
+// Fused encode + min/max: the FP64 atan2 per pixel needs the latency hiding of
+// k_encode's full grid (2048 blocks ran at half its rate), ENCODE_MM_BLOCKS
+// blocks.  Each block stores its min/max keys to its own pair of words and one
+// block reduces them (k_mm_final): 16k 64-bit atomics on the two result words
+// serialise at the L2 and cost ~160 us per launch.
 
 __global__ __launch_bounds__(EB) void k_minmax(ChanPtrs cp, int nch, int64_t n, unsigned long long* mm)
 {
@@ -215,8 +216,70 @@ struct EncPtrs {
     double* e[MAXCH];
 };
 
+// Block-wide min/max of the order keys, stored to part[2 b], part[2 b + 1].
+__device__ inline void store_block_minmax(unsigned long long kmin, unsigned long long kmax,
+                                          unsigned long long* __restrict__ part)
+{
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long a = __shfl_xor(kmin, off);
+        const unsigned long long b = __shfl_xor(kmax, off);
+        kmin = a < kmin ? a : kmin;
+        kmax = b > kmax ? b : kmax;
+    }
+    __shared__ unsigned long long smin[EB / 64], smax[EB / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) {
+        smin[wave] = kmin;
+        smax[wave] = kmax;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < EB / 64; ++w) {
+            kmin = smin[w] < kmin ? smin[w] : kmin;
+            kmax = smax[w] > kmax ? smax[w] : kmax;
+        }
+        part[2 * blockIdx.x] = kmin;
+        part[2 * blockIdx.x + 1] = kmax;
+    }
+}
+
+// mm[0] / mm[1] = min / max over the nb blocks' keys (one block)
+__global__ __launch_bounds__(EB) void k_mm_final(const unsigned long long* __restrict__ part, int nb,
+                                                 unsigned long long* __restrict__ mm)
+{
+    unsigned long long kmin = ~0ull, kmax = 0ull;
+    for (int b = threadIdx.x; b < nb; b += EB) {
+        const unsigned long long a = part[2 * b], c = part[2 * b + 1];
+        kmin = a < kmin ? a : kmin;
+        kmax = c > kmax ? c : kmax;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long a = __shfl_xor(kmin, off);
+        const unsigned long long b = __shfl_xor(kmax, off);
+        kmin = a < kmin ? a : kmin;
+        kmax = b > kmax ? b : kmax;
+    }
+    __shared__ unsigned long long smin[EB / 64], smax[EB / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) {
+        smin[wave] = kmin;
+        smax[wave] = kmax;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < EB / 64; ++w) {
+            kmin = smin[w] < kmin ? smin[w] : kmin;
+            kmax = smax[w] > kmax ? smax[w] : kmax;
+        }
+        // merged into k_minmax_init's starting keys, as k_minmax's atomics are
+        const unsigned long long k0 = order_key(DBL_MAX), k1 = order_key(DBL_MIN);
+        mm[0] = kmin < k0 ? kmin : k0;
+        mm[1] = kmax > k1 ? kmax : k1;
+    }
+}
+
 template <int FLAG>
-__global__ __launch_bounds__(EB) void k_encode_mm(EncPtrs ep, int nch, int64_t n, unsigned long long* mm)
+__global__ __launch_bounds__(EB) void k_encode_mm(EncPtrs ep, int nch, int64_t n, unsigned long long* part)
 {
     unsigned long long kmin = ~0ull, kmax = 0ull;
     const int64_t tid = (int64_t)blockIdx.x * EB + threadIdx.x;
@@ -239,7 +302,7 @@ __global__ __launch_bounds__(EB) void k_encode_mm(EncPtrs ep, int nch, int64_t n
             minmax_upd(v, kmin, kmax);
         }
     }
-    commit_minmax(kmin, kmax, mm);
+    store_block_minmax(kmin, kmax, part);
 }
 
 __device__ inline uint8_t to_u8(double v, double mn, double gap)
@@ -358,7 +421,8 @@ extern "C" int ohx_normalize_u8(ohx_ctx* c, const double* const* d_enc, int nch,
 
 namespace ohx {
 
-int launch_encode_normalize_u8(hipStream_t stream, unsigned long long* mm, const double* const* d_field, int nch,
+int launch_encode_normalize_u8(hipStream_t stream, unsigned long long* mm, unsigned long long* part,
+                               const double* const* d_field, int nch,
                                int64_t n, uint32_t flag, double* const* d_enc, uint8_t* const* d_out)
 {
     EncPtrs ep{};
@@ -369,22 +433,23 @@ int launch_encode_normalize_u8(hipStream_t stream, unsigned long long* mm, const
         cp.p[i] = d_enc[i];
         cp.o[i] = d_out[i];
     }
-    hipLaunchKernelGGL(k_minmax_init, dim3(1), dim3(64), 0, stream, mm);
-    const dim3 g(std::min<unsigned>(egrid(n), ENCODE_MM_BLOCKS)), b(EB);
+    const unsigned nb = std::min<unsigned>(egrid(n), ENCODE_MM_BLOCKS);
+    const dim3 g(nb), b(EB);
     switch (flag) {
     case OHX_ENCODE_PHASE:
-        hipLaunchKernelGGL(k_encode_mm<OHX_ENCODE_PHASE>, g, b, 0, stream, ep, nch, n, mm);
+        hipLaunchKernelGGL(k_encode_mm<OHX_ENCODE_PHASE>, g, b, 0, stream, ep, nch, n, part);
         break;
     case OHX_ENCODE_AMPLITUDE:
-        hipLaunchKernelGGL(k_encode_mm<OHX_ENCODE_AMPLITUDE>, g, b, 0, stream, ep, nch, n, mm);
+        hipLaunchKernelGGL(k_encode_mm<OHX_ENCODE_AMPLITUDE>, g, b, 0, stream, ep, nch, n, part);
         break;
     case OHX_ENCODE_REAL:
-        hipLaunchKernelGGL(k_encode_mm<OHX_ENCODE_REAL>, g, b, 0, stream, ep, nch, n, mm);
+        hipLaunchKernelGGL(k_encode_mm<OHX_ENCODE_REAL>, g, b, 0, stream, ep, nch, n, part);
         break;
     default:
-        hipLaunchKernelGGL(k_encode_mm<OHX_ENCODE_IMAGINARY>, g, b, 0, stream, ep, nch, n, mm);
+        hipLaunchKernelGGL(k_encode_mm<OHX_ENCODE_IMAGINARY>, g, b, 0, stream, ep, nch, n, part);
         break;
     }
+    hipLaunchKernelGGL(k_mm_final, dim3(1), dim3(EB), 0, stream, (const unsigned long long*)part, (int)nb, mm);
     hipLaunchKernelGGL(k_to_u8, dim3(egrid((n + 3) / 4)), dim3(EB), 0, stream, cp, nch, n,
                        (const unsigned long long*)mm);
     return hipGetLastError() == hipSuccess ? OHX_OK : OHX_ERR_DEVICE;
@@ -406,7 +471,8 @@ extern "C" int ohx_encode_normalize_u8(ohx_ctx* c, const double* const* d_field,
     if (n == 0) return OHX_OK;
     std::lock_guard<std::mutex> lk(c->mu);
     (void)hipSetDevice(c->device);
-    if (launch_encode_normalize_u8(c->stream, c->minmax, d_field, nch, n, flag, d_enc, d_out) != OHX_OK)
+    if (launch_encode_normalize_u8(c->stream, c->minmax, c->minmax + 8, d_field, nch, n, flag, d_enc, d_out) !=
+        OHX_OK)
         return check_hip(c, hipGetLastError(), "k_encode_normalize");
     return OHX_OK;
 }

@@ -212,7 +212,7 @@ int ohx_ctx_create(int device, ohx_ctx** out)
         hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&c->copy_ready, hipEventDisableTiming) != hipSuccess ||
-        hipMalloc(&c->minmax, 8 * sizeof(unsigned long long)) != hipSuccess) {
+        hipMalloc(&c->minmax, (8 + ENCODE_MM_PART_WORDS) * sizeof(unsigned long long)) != hipSuccess) {
         if (c->copy_ready) (void)hipEventDestroy(c->copy_ready);
         if (c->copy_stream) (void)hipStreamDestroy(c->copy_stream);
         if (c->stream) (void)hipStreamDestroy(c->stream);

@@ -21,7 +21,8 @@ struct ohx_ctx {
     void* pts = nullptr;
     size_t pts_bytes = 0;
     // small device scratch: [0..1] min/max of the normaliser, [4] pixels in
-    // the mask boxes of the last launch, [5] FP32 direct kernel left EXACT pairs
+    // the mask boxes of the last launch, [5] FP32 direct kernel left EXACT pairs,
+    // [8 ..] the fused encoder's per-block min/max (ENCODE_MM_PART_WORDS)
     unsigned long long* minmax = nullptr;
     // split-K partial sums of the matrix-core Fresnel path and the point-split
     // partial planes of the direct kernels (grown on demand)
@@ -110,8 +111,12 @@ int launch_prep_on(hipStream_t stream, const void* d_vertices, const Geometry& g
 // fill kernel on a given stream (ohx_core.hip)
 int field_zero_on(hipStream_t stream, double* f, int64_t n_px);
 // encode every channel, min/max in the same pass, then 8-bit (encode.hip);
-// mm: two device words of scratch
-int launch_encode_normalize_u8(hipStream_t stream, unsigned long long* mm, const double* const* d_field, int nch,
+// mm: two device words (the min/max keys), part: ENCODE_MM_PART_WORDS device
+// words (per-block min/max keys of the encode pass)
+constexpr int ENCODE_MM_BLOCKS = 8192;
+constexpr int ENCODE_MM_PART_WORDS = 2 * ENCODE_MM_BLOCKS;
+int launch_encode_normalize_u8(hipStream_t stream, unsigned long long* mm, unsigned long long* part,
+                               const double* const* d_field, int nch,
                                int64_t n, uint32_t flag, double* const* d_enc, uint8_t* const* d_out);
 int launch_direct(ohx_ctx* ctx, const Geometry& g, const PointSoA& soa, double* d_field, bool fp64);
 // flag: device word for the FP32 kernel's "EXACT pairs left" flag;

@@ -201,7 +201,7 @@ struct ohx_server {
     std::chrono::steady_clock::time_point t0;
     hipStream_t s_setup = nullptr, s_finish = nullptr;   // compute: ctx->stream
     ohx::Pool pool, host_pool;
-    unsigned long long* d_mm = nullptr;   // min/max words of the finish stream's normaliser
+    unsigned long long* d_mm = nullptr;   // finish stream's normaliser: min/max words, then per-block keys
     // direct-sum job tables, one per iteration in flight
     ohx::DirectJob* h_table[ohx::DEPTH] = {};
     int* h_njobs[ohx::DEPTH] = {};
@@ -338,7 +338,7 @@ static int enqueue_finish(ohx_server* sv, SJob& j, hipEvent_t ev)
         const double* f = j.field;
         double* e = j.enc;
         uint8_t* o = j.u8;
-        st = launch_encode_normalize_u8(s, sv->d_mm, &f, 1, npx, OHX_ENCODE_PHASE, &e, &o);
+        st = launch_encode_normalize_u8(s, sv->d_mm, sv->d_mm + 8, &f, 1, npx, OHX_ENCODE_PHASE, &e, &o);
         if (!st && hipMemcpyAsync(stage, j.u8, npx, hipMemcpyDeviceToHost, s) != hipSuccess) st = OHX_ERR_DEVICE;
     }
     if (!st && hipEventRecord(j.ev_last, s) != hipSuccess) st = OHX_ERR_DEVICE;
@@ -680,7 +680,7 @@ int ohx_server_create(const ohx_server_opts* opts, ohx_server** out)
     sv->opts = o;
     bool ok = hipStreamCreateWithFlags(&sv->s_setup, hipStreamNonBlocking) == hipSuccess &&
               hipStreamCreateWithFlags(&sv->s_finish, hipStreamNonBlocking) == hipSuccess &&
-              hipMalloc((void**)&sv->d_mm, 256) == hipSuccess;
+              hipMalloc((void**)&sv->d_mm, (8 + ENCODE_MM_PART_WORDS) * sizeof(unsigned long long)) == hipSuccess;
     for (int s = 0; ok && s < DEPTH; ++s) {
         ok = hipHostMalloc((void**)&sv->h_table[s], sizeof(DirectJob) * MAX_TABLE, hipHostMallocDefault) ==
                  hipSuccess &&

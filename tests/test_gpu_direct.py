@@ -286,7 +286,7 @@ def test_encode_and_normalize_match_reference(ctx, orc):
 
 
 @pytest.mark.parametrize("flag", [0, 1, 2, 3])
-@pytest.mark.parametrize("npx,nch", [(1, 1), (7, 2), (70_001, 3), (1 << 20, 3)])
+@pytest.mark.parametrize("npx,nch", [(1, 1), (7, 2), (70_001, 3), (1 << 20, 3), (3_000_001, 1)])
 def test_fused_encode_normalize_equals_two_calls(ctx, orc, flag, npx, nch):
     """ohx_encode_normalize_u8 (encode pass that also takes the min/max) writes
     exactly what ohx_encode per channel followed by ohx_normalize_u8 writes:
@@ -500,3 +500,25 @@ def test_d2h_after_overlaps_next_launch(ctx):
         host.free()
         for b in (dv, dvl, f0, f1):
             b.free()
+
+
+def test_fused_encode_normalize_one_signed_range(ctx):
+    """Encoded values all negative (real part of a field with negative real
+    parts) or all NaN: the min/max keys start from the reference normaliser's
+    DBL_MAX / DBL_MIN in both the fused and the two-call path, so the 8-bit
+    planes agree bit for bit there too."""
+    npx = 100_000
+    rng = np.random.default_rng(9)
+    cases = [-(np.abs(rng.normal(size=npx)) + 0.1) + 1j * rng.normal(size=npx),
+             np.full(npx, complex(float("nan"), float("nan")))]
+    for f in cases:
+        df = ctx.to_device(f.astype(np.complex128))
+        e1, u1, e2, u2 = ctx.alloc(8 * npx), ctx.alloc(npx), ctx.alloc(8 * npx), ctx.alloc(npx)
+        try:
+            ctx.encode(df, npx, e1, capi.ENCODE_REAL)
+            ctx.normalize_u8([e1], npx, [u1])
+            ctx.encode_normalize_u8([df], npx, capi.ENCODE_REAL, [e2], [u2])
+            assert np.array_equal(u1.download(np.uint8, npx), u2.download(np.uint8, npx))
+        finally:
+            for b in (df, e1, u1, e2, u2):
+                b.free()

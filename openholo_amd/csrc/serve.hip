@@ -7,11 +7,16 @@
 // so a warm server allocates nothing per job.
 //
 // Three streams, all at the default priority:
-//   * setup  : an admitted job's vertex upload, per-point prep and field zero,
-//              then an event its first slice waits for (no host sync);
+//   * setup  : an admitted job's vertex upload, per-point prep and field zero
+//              (RS with a BF16 matrix-core mode: also the partition of its
+//              points into matrix-core and direct ones, whose count is copied
+//              to page-locked memory), then an event its first slice waits for
+//              (no host sync; an RS matrix-core job is scheduled once that
+//              event has completed);
 //   * compute: iterations.  One iteration advances a set of jobs by one slice
 //              each, chosen shortest remaining work first up to a GPU-time
-//              budget: the GEMM slices of the matrix-core Fresnel jobs, then
+//              budget: the GEMM slices of the matrix-core jobs (Fresnel, and
+//              RS over its matrix-core points before its direct ones), then
 //              ONE persistent kernel, k_pc_direct_jobs, striding over the
 //              (job, tile) items of a table of the direct-sum jobs (RS,
 //              Fresnel FP32/FP64), copied from pinned host memory (copy +

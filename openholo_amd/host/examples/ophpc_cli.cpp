@@ -8,7 +8,8 @@
 // Extensions used by the test suite:
 //   -o DIR            output directory (default Result)
 //   --vtx FILE        raw 56-byte Vertex records instead of -i
-//   --mfma bf16x1|bf16x3|fp16x3   matrix-core Fresnel variant
+//   --mfma bf16x1|bf16x3|fp16x3   matrix-core variant (Fresnel: all three; RS: the BF16
+//                                 ones, for the points whose mask is the whole hologram)
 //   --dump-field F    raw complex128 fields of all channels (row-major)
 //   --dump-enc F      raw float64 encoded buffers
 //   --dump-u8 F       raw uint8 normalised buffers

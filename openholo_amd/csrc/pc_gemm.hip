@@ -340,8 +340,11 @@ __device__ inline bool stage_body(const unsigned* __restrict__ rv, const unsigne
 // st+1 (buffer RB^1) from R in one basic block, so that they interleave; R
 // then receives stage st+2's point data.  Past the last stage the generation
 // runs on clamped, zero-amplitude data into the unused buffer: the step stays
-// branch-free, which keeps the accumulators in place.
-template <int VAR, int RB>
+// branch-free, which keeps the accumulators in place.  MASK: some point of
+// the launch has a box smaller than the hologram, so waves holding a point
+// whose box does not cover the tile zero its out-of-box elements (a ballot
+// and a branch per stage, ~5 % of the kernel, paid only by such launches).
+template <int VAR, int RB, bool MASK>
 __device__ inline void step(Smem<Traits<VAR>::three>& sm, f32x16 (&acc)[4][2], Raw& R, const PointSoA& s,
                             const float4* pconst, int64_t kfirst, int st, int64_t kb, int64_t ke, double ythr,
                             double xthr, double dy, double dx, int x0, int y0, int p, int grp, int lrow, int koff,
@@ -352,19 +355,21 @@ __device__ inline void step(Smem<Traits<VAR>::three>& sm, f32x16 (&acc)[4][2], R
                                          &sm.u[RB ^ 1][0][0][0], acc, P, R, s, pconst, kfirst,
                                          kb + (int64_t)(st + 2) * KP + p, ke, ythr, xthr, dy, dx, x0, y0, p, grp,
                                          lrow, koff, ucolx);
-    if (__builtin_amdgcn_ballot_w64(partial)) mask_stage<VAR, RB ^ 1>(sm, P, p, grp);
+    if constexpr (MASK) {
+        if (__builtin_amdgcn_ballot_w64(partial)) mask_stage<VAR, RB ^ 1>(sm, P, p, grp);
+    } else {
+        (void)partial;
+    }
     __syncthreads();
 }
 
-// pconst: per-point constants of points kfirst .. klast-1 (k_gemm_consts)
-template <int VAR>
-__global__ __launch_bounds__(NT, 2) void k_fresnel_gemm(Geometry g, PointSoA s, const float4* __restrict__ pconst,
-                                                        float2* __restrict__ part, int tiles_x, int tiles,
-                                                        int64_t kfirst, int64_t klast, int64_t kslice)
+// One workgroup's tile and split: prologue, K loop, FP32 partials.
+template <int VAR, bool MASK>
+__device__ __attribute__((always_inline)) inline void fresnel_tile(Smem<Traits<VAR>::three>& sm, const Geometry& g,
+                                                                   const PointSoA& s, const float4* pconst,
+                                                                   float2* part, int tiles_x, int tiles,
+                                                                   int64_t kfirst, int64_t klast, int64_t kslice)
 {
-    constexpr bool THREE = Traits<VAR>::three;
-    __shared__ __align__(16) Smem<THREE> sm;
-
     const int tile = blockIdx.x % tiles;
     const int split = blockIdx.x / tiles;
     const int tx = tile % tiles_x, ty = tile / tiles_x;
@@ -388,7 +393,11 @@ __global__ __launch_bounds__(NT, 2) void k_fresnel_gemm(Geometry g, PointSoA s, 
         Pt P;
         const bool partial = prep(P, R, ythr, xthr, dy, dx, x0, y0);
         gen_stage<VAR, 0>(sm, P, p, grp);
-        if (partial) mask_stage<VAR, 0>(sm, P, p, grp);
+        if constexpr (MASK) {
+            if (partial) mask_stage<VAR, 0>(sm, P, p, grp);
+        } else {
+            (void)partial;
+        }
         load_raw(R, s, pconst, kfirst, kb + KP + p, ke);
     }
     __syncthreads();
@@ -396,14 +405,29 @@ __global__ __launch_bounds__(NT, 2) void k_fresnel_gemm(Geometry g, PointSoA s, 
 #pragma unroll
     for (int i = 0; i < 4; ++i) acc[i][0] = acc[i][1] = f32x16{};
     for (int st = 0; st < nst; st += 2) {
-        step<VAR, 0>(sm, acc, R, s, pconst, kfirst, st, kb, ke, ythr, xthr, dy, dx, x0, y0, p, grp, lrow, koff,
-                     ucolx);
+        step<VAR, 0, MASK>(sm, acc, R, s, pconst, kfirst, st, kb, ke, ythr, xthr, dy, dx, x0, y0, p, grp, lrow,
+                           koff, ucolx);
         if (st + 1 >= nst) break;
-        step<VAR, 1>(sm, acc, R, s, pconst, kfirst, st + 1, kb, ke, ythr, xthr, dy, dx, x0, y0, p, grp, lrow, koff,
-                     ucolx);
+        step<VAR, 1, MASK>(sm, acc, R, s, pconst, kfirst, st + 1, kb, ke, ythr, xthr, dy, dx, x0, y0, p, grp, lrow,
+                           koff, ucolx);
     }
 
     store_partials(g, acc, part, split, x0, y0, ucolx, lane);
+}
+
+// pconst: per-point constants of points kfirst .. klast-1 (k_gemm_consts).
+// Both MASK instances are launched; the one that does not match
+// *partial_flag (!= 0: some point has a box smaller than the hologram) returns
+// at once, so each keeps its own register allocation.
+template <int VAR, bool MASK>
+__global__ __launch_bounds__(NT, 2) void k_fresnel_gemm(Geometry g, PointSoA s, const float4* __restrict__ pconst,
+                                                        const unsigned* __restrict__ partial_flag,
+                                                        float2* __restrict__ part, int tiles_x, int tiles,
+                                                        int64_t kfirst, int64_t klast, int64_t kslice)
+{
+    __shared__ __align__(16) Smem<Traits<VAR>::three> sm;
+    if ((__builtin_amdgcn_readfirstlane(partial_flag[0]) != 0) != MASK) return;
+    fresnel_tile<VAR, MASK>(sm, g, s, pconst, part, tiles_x, tiles, kfirst, klast, kslice);
 }
 
 // field += -j * amax * sum_split S   (FP64, fixed summation order)
@@ -438,27 +462,40 @@ __global__ __launch_bounds__(256) void k_amp_max(const double* __restrict__ kamp
     if ((threadIdx.x & 63) == 0 && m > 0.f) atomicMax((unsigned*)amax, __float_as_uint(m));
 }
 
+// amax[0] = 0 (max |A|), amax[1] = 0 as an unsigned word (the partial-box flag)
 __global__ void k_amp_init(float* amax)
 {
-    if (threadIdx.x == 0) amax[0] = 0.f;
+    if (threadIdx.x == 0) {
+        amax[0] = 0.f;
+        reinterpret_cast<unsigned*>(amax)[1] = 0u;
+    }
 }
 
 // Per-point constants of the stage generation, once per launch instead of once
 // per (point, tile, thread): the k^2 coefficients v2 = frac(q dy^2) and
 // u2 = frac(q dx^2) (dy = -NGRP pp_y, dx = NGRP pp_x; FP64 then reduced mod 1
-// and rounded) and the amplitude a/(lambda z), times 1/max|A| for FP16.
+// and rounded) and the amplitude a/(lambda z), times 1/max|A| for FP16.  Also
+// raises *partial_flag when a point's box is smaller than the hologram (one
+// atomic per wave holding such a point, skipped once the flag reads as set).
 __global__ __launch_bounds__(256) void k_gemm_consts(Geometry g, PointSoA s, int64_t p0, int64_t p1,
                                                      const float* __restrict__ amax, int scaled,
-                                                     float4* __restrict__ pconst)
+                                                     float4* __restrict__ pconst, unsigned* partial_flag)
 {
     const int64_t i = p0 + (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= p1) return;
-    const double dy = -NGRP * g.pp_y, dx = NGRP * g.pp_x;
-    const double q = s.q[i];
-    const float am = amax[0];
-    const float vscale = (scaled && am > 0.f) ? 1.0f / am : 1.0f;
-    pconst[i - p0] = make_float4((float)frac64(q * (dy * dy)), (float)frac64(q * (dx * dx)),
-                                 (float)s.kamp[i] * vscale, 0.f);
+    bool partial = false;
+    if (i < p1) {
+        const double dy = -NGRP * g.pp_y, dx = NGRP * g.pp_x;
+        const double q = s.q[i];
+        const float am = amax[0];
+        const float vscale = (scaled && am > 0.f) ? 1.0f / am : 1.0f;
+        pconst[i - p0] = make_float4((float)frac64(q * (dy * dy)), (float)frac64(q * (dx * dx)),
+                                     (float)s.kamp[i] * vscale, 0.f);
+        const int4 b = s.box[i];
+        partial = b.x > 0 || b.y < g.pn_x || b.z > 0 || b.w < g.pn_y;
+    }
+    if (__builtin_amdgcn_ballot_w64(partial) && (threadIdx.x & 63) == 0 &&
+        __atomic_load_n(partial_flag, __ATOMIC_RELAXED) == 0u)
+        atomicOr(partial_flag, 1u);
 }
 
 }  // namespace gemm
@@ -487,21 +524,29 @@ int launch_fresnel_gemm_range(hipStream_t stream, void* scratch, size_t scratch_
     float2* part = (float2*)((char*)scratch + 256 + consts_bytes(p1 - p0));
     hipLaunchKernelGGL(k_amp_init, dim3(1), dim3(64), 0, stream, amax);
     hipLaunchKernelGGL(k_amp_max, dim3(256), dim3(256), 0, stream, (const double*)soa.kamp, p0, p1, amax);
+    unsigned* partial_flag = reinterpret_cast<unsigned*>(amax) + 1;
     hipLaunchKernelGGL(k_gemm_consts, dim3((unsigned)((p1 - p0 + 255) / 256)), dim3(256), 0, stream, g, soa, p0,
-                       p1, (const float*)amax, var == V_FP16X3 ? 1 : 0, pconst);
+                       p1, (const float*)amax, var == V_FP16X3 ? 1 : 0, pconst, partial_flag);
     const dim3 grid((unsigned)((int64_t)pl.tiles * pl.splits)), block(NT);
+    const unsigned* pf = partial_flag;
     switch (var) {
     case V_BF16X1:
-        hipLaunchKernelGGL(k_fresnel_gemm<V_BF16X1>, grid, block, 0, stream, g, soa, (const float4*)pconst, part,
-                           pl.tiles_x, pl.tiles, p0, p1, pl.kslice);
+        hipLaunchKernelGGL((k_fresnel_gemm<V_BF16X1, false>), grid, block, 0, stream, g, soa, (const float4*)pconst,
+                           pf, part, pl.tiles_x, pl.tiles, p0, p1, pl.kslice);
+        hipLaunchKernelGGL((k_fresnel_gemm<V_BF16X1, true>), grid, block, 0, stream, g, soa, (const float4*)pconst,
+                           pf, part, pl.tiles_x, pl.tiles, p0, p1, pl.kslice);
         break;
     case V_BF16X3:
-        hipLaunchKernelGGL(k_fresnel_gemm<V_BF16X3>, grid, block, 0, stream, g, soa, (const float4*)pconst, part,
-                           pl.tiles_x, pl.tiles, p0, p1, pl.kslice);
+        hipLaunchKernelGGL((k_fresnel_gemm<V_BF16X3, false>), grid, block, 0, stream, g, soa, (const float4*)pconst,
+                           pf, part, pl.tiles_x, pl.tiles, p0, p1, pl.kslice);
+        hipLaunchKernelGGL((k_fresnel_gemm<V_BF16X3, true>), grid, block, 0, stream, g, soa, (const float4*)pconst,
+                           pf, part, pl.tiles_x, pl.tiles, p0, p1, pl.kslice);
         break;
     default:
-        hipLaunchKernelGGL(k_fresnel_gemm<V_FP16X3>, grid, block, 0, stream, g, soa, (const float4*)pconst, part,
-                           pl.tiles_x, pl.tiles, p0, p1, pl.kslice);
+        hipLaunchKernelGGL((k_fresnel_gemm<V_FP16X3, false>), grid, block, 0, stream, g, soa, (const float4*)pconst,
+                           pf, part, pl.tiles_x, pl.tiles, p0, p1, pl.kslice);
+        hipLaunchKernelGGL((k_fresnel_gemm<V_FP16X3, true>), grid, block, 0, stream, g, soa, (const float4*)pconst,
+                           pf, part, pl.tiles_x, pl.tiles, p0, p1, pl.kslice);
         break;
     }
     if (hipGetLastError() != hipSuccess) return OHX_ERR_DEVICE;

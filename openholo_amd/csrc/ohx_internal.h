@@ -113,7 +113,7 @@ int field_zero_on(hipStream_t stream, double* f, int64_t n_px);
 // encode every channel, min/max in the same pass, then 8-bit (encode.hip);
 // mm: two device words (the min/max keys), part: ENCODE_MM_PART_WORDS device
 // words (per-block min/max keys of the encode pass)
-constexpr int ENCODE_MM_BLOCKS = 8192;
+constexpr int ENCODE_MM_BLOCKS = 2048;
 constexpr int ENCODE_MM_PART_WORDS = 2 * ENCODE_MM_BLOCKS;
 int launch_encode_normalize_u8(hipStream_t stream, unsigned long long* mm, unsigned long long* part,
                                const double* const* d_field, int nch,

@@ -331,7 +331,7 @@ def serving_burst(capi, synth, n_requests: int, rs_kernel: str = "bf16x3"):
                               "iterations": int(c.iterations), "kernels": {"RS": rs_kernel, "Fresnel": "bf16x3"}}}
 
 
-def serving_stream(capi, synth, n_requests: int, rs_kernel: str = "bf16x3"):
+def serving_stream(capi, synth, n_requests: int, rs_kernel: str = "bf16x3", server_kw=None):
     """Mixed request stream on the continuous-batching server.  Pass 1 runs
     every request alone (submit, wait) for its alone latency; pass 2 submits
     every request at once to measure capacity (jobs/s); pass 3 replays the
@@ -351,7 +351,7 @@ def serving_stream(capi, synth, n_requests: int, rs_kernel: str = "bf16x3"):
                           want_field=False, want_u8=True)
 
     pairs = sum(r["n"] * r["pn"][0] * r["pn"][1] for r in reqs)
-    with capi.Server() as srv:
+    with capi.Server(**(server_kw or {})) as srv:
         # warm-up: one request of each kind
         for i in range(min(4, len(reqs))):
             srv.wait(submit(srv, i))

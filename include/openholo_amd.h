@@ -283,8 +283,9 @@ typedef struct ohx_server_opts {
     int32_t use_graph;         /* 1: replay a captured hipGraph per iteration  */
     int32_t reserved;
     uint64_t mem_budget;       /* HBM bytes for jobs (0: free HBM - 4 GiB)     */
-    double slice_pairs;        /* point-pixel pairs per job per iteration (0: 1.5e10) */
-    double slice_pairs_mfma;   /* same for matrix-core jobs (0: 1.5e11)        */
+    double slice_pairs;        /* point-pixel pairs per job per iteration (0: 7.5e9) */
+    double slice_pairs_mfma;   /* same for matrix-core jobs (0: 7.5e10; RS on
+                                  matrix cores takes half of it)              */
 } ohx_server_opts;
 
 typedef struct ohx_job {

@@ -142,14 +142,18 @@ int launch_gemm_reduce(hipStream_t stream, const void* part, int64_t splits, int
 // every tile go to the GEMM, the others to the FP32 direct kernel
 int launch_rs_gemm(ohx_ctx* ctx, const Geometry& g, const PointSoA& soa, double* d_field, uint32_t mode);
 // its pieces, for callers that slice the work: the device-side partition of
-// soa into part (eligible points first, in order; *d_count = their number,
-// a device word inside scratch) and the GEMM over part[p0, p1) into d_field
+// soa into part (matrix-core points first, those whose box is the whole
+// hologram before the masked ones, each group in order; (*d_count)[0] = the
+// matrix-core points, (*d_count)[1] = the unmasked ones, device words inside
+// scratch) and the GEMM over part[p0, p1) into d_field (points from n_full
+// on masked; wgs, if given, adds the workgroups launched)
 size_t rs_partition_scratch_bytes(int64_t n);
 int launch_rs_partition(hipStream_t st, void* scratch, size_t scratch_bytes, const Geometry& g, const PointSoA& soa,
                         const PointSoA& part, uint32_t mode, unsigned long long** d_count);
 size_t rs_gemm_scratch_bytes(const Geometry& g, int64_t npoints);
 int launch_rs_gemm_range(hipStream_t st, void* scratch, size_t scratch_bytes, const Geometry& g,
-                         const PointSoA& part, int64_t p0, int64_t p1, double* d_field, uint32_t mode);
+                         const PointSoA& part, int64_t p0, int64_t p1, int64_t n_full, double* d_field,
+                         uint32_t mode, int64_t* wgs = nullptr);
 int launch_probe(ohx_ctx* ctx, const Geometry& g, const PointSoA& soa, const int64_t* d_pix,
                  int64_t n_pix, double* d_out);
 

@@ -14,26 +14,35 @@
 //   z/r^2 = (z/r_c^2) (1 + mu n)(1 + mu N) + O(e^2)
 // with e = |n + N|/r_c^2, h = 1/(2 r_c lambda), k = -1/(8 r_c^3 lambda) and
 // mu = -1/r_c^2.  Writing X(n) = h n + k n^2, Y(N) = h N + k N^2, the cross
-// term 2 k n N of the phase is theta = 4 pi k n N radians, and e^{j theta} is
-// 1 + j theta to within theta^2/2, so the term A e^{j 2pi c} is
+// term 2 k n N of the phase is theta = 4 pi k n N radians.  With A, B the
+// largest |n|, |N| on the tile (theta_max = |4 pi k| A B), e^{j theta} is
+//   (1 - alpha n^2)(1 - beta N^2) + j theta,  alpha = (4 pi k B)^2 / 6,
+//   beta = (4 pi k A)^2 / 6,
+// to within theta_max^2 / 6 (the minimax choice for the real part: the plain
+// 1 + j theta is off by theta_max^2 / 2 at the tile corners), so the term
+// A e^{j 2pi c} is
 //   U1[x] V1[y] + U2[x] V2[y]
-//   V1 = (1 + mu N) e^{j 2pi Y(N)},                 V2 = N V1
-//   U1 = A_c (1 + mu n) e^{j 2pi (phi_c + X(n))},  U2 = j 4 pi k n U1
+//   V1 = (1 + mu N - beta N^2) e^{j 2pi Y(N)},                 V2 = N V1
+//   U1 = A_c (1 + mu n - alpha n^2) e^{j 2pi (phi_c + X(n))},  U2 = j 4 pi k n U1
+// (the products mu beta N^3, mu alpha n^3 are dropped: below e theta_max^2),
 // with A_c = a z/(lambda r_c^2) and phi_c = frac(c_p + (r_c - z)/lambda): two
 // K entries of the Fresnel kernel's complex GEMM per point (S = sum U V^T,
 // field += -j S), 8 points per 16-entry stage.
 //
-// Which points: a point goes to the matrix cores when, over the whole
-// hologram, (a) its mask is every pixel -- the box of ophGen.cpp:384-407 is
-// the hologram and the per-pixel range test of ophGen.cpp:415-429 holds on
-// every pixel with a 1e-12 margin -- and (b) on every tile the neglected terms
-// are small: e^3 r_c/(16 lambda) <= 2e-7 cycles and e^2 <= 1e-7 (the FP32
-// direct kernel's bounds), and theta <= 5e-3 rad for BF16x3 (theta^2/2 <=
-// 1.25e-5 of the term, the size of the 3-term split's own error), 2e-2 rad
-// for BF16x1 (2e-4, a tenth of its BF16 rounding).  k_rs_classify bounds all
-// three with the largest |dx_c|, |dy_c| over the tiles and r_c >= z.  A
-// stable partition puts these points first; the rest run through the FP32
-// direct kernels (launch_direct_with) after the GEMM.
+// Which points: a point goes to the matrix cores when (a) the per-pixel range
+// test of ophGen.cpp:415-429 holds with a 1e-12 margin on every pixel of its
+// box (ophGen.cpp:384-407), so that its mask is the box, a product of a row
+// and a column interval, and (b) on every tile the box touches the neglected
+// terms are small: e^3 r_c/(16 lambda) <= 2e-7 cycles and e^2 <= 1e-7 (the
+// FP32 direct kernel's bounds), and theta_max <= 8e-3 rad for BF16x3
+// (theta_max^2/6 <= 1.1e-5 of the term, the size of the 3-term split's own
+// error), 3.5e-2 rad for BF16x1 (2e-4, a tenth of its BF16 rounding).
+// k_rs_classify bounds all three with the largest |dx_c|, |dy_c| over those
+// tiles and r_c >= z.  A stable three-way partition puts the points whose box
+// is the whole hologram first, then the other matrix-core points, which the
+// masked kernel instance takes (zero amplitude for rows and columns outside
+// the box), then the rest, which run through the FP32 direct kernels
+// (launch_direct_with) after the GEMMs.
 //
 // Operand generation, per stage and thread (point t % 8, rows and columns
 // grp + 32 i, grp = t / 8): the per-(point, tile) constants in FP64 (r_c from
@@ -42,7 +51,7 @@
 // pair for both V1 and V2 (and for U1, U2).  Precision: V1 and U1 in the
 // 3-term split (hi hi + hi lo + lo hi), V2 and U2 hi only (lo part zero): the
 // rank-2 correction is below theta of the term, so its 2^-9 BF16 rounding
-// costs below 1e-5 of the term (theta <= 5e-3).  BF16 only: A_c reaches ~1e7,
+// costs below 2e-5 of the term (theta <= 8e-3).  BF16 only: A_c reaches ~1e7,
 // beyond the FP16 range.
 #include <algorithm>
 #include <climits>
@@ -67,16 +76,19 @@ constexpr double FOUR_PI = 12.566370614359172953850573533118;
 // neglected terms on the matrix-core path (see the header comment)
 constexpr double MAX_PHASE3 = 2e-7;   // cycles
 constexpr double MAX_AMP2 = 1e-7;
-constexpr double MAX_THETA_X3 = 5e-3; // rad, BF16x3
-constexpr double MAX_THETA_X1 = 2e-2; // rad, BF16x1
+constexpr double MAX_THETA_X3 = 8e-3; // rad, BF16x3
+constexpr double MAX_THETA_X1 = 3.5e-2; // rad, BF16x1
 constexpr double RANGE_TOL = 1e-12;   // m, margin of the range test
 
 // Point data of one stage: thread t holds point t % RP.  Lanes past the slice
 // end read the slice's last point and are marked invalid (zero amplitude).
+// MASK: the points' boxes may be smaller than the hologram (loaded too).
 struct Raw {
     double px, py, pz, cyc0, kamp;
+    int4 box;
     bool valid;
 };
+template <bool MASK>
 __device__ inline void load_raw(Raw& r, const PointSoA& s, int64_t pi, int64_t p1)
 {
     r.valid = pi < p1;
@@ -86,6 +98,7 @@ __device__ inline void load_raw(Raw& r, const PointSoA& s, int64_t pi, int64_t p
     r.pz = s.pz[j];
     r.cyc0 = s.cyc0[j];
     r.kamp = s.kamp[j];
+    if constexpr (MASK) r.box = s.box[j];
 }
 
 // Per-(point, tile) constants of a stage
@@ -94,6 +107,9 @@ struct Pt {
     double h, k;              // 1/(2 r_c lambda), -1/(8 r_c^3 lambda)
     double phic;              // frac(c_p + (r_c - z)/lambda)
     float mu, ac, acmu, g;    // -1/r_c^2, A_c (0 for invalid lanes), A_c mu, 4 pi k
+    float beta, acal;         // cross-term correction: beta, A_c alpha
+    unsigned box;             // MASK: box relative to the tile origin, clamped to [0, 128]:
+                              // row range [b0, b1), column range [b2, b3), byte k = bk
 };
 struct Tmp {
     double dyc, dxc, zz, rc2, y, irc2;
@@ -103,6 +119,8 @@ struct Frame {
     double yc, xc;            // reference pixel (x0 + 64, y0 + 64)
     double vb, dv, ub, du;    // v of row grp + RG i = vb + i dv; u of column grp + RG j = ub + j du
     double il, hl;            // 1/lambda, 1/(2 lambda)
+    double um, vm;            // 64 pitches: the largest |u|, |v| on the tile
+    int x0, y0;               // tile origin
 };
 
 // The constants in NPR pieces of a few dependent FP64 operations each, spread
@@ -134,29 +152,57 @@ __device__ inline void prep_piece(Pt& P, Tmp& t, const Raw& r, const Frame& f)
         P.g = (float)(FOUR_PI * P.k);
         P.dy2 = 2.0 * t.dyc;
         P.dx2 = 2.0 * t.dxc;
+        // cross-term correction: with A, B the largest |n|, |N| on the tile
+        // and theta = g n N, (1 - alpha n^2)(1 - beta N^2) + j theta with
+        // alpha = g^2 B^2 / 6, beta = g^2 A^2 / 6 approximates e^{j theta}
+        // to within theta_max^2 / 6 (the plain 1 + j theta: theta_max^2 / 2)
+        const float A = (float)(f.um * (fabs(P.dx2) + f.um)), B = (float)(f.vm * (fabs(P.dy2) + f.vm));
+        const float g2 = P.g * P.g * (1.f / 6.f);
+        P.beta = g2 * A * A;
+        P.acal = P.ac * (g2 * B * B);
+        // box relative to the tile (invalid lanes: the whole tile)
+        auto cl = [](int v) { return (unsigned)min(max(v, 0), 128); };
+        P.box = r.valid ? cl(r.box.z - f.y0) | cl(r.box.w - f.y0) << 8 | cl(r.box.x - f.x0) << 16 |
+                              cl(r.box.y - f.x0) << 24
+                        : 128u << 8 | 128u << 24;
     }
 }
 
 // Unit E of a thread's stage: even E -> V row E/2, odd E -> U column E/2.
 // Value: (Re, Im) of the rank-1 and the rank-2 entry.
-template <int E>
-__device__ inline float4 unit_value(const Pt& P, const Frame& f)
+// MASK: rows / columns outside the point's box get zero amplitude.
+template <int E, bool MASK>
+__device__ inline float4 unit_value(const Pt& P, const Frame& f, int grp)
 {
     constexpr int I = E >> 1;
     if constexpr ((E & 1) == 0) {
-        const double v = fma((double)I, f.dv, f.vb);
+        // the row offsets are recomputed per stage instead of being hoisted
+        // out of the K loop, which would hold eight FP64 values in registers
+        double vb = f.vb;
+        asm volatile("" : "+v"(vb));
+        const double v = fma((double)I, f.dv, vb);
         const double N = v * (P.dy2 + v);
         const float ph = (float)frac64(N * fma(P.k, N, P.h));
         const float Nf = (float)N;
-        const float a = fmaf(P.mu, Nf, 1.f);
+        float a = fmaf(fmaf(-P.beta, Nf, P.mu), Nf, 1.f);   // (1 + mu N)(1 - beta N^2) to first order in each
+        if constexpr (MASK) {
+            const unsigned row = grp + RG * I;
+            a = (row >= __builtin_amdgcn_ubfe(P.box, 0, 8) && row < __builtin_amdgcn_ubfe(P.box, 8, 8)) ? a : 0.f;
+        }
         const float c = a * cos_rev(ph), s = a * sin_rev(ph);
         return make_float4(c, s, Nf * c, Nf * s);
     } else {
-        const double u = fma((double)I, f.du, f.ub);
+        double ub = f.ub;
+        asm volatile("" : "+v"(ub));
+        const double u = fma((double)I, f.du, ub);
         const double n = u * (P.dx2 + u);
         const float ph = (float)frac64(fma(n, fma(P.k, n, P.h), P.phic));
         const float nf = (float)n;
-        const float a = fmaf(P.acmu, nf, P.ac);
+        float a = fmaf(fmaf(-P.acal, nf, P.acmu), nf, P.ac);
+        if constexpr (MASK) {
+            const unsigned col = grp + RG * I;
+            a = (col >= __builtin_amdgcn_ubfe(P.box, 16, 8) && col < __builtin_amdgcn_ubfe(P.box, 24, 8)) ? a : 0.f;
+        }
         const float c = a * cos_rev(ph), s = a * sin_rev(ph);
         const float gn = P.g * nf;
         return make_float4(c, s, -gn * s, gn * c);
@@ -226,7 +272,7 @@ struct StageCtx {
     float4 cs[RU];
 };
 
-template <int VAR, int S>
+template <int VAR, bool MASK, int S>
 __device__ inline void slot(const unsigned* __restrict__ rv, const unsigned* __restrict__ ru,
                            unsigned* __restrict__ wv, unsigned* __restrict__ wu, StageCtx<VAR>& c)
 {
@@ -237,27 +283,27 @@ __device__ inline void slot(const unsigned* __restrict__ rv, const unsigned* __r
         if constexpr (THREE && S < NPR) prep_piece<S>(c.P, c.t, c.R, c.f);
         if constexpr (!THREE && 2 * S < NPR) prep_piece<2 * S>(c.P, c.t, c.R, c.f);
         if constexpr (!THREE && 2 * S + 1 < NPR) prep_piece<2 * S + 1>(c.P, c.t, c.R, c.f);
-        if constexpr (S == (THREE ? NPR : (NPR + 1) / 2)) load_raw(c.R, c.s, c.pnext, c.ke);
+        if constexpr (S == (THREE ? NPR : (NPR + 1) / 2)) load_raw<MASK>(c.R, c.s, c.pnext, c.ke);
         if constexpr (THREE) {
             constexpr int h = half_at(S);
-            if constexpr (h >= 0 && (h & 1) == 0) c.cs[h >> 1] = unit_value<(h >> 1)>(c.P, c.f);
+            if constexpr (h >= 0 && (h & 1) == 0) c.cs[h >> 1] = unit_value<(h >> 1), MASK>(c.P, c.f, c.grp);
             if constexpr (h >= 0 && (h & 1) == 1) unit_store<VAR, (h >> 1)>(wv, wu, c.cs[h >> 1], c.p, c.grp);
         } else {
             constexpr int e = unit_at(S);
-            if constexpr (e >= 0) unit_store<VAR, e>(wv, wu, unit_value<e>(c.P, c.f), c.p, c.grp);
+            if constexpr (e >= 0) unit_store<VAR, e>(wv, wu, unit_value<e, MASK>(c.P, c.f, c.grp), c.p, c.grp);
         }
         __builtin_amdgcn_sched_barrier(0);
-        slot<VAR, S + 1>(rv, ru, wv, wu, c);
+        slot<VAR, MASK, S + 1>(rv, ru, wv, wu, c);
     }
 }
 
 // Whole-stage generation without MFMAs (the prologue's stage 0).
-template <int VAR, int E>
+template <int VAR, bool MASK, int E>
 __device__ inline void gen_units(unsigned* wv, unsigned* wu, const Pt& P, const Frame& f, int p, int grp)
 {
     if constexpr (E < RU) {
-        unit_store<VAR, E>(wv, wu, unit_value<E>(P, f), p, grp);
-        gen_units<VAR, E + 1>(wv, wu, P, f, p, grp);
+        unit_store<VAR, E>(wv, wu, unit_value<E, MASK>(P, f, grp), p, grp);
+        gen_units<VAR, MASK, E + 1>(wv, wu, P, f, p, grp);
     }
 }
 __device__ inline void prep_all(Pt& P, const Raw& r, const Frame& f)
@@ -274,7 +320,7 @@ __device__ inline void prep_all(Pt& P, const Raw& r, const Frame& f)
 // (buffer RB^1) from R, then R <- stage st+2's point data.  Past the last
 // stage the generation runs on clamped, zero-amplitude data into the unused
 // buffer, which keeps the step branch-free.
-template <int VAR, int RB>
+template <int VAR, bool MASK, int RB>
 __device__ inline void step(Smem<Traits<VAR>::three>& sm, f32x16 (&acc)[4][2], Raw& R, const PointSoA& s,
                             const Frame& f, int64_t pnext, int64_t ke, int p, int grp, int lrow, int koff, int ucol)
 {
@@ -282,14 +328,15 @@ __device__ inline void step(Smem<Traits<VAR>::three>& sm, f32x16 (&acc)[4][2], R
     const unsigned* __restrict__ ru = &sm.u[RB][0][0][0];
     StageCtx<VAR> c{acc, R, s, f, pnext, ke, p, grp, lrow, koff, ucol};
     stage_operands<VAR>(rv, ru, c);
-    slot<VAR, 0>(rv, ru, &sm.v[RB ^ 1][0][0][0], &sm.u[RB ^ 1][0][0][0], c);
+    slot<VAR, MASK, 0>(rv, ru, &sm.v[RB ^ 1][0][0][0], &sm.u[RB ^ 1][0][0][0], c);
     __syncthreads();
 }
 
 // S = sum_p U V^T over points kfirst .. klast-1 of s (all matrix-core
 // eligible), split-K: workgroup (tile, split) sums kslice points into plane
-// `split` of part.
-template <int VAR>
+// `split` of part.  MASK: the points' boxes may be smaller than the
+// hologram; out-of-box rows and columns get zero amplitude.
+template <int VAR, bool MASK>
 __global__ __launch_bounds__(NT, 2) void k_rs_gemm(Geometry g, PointSoA s, float2* __restrict__ part, int tiles_x,
                                                    int tiles, int64_t kfirst, int64_t klast, int64_t kslice)
 {
@@ -312,6 +359,10 @@ __global__ __launch_bounds__(NT, 2) void k_rs_gemm(Geometry g, PointSoA s, float
     f.du = (double)RG * g.pp_x;
     f.il = g.inv_lambda;
     f.hl = 0.5 * g.inv_lambda;
+    f.um = (BN / 2) * g.pp_x;
+    f.vm = (BM / 2) * g.pp_y;
+    f.x0 = x0;
+    f.y0 = y0;
 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int lrow = lane & 31, koff = (lane >> 5) * 4;
@@ -319,37 +370,43 @@ __global__ __launch_bounds__(NT, 2) void k_rs_gemm(Geometry g, PointSoA s, float
 
     Raw R;
     if (nst > 0) {
-        load_raw(R, s, kb + p, ke);
+        load_raw<MASK>(R, s, kb + p, ke);
         Pt P;
         prep_all(P, R, f);
-        gen_units<VAR, 0>(&sm.v[0][0][0][0], &sm.u[0][0][0][0], P, f, p, grp);
-        load_raw(R, s, kb + RP + p, ke);
+        gen_units<VAR, MASK, 0>(&sm.v[0][0][0][0], &sm.u[0][0][0][0], P, f, p, grp);
+        load_raw<MASK>(R, s, kb + RP + p, ke);
     }
     __syncthreads();
     f32x16 acc[4][2];
 #pragma unroll
     for (int i = 0; i < 4; ++i) acc[i][0] = acc[i][1] = f32x16{};
     for (int st = 0; st < nst; st += 2) {
-        step<VAR, 0>(sm, acc, R, s, f, kb + (int64_t)(st + 2) * RP + p, ke, p, grp, lrow, koff, ucol);
+        step<VAR, MASK, 0>(sm, acc, R, s, f, kb + (int64_t)(st + 2) * RP + p, ke, p, grp, lrow, koff, ucol);
         if (st + 1 >= nst) break;
-        step<VAR, 1>(sm, acc, R, s, f, kb + (int64_t)(st + 3) * RP + p, ke, p, grp, lrow, koff, ucol);
+        step<VAR, MASK, 1>(sm, acc, R, s, f, kb + (int64_t)(st + 3) * RP + p, ke, p, grp, lrow, koff, ucol);
     }
     store_partials(g, acc, part, split, x0, y0, ucol, lane);
 }
 
-// flag[i] = 1 when point i meets the matrix-core conditions of the header on
-// every tile of the hologram.
-__global__ __launch_bounds__(256) void k_rs_classify(Geometry g, PointSoA s, int tiles_x, int tiles_y,
-                                                     double max_theta, unsigned* __restrict__ flag)
+// Class of point i as a scan key: 1 (matrix cores, box = hologram), 1 << 32
+// (matrix cores, box smaller than the hologram: masked), 0 (direct kernel).
+// Matrix-core points meet the conditions of the header comment: the
+// reference's range test holds with margin on every pixel of the box (the
+// mask is then the box, a product of a row and a column interval), and the
+// neglected terms are small on every tile the box touches.
+__global__ __launch_bounds__(256) void k_rs_classify(Geometry g, PointSoA s, double max_theta,
+                                                     unsigned long long* __restrict__ key)
 {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= s.n) return;
     const double px = s.px[i], py = s.py[i], z = s.pz[i];
     const int4 b = s.box[i];
-    bool ok = z > 0.0 && b.x == 0 && b.y == g.pn_x && b.z == 0 && b.w == g.pn_y;
-    // the reference's range test on every pixel, with margin
-    const double xl = gpix_x(g, 0), xh = gpix_x(g, g.pn_x - 1);
-    const double yh = gpix_y(g, 0), yl = gpix_y(g, g.pn_y - 1);
+    bool ok = z > 0.0 && b.x < b.y && b.z < b.w;
+    const bool full = b.x == 0 && b.y == g.pn_x && b.z == 0 && b.w == g.pn_y;
+    const int bx1 = ok ? b.y - 1 : 0, by1 = ok ? b.w - 1 : 0, bx0 = ok ? b.x : 0, by0 = ok ? b.z : 0;
+    // the reference's range test on every pixel of the box, with margin
+    const double xl = gpix_x(g, bx0), xh = gpix_x(g, bx1);
+    const double yh = gpix_y(g, by0), yl = gpix_y(g, by1);
     const double dxmax = fmax(fabs(xl - px), fabs(xh - px));
     const double dymax = fmax(fabs(yl - py), fabs(yh - py));
     const double dxmin = (px >= xl && px <= xh) ? 0.0 : fmin(fabs(xl - px), fabs(xh - px));
@@ -357,11 +414,11 @@ __global__ __launch_bounds__(256) void k_rs_classify(Geometry g, PointSoA s, int
     const double zz = z * z;
     ok = ok && fabs(g.hx_coef) * sqrt(dymin * dymin + zz) - dxmax > RANGE_TOL &&
          fabs(g.hy_coef) * sqrt(dxmin * dxmin + zz) - dymax > RANGE_TOL;
-    // neglected terms on every tile: reference pixels at tile offset (64, 64),
-    // |u|, |v| <= 64 pitches, r_c >= z
+    // neglected terms on every tile the box touches: reference pixels at tile
+    // offset (64, 64), |u|, |v| <= 64 pitches, r_c >= z
     const double um = (BN / 2) * g.pp_x, vm = (BM / 2) * g.pp_y;
-    const double xc0 = gpix_x(g, BN / 2), xc1 = gpix_x(g, (tiles_x - 1) * BN + BN / 2);
-    const double yc0 = gpix_y(g, BM / 2), yc1 = gpix_y(g, (tiles_y - 1) * BM + BM / 2);
+    const double xc0 = gpix_x(g, (bx0 / BN) * BN + BN / 2), xc1 = gpix_x(g, (bx1 / BN) * BN + BN / 2);
+    const double yc0 = gpix_y(g, (by0 / BM) * BM + BM / 2), yc1 = gpix_y(g, (by1 / BM) * BM + BM / 2);
     const double dxc = fmax(fabs(xc0 - px), fabs(xc1 - px));
     const double dyc = fmax(fabs(yc0 - py), fabs(yc1 - py));
     const double nx = um * (2.0 * dxc + um), ny = vm * (2.0 * dyc + vm);
@@ -369,20 +426,25 @@ __global__ __launch_bounds__(256) void k_rs_classify(Geometry g, PointSoA s, int
     const double kmax = 0.125 * g.inv_lambda / (zz * z);
     ok = ok && e * e * e * z * g.inv_lambda / 16.0 <= MAX_PHASE3 && e * e <= MAX_AMP2 &&
          FOUR_PI * kmax * nx * ny <= max_theta;
-    flag[i] = ok ? 1u : 0u;
+    key[i] = ok ? (full ? 1ull : 1ull << 32) : 0ull;
 }
 
-// Stable partition: eligible points to o[0, ne) and the others to o[ne, n),
-// each in point order; pos = exclusive prefix sum of flag.
-__global__ __launch_bounds__(256) void k_rs_gather(PointSoA s, const unsigned* __restrict__ flag,
-                                                   const unsigned* __restrict__ pos, PointSoA o,
+// Stable three-way partition: unmasked matrix-core points to o[0, n0), masked
+// ones to o[n0, n0 + n1), the others after them, each in point order; pos =
+// exclusive prefix sum of key (low word: n0 before i, high word: n1 before i).
+// count[0] = n0 + n1, count[1] = n0.
+__global__ __launch_bounds__(256) void k_rs_gather(PointSoA s, const unsigned long long* __restrict__ key,
+                                                   const unsigned long long* __restrict__ pos, PointSoA o,
                                                    unsigned long long* __restrict__ count)
 {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     const int64_t n = s.n;
     if (i >= n) return;
-    const int64_t ne = (int64_t)pos[n - 1] + flag[n - 1];
-    const int64_t j = flag[i] ? (int64_t)pos[i] : ne + (i - (int64_t)pos[i]);
+    const unsigned long long tot = pos[n - 1] + key[n - 1];
+    const int64_t n0 = (int64_t)(tot & 0xffffffffull), n1 = (int64_t)(tot >> 32);
+    const unsigned long long k = key[i], ps = pos[i];
+    const int64_t lo = (int64_t)(ps & 0xffffffffull), hi = (int64_t)(ps >> 32);
+    const int64_t j = k == 1ull ? lo : k ? n0 + hi : n0 + n1 + (i - lo - hi);
     o.px[j] = s.px[i];
     o.py[j] = s.py[i];
     o.pz[j] = s.pz[i];
@@ -390,7 +452,10 @@ __global__ __launch_bounds__(256) void k_rs_gather(PointSoA s, const unsigned* _
     o.kamp[j] = s.kamp[i];
     o.q[j] = s.q[i];
     o.box[j] = s.box[i];
-    if (i == 0) *count = (unsigned long long)ne;
+    if (i == 0) {
+        count[0] = (unsigned long long)(n0 + n1);
+        count[1] = (unsigned long long)n0;
+    }
 }
 
 }  // namespace rsgemm
@@ -415,18 +480,18 @@ PointSoA soa_tail(const PointSoA& s, int64_t off)
 size_t scan_temp_bytes(int64_t n)
 {
     size_t b = 0;
-    if (hipcub::DeviceScan::ExclusiveSum(nullptr, b, (const unsigned*)nullptr, (unsigned*)nullptr, (int)n,
-                                         (hipStream_t)0) != hipSuccess)
+    if (hipcub::DeviceScan::ExclusiveSum(nullptr, b, (const unsigned long long*)nullptr, (unsigned long long*)nullptr,
+                                         (int)n, (hipStream_t)0) != hipSuccess)
         return 0;
     return b;
 }
 constexpr size_t MAX_RS_PART = size_t(4) << 30;
 }  // namespace
 
-// [count word, 256 B] [flag] [pos] [scan temp]
+// [count words, 256 B] [key] [pos] [scan temp]
 size_t rs_partition_scratch_bytes(int64_t n)
 {
-    return 256 + 2 * al256((size_t)n * sizeof(unsigned)) + al256(scan_temp_bytes(n)) + 256;
+    return 256 + 2 * al256((size_t)n * sizeof(unsigned long long)) + al256(scan_temp_bytes(n)) + 256;
 }
 
 int launch_rs_partition(hipStream_t st, void* scratch, size_t scratch_bytes, const Geometry& g, const PointSoA& soa,
@@ -439,28 +504,34 @@ int launch_rs_partition(hipStream_t st, void* scratch, size_t scratch_bytes, con
     if (!cub_bytes || rs_partition_scratch_bytes(n) > scratch_bytes) return OHX_ERR_INVALID_ARG;
     char* b = (char*)scratch;
     *d_count = (unsigned long long*)b;
-    unsigned* flag = (unsigned*)(b + 256);
-    unsigned* pos = (unsigned*)(b + 256 + al256((size_t)n * sizeof(unsigned)));
-    void* tmp = b + 256 + 2 * al256((size_t)n * sizeof(unsigned));
-    const int tiles_x = (g.pn_x + BN - 1) / BN, tiles_y = (g.pn_y + BM - 1) / BM;
+    unsigned long long* key = (unsigned long long*)(b + 256);
+    unsigned long long* pos = (unsigned long long*)(b + 256 + al256((size_t)n * sizeof(unsigned long long)));
+    void* tmp = b + 256 + 2 * al256((size_t)n * sizeof(unsigned long long));
     const unsigned pgrid = (unsigned)((n + 255) / 256);
     const double max_theta = (mode & OHX_MODE_MFMA_BF16X3) ? MAX_THETA_X3 : MAX_THETA_X1;
-    hipLaunchKernelGGL(k_rs_classify, dim3(pgrid), dim3(256), 0, st, g, soa, tiles_x, tiles_y, max_theta, flag);
-    if (hipcub::DeviceScan::ExclusiveSum(tmp, cub_bytes, (const unsigned*)flag, pos, (int)n, st) != hipSuccess)
+    hipLaunchKernelGGL(k_rs_classify, dim3(pgrid), dim3(256), 0, st, g, soa, max_theta, key);
+    if (hipcub::DeviceScan::ExclusiveSum(tmp, cub_bytes, (const unsigned long long*)key, pos, (int)n, st) !=
+        hipSuccess)
         return OHX_ERR_DEVICE;
-    hipLaunchKernelGGL(k_rs_gather, dim3(pgrid), dim3(256), 0, st, soa, (const unsigned*)flag,
-                       (const unsigned*)pos, part, *d_count);
+    hipLaunchKernelGGL(k_rs_gather, dim3(pgrid), dim3(256), 0, st, soa, (const unsigned long long*)key,
+                       (const unsigned long long*)pos, part, *d_count);
     return hipGetLastError() == hipSuccess ? OHX_OK : OHX_ERR_DEVICE;
 }
 
+// Enough split-K planes for any launch of at most npoints points (the plan's
+// split count, before rounding the slice to whole stages, bounds them all).
 size_t rs_gemm_scratch_bytes(const Geometry& g, int64_t npoints)
 {
-    const GemmPlan p = plan_gemm(g, npoints, rsgemm::RP);
-    return (size_t)p.splits * p.npx * sizeof(float2);
+    using namespace gemm;
+    const int64_t tiles = (int64_t)((g.pn_x + BN - 1) / BN) * ((g.pn_y + BM - 1) / BM);
+    const int64_t n = npoints > 1 ? npoints : 1;
+    const int64_t splits = std::max<int64_t>((1024 + tiles - 1) / tiles, (n + 65535) / 65536);
+    return (size_t)splits * g.pn_x * g.pn_y * sizeof(float2);
 }
 
-int launch_rs_gemm_range(hipStream_t st, void* scratch, size_t scratch_bytes, const Geometry& g,
-                         const PointSoA& part, int64_t p0, int64_t p1, double* d_field, uint32_t mode)
+namespace {
+int rs_gemm_launch(hipStream_t st, void* scratch, size_t scratch_bytes, const Geometry& g, const PointSoA& part,
+                   int64_t p0, int64_t p1, bool masked, double* d_field, uint32_t mode, int64_t* wgs)
 {
     using namespace rsgemm;
     if (p1 <= p0) return OHX_OK;
@@ -468,14 +539,33 @@ int launch_rs_gemm_range(hipStream_t st, void* scratch, size_t scratch_bytes, co
     if ((size_t)pl.splits * pl.npx * sizeof(float2) > scratch_bytes) return OHX_ERR_INVALID_ARG;
     const dim3 grid((unsigned)((int64_t)pl.tiles * pl.splits)), block(NT);
     float2* pp = (float2*)scratch;
-    if (mode & OHX_MODE_MFMA_BF16X3)
-        hipLaunchKernelGGL(k_rs_gemm<V_BF16X3>, grid, block, 0, st, g, part, pp, pl.tiles_x, pl.tiles, p0, p1,
+    const bool x3 = (mode & OHX_MODE_MFMA_BF16X3) != 0;
+    if (x3 && masked)
+        hipLaunchKernelGGL((k_rs_gemm<V_BF16X3, true>), grid, block, 0, st, g, part, pp, pl.tiles_x, pl.tiles, p0, p1,
+                           pl.kslice);
+    else if (x3)
+        hipLaunchKernelGGL((k_rs_gemm<V_BF16X3, false>), grid, block, 0, st, g, part, pp, pl.tiles_x, pl.tiles, p0,
+                           p1, pl.kslice);
+    else if (masked)
+        hipLaunchKernelGGL((k_rs_gemm<V_BF16X1, true>), grid, block, 0, st, g, part, pp, pl.tiles_x, pl.tiles, p0, p1,
                            pl.kslice);
     else
-        hipLaunchKernelGGL(k_rs_gemm<V_BF16X1>, grid, block, 0, st, g, part, pp, pl.tiles_x, pl.tiles, p0, p1,
-                           pl.kslice);
+        hipLaunchKernelGGL((k_rs_gemm<V_BF16X1, false>), grid, block, 0, st, g, part, pp, pl.tiles_x, pl.tiles, p0,
+                           p1, pl.kslice);
     if (hipGetLastError() != hipSuccess) return OHX_ERR_DEVICE;
+    if (wgs) *wgs += (int64_t)pl.tiles * pl.splits;
     return launch_gemm_reduce(st, pp, pl.splits, pl.npx, d_field);
+}
+}  // namespace
+
+int launch_rs_gemm_range(hipStream_t st, void* scratch, size_t scratch_bytes, const Geometry& g,
+                         const PointSoA& part, int64_t p0, int64_t p1, int64_t n_full, double* d_field, uint32_t mode,
+                         int64_t* wgs)
+{
+    const int64_t m = std::min(std::max(n_full, p0), p1);
+    int e = rs_gemm_launch(st, scratch, scratch_bytes, g, part, p0, m, false, d_field, mode, wgs);
+    if (!e) e = rs_gemm_launch(st, scratch, scratch_bytes, g, part, m, p1, true, d_field, mode, wgs);
+    return e;
 }
 
 int launch_rs_gemm(ohx_ctx* ctx, const Geometry& g, const PointSoA& soa, double* d_field, uint32_t mode)
@@ -505,26 +595,22 @@ int launch_rs_gemm(ohx_ctx* ctx, const Geometry& g, const PointSoA& soa, double*
     unsigned long long* d_count = nullptr;
     int e = launch_rs_partition(st, ctx->rs_scratch, pbytes, g, soa, part, mode, &d_count);
     if (e) return set_error(ctx, e, "RS matrix-core partition failed");
-    unsigned long long ne_dev = 0;
-    e = check_hip(ctx, hipMemcpyAsync(&ne_dev, d_count, sizeof(ne_dev), hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
+    unsigned long long cnt[2] = {0, 0};
+    e = check_hip(ctx, hipMemcpyAsync(cnt, d_count, sizeof(cnt), hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
     if (!e) e = check_hip(ctx, hipStreamSynchronize(st), "hipStreamSynchronize");
     if (e) return e;
-    const int64_t ne = (int64_t)ne_dev;
+    const int64_t ne = (int64_t)cnt[0], n_full = (int64_t)cnt[1];
 
     int64_t wgs = 0;
     if (ne > 0) {
         int64_t chunk = ne;
         while (chunk > RP && rs_gemm_scratch_bytes(g, chunk) > MAX_RS_PART) chunk = (chunk + 1) / 2;
-        // the last, shorter chunk may plan more split planes than a full one
-        const size_t gbytes =
-            std::max(rs_gemm_scratch_bytes(g, chunk), rs_gemm_scratch_bytes(g, ne - (ne - 1) / chunk * chunk));
-        if ((e = ensure_gemm_scratch(ctx, gbytes))) return e;
+        if ((e = ensure_gemm_scratch(ctx, rs_gemm_scratch_bytes(g, chunk)))) return e;
         for (int64_t p0 = 0; p0 < ne; p0 += chunk) {
             const int64_t p1 = std::min<int64_t>(ne, p0 + chunk);
-            e = launch_rs_gemm_range(st, ctx->gemm_scratch, ctx->gemm_scratch_bytes, g, part, p0, p1, d_field, mode);
+            e = launch_rs_gemm_range(st, ctx->gemm_scratch, ctx->gemm_scratch_bytes, g, part, p0, p1, n_full,
+                                     d_field, mode, &wgs);
             if (e) return set_error(ctx, e, "RS matrix-core launch failed");
-            const GemmPlan pl = plan_gemm(g, p1 - p0, RP);
-            wgs += (int64_t)pl.tiles * pl.splits;
         }
     }
     if (ne < n) {

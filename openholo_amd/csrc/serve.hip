@@ -162,7 +162,8 @@ struct SJob {
     PointSoA part{};
     void* rs_tmp = nullptr;
     size_t rs_tmp_bytes = 0;
-    int64_t ne = -1;                   // -1 until the setup has completed
+    int64_t ne = -1;                   // matrix-core points; -1 until the setup has completed
+    int64_t n_full = 0;                // of which the first n_full have the whole hologram as box
     int64_t slice_d = 0;               // direct slice of an RS matrix-core job
     double* out_field = nullptr;
     uint8_t* out_u8 = nullptr;
@@ -314,7 +315,7 @@ static int setup_job(ohx_server* sv, SJob& j)
         if (!st && j.rsm) {
             unsigned long long* dc = nullptr;
             st = launch_rs_partition(sv->s_setup, j.rs_tmp, j.rs_tmp_bytes, j.g, j.soa, j.part, j.mode, &dc);
-            if (!st && hipMemcpyAsync(rs_count(j), dc, sizeof(unsigned long long), hipMemcpyDeviceToHost,
+            if (!st && hipMemcpyAsync(rs_count(j), dc, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                                       sv->s_setup) != hipSuccess)
                 st = OHX_ERR_DEVICE;
         }
@@ -431,7 +432,8 @@ static bool enqueue_iteration(ohx_server* sv, const std::vector<SJob*>& act, int
         if (j->state != J_ACTIVE || j->cancel || j->next >= j->n) continue;
         if (j->rsm && j->ne < 0) {   // its split into GEMM and direct points is known once the setup is done
             if (!j->ev_ready || hipEventQuery(j->ev_ready) != hipSuccess) continue;
-            j->ne = (int64_t)*rs_count(*j);
+            j->ne = (int64_t)rs_count(*j)[0];
+            j->n_full = (int64_t)rs_count(*j)[1];
         }
         cand.push_back(j);
     }
@@ -483,7 +485,8 @@ static bool enqueue_iteration(ohx_server* sv, const std::vector<SJob*>& act, int
         const int64_t p0 = j->next, p1 = slice_end(*j);
         if (!st)
             st = j->mfma ? launch_fresnel_gemm_range(s, j->gemm, j->gemm_bytes, j->g, j->soa, p0, p1, j->field, j->mode)
-                         : launch_rs_gemm_range(s, j->gemm, j->gemm_bytes, j->g, j->part, p0, p1, j->field, j->mode);
+                         : launch_rs_gemm_range(s, j->gemm, j->gemm_bytes, j->g, j->part, p0, p1, j->n_full, j->field,
+                                                j->mode);
         if (!st && p1 >= j->n) {
             hipEvent_t e = get_event(sv);
             if (!e || hipEventRecord(e, s) != hipSuccess) st = OHX_ERR_DEVICE;

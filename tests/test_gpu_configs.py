@@ -166,10 +166,10 @@ def test_4320p_partial_masks_vs_oracle(ctx, orc, diff):
 
 def test_4320p_rs_matrix_core_vs_oracle(ctx, orc):
     """2M-point workload geometry, first 2,000 points, all three channels, RS
-    with MODE_MFMA_BF16X3: points whose mask is the whole hologram and whose
-    tile expansion holds (most of them) run on the matrix cores, the rest
-    (partial masks, z near 0.3 m far off axis) on the FP32 direct kernel.
-    Same pixels as the direct-kernel case (random plus box edges)."""
+    with MODE_MFMA_BF16X3: points whose tile expansion holds (most of them;
+    those whose box ends inside the hologram on the masked instance) run on
+    the matrix cores, the rest (z near 0.3 m far off axis) on the FP32 direct
+    kernel.  Same pixels as the direct-kernel case (random plus box edges)."""
     w = synth.WORKLOADS[5]
     v = synth.points(w, 2_000)
     g = orc.make_geom(w.pn, w.pp)
@@ -195,7 +195,7 @@ def test_4320p_rs_matrix_core_vs_oracle(ctx, orc):
         err_edge = rel_l2(got[sel], want[sel]) if sel.any() else 0.0
         print(f"ch={ch}: rel-L2 {err:.3e} (box-edge pixels {err_edge:.3e}), "
               f"{st.gemm_points} of {len(v)} points on matrix cores")
-        assert 0.75 * len(v) < st.gemm_points < len(v) - n_partial + 1
+        assert 0.9 * len(v) < st.gemm_points <= len(v)
         assert err < tol, (ch, err)
         if sel.any():
             assert err_edge < tol, (ch, err_edge)

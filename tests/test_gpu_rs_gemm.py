@@ -129,3 +129,103 @@ def test_rs_gemm_deterministic_and_close_to_direct(ctx):
     print(f"vs FP32 direct: rel-L2 {err:.3e}, {sa.gemm_points} of {len(v)} on matrix cores")
     assert sa.gemm_points > 0
     assert err < 3e-5
+
+
+def test_rs_gemm_complex_amplitude_vs_oracle(ctx, orc):
+    """Complex amplitudes (Vertex::phase used): the 1M-point workload's
+    geometry and amplitude distribution, first 5,000 points, RS on the matrix
+    cores vs the oracle with the phase term on 3,000 pixels."""
+    mode, tol = RS_MFMA["bf16x3"]
+    w = synth.WORKLOADS[3]
+    v = synth.points(w, 5_000)
+    g = orc.make_geom(w.pn, w.pp)
+    pix = np.random.default_rng(510).choice(w.n_pixels, 3000, replace=False)
+    want = orc.field_exact(g, v, w.lambdas[0], 0, 0, pix, use_phase=True)
+    npx = w.n_pixels
+    dv = ctx.to_device(v)
+    fld = ctx.alloc(16 * npx)
+    try:
+        ctx.field_zero(fld, npx)
+        ctx.diffract(dv, len(v), fld, capi.params(w.pn, w.pp, w.lambdas[0], use_phase=True), 0, mode)
+        st = ctx.last_stats()
+        got = fld.download(np.complex128, npx)[pix]
+    finally:
+        dv.free()
+        fld.free()
+    err = rel_l2(got, want)
+    print(f"complex amplitudes: rel-L2 {err:.3e}, {st.gemm_points} on matrix cores")
+    assert st.gemm_points == len(v)
+    assert err < tol
+
+
+def test_rs_gemm_chunked_point_range(ctx):
+    """5M points at 3840x2160: the RS split-K planes would exceed their 4 GiB
+    cap, so the matrix-core part runs in two point chunks, each reduced into
+    the field.  Against the library's FP64 probe within the BF16x3 bound."""
+    mode, tol = RS_MFMA["bf16x3"]
+    w = synth.WORKLOADS[2]
+    n = 5_000_000
+    v = synth.random_cloud(78, n, 8e-3, (0.45, 0.55))
+    p = capi.params(w.pn, w.pp, w.lambdas[1])
+    dv = ctx.to_device(v)
+    npx = w.n_pixels
+    fld = ctx.alloc(16 * npx)
+    try:
+        ctx.field_zero(fld, npx)
+        ctx.diffract(dv, n, fld, p, 0, mode)
+        st = ctx.last_stats()
+        pix = np.random.default_rng(6).choice(npx, 128, replace=False)
+        got = fld.download(np.complex128, npx)[pix]
+        want = ctx.probe_f64(dv, n, p, 0, pix)
+    finally:
+        dv.free()
+        fld.free()
+    assert st.kernel_kind == 2 and st.gemm_points == n
+    assert st.workgroups == 2 * 510 * 39   # two chunks of 2.5M points, 39 splits each
+    err = rel_l2(got, want)
+    print(f"chunked RS bf16x3: rel-L2 {err:.3e}")
+    assert err < tol
+
+
+def test_rs_gemm_masked_points_exact_mask(ctx, orc):
+    """7680x4320 at 3.74 um, 473 nm: points of the 2M-point workload whose box
+    ends inside the hologram but whose tile expansion holds run on the masked
+    matrix-core instance.  Three such points, each alone: the nonzero pixels
+    are exactly its box (zero amplitude outside, as the reference's mask), and
+    the values match the oracle on pixels inside and on both sides of every
+    box edge."""
+    mode, tol = RS_MFMA["bf16x3"]
+    w = synth.WORKLOADS[5]
+    ch = 2
+    lam = w.lambdas[ch]
+    v = synth.points(w, 3000)
+    g = orc.make_geom(w.pn, w.pp)
+    pnx, pny = w.pn
+    rng = np.random.default_rng(520)
+    found = 0
+    for i in range(len(v)):
+        x0, x1, y0, y1 = orc.box(g, v[i]["pos"], lam, 0)
+        if (x0, x1, y0, y1) == (0, pnx, 0, pny) or x1 <= x0 or y1 <= y0:
+            continue
+        f, st = run(ctx, v[i:i + 1], w.pn, w.pp, lam, mode, ch)
+        if st.gemm_points != 1:
+            continue   # this point's expansion does not hold: direct kernel
+        f = f.reshape(pny, pnx)
+        assert np.count_nonzero(f) == (x1 - x0) * (y1 - y0), i
+        assert np.all(f[y0:y1, x0:x1] != 0), i
+        rows = [y0, y1 - 1] + ([y0 - 1] if y0 > 0 else []) + ([y1] if y1 < pny else [])
+        cols = [x0, x1 - 1] + ([x0 - 1] if x0 > 0 else []) + ([x1] if x1 < pnx else [])
+        pix = [int(r) * pnx + int(c) for r in rows for c in rng.integers(max(x0 - 1, 0), min(x1 + 1, pnx), 40)]
+        pix += [int(r) * pnx + int(c) for c in cols for r in rng.integers(max(y0 - 1, 0), min(y1 + 1, pny), 40)]
+        pix += [int(r) * pnx + int(c) for r, c in zip(rng.integers(y0, y1, 200), rng.integers(x0, x1, 200))]
+        pix = np.unique(np.array(pix, np.int64))
+        want = orc.field_exact(g, v[i:i + 1], lam, ch, 0, pix)
+        got = f.ravel()[pix]
+        assert np.array_equal(got != 0, want != 0), i
+        err = rel_l2(got, want)
+        print(f"point {i}: box {x0}-{x1} x {y0}-{y1}, rel-L2 {err:.3e} on {len(pix)} pixels")
+        assert err < tol, (i, err)
+        found += 1
+        if found == 3:
+            break
+    assert found == 3

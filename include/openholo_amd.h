@@ -46,10 +46,10 @@ enum { OHX_DIFF_RS = 0, OHX_DIFF_FRESNEL = 1 };
  *   MODE_FLOAT [|FASTMATH]   : FP32 direct sum with FP64 per-point phase constants
  *   MFMA_*                   : Fresnel: H = U diag(a) V^T on matrix cores.
  *                              RS with BF16X1 / BF16X3 (and not FP16X3): points
- *                              whose mask is the whole hologram and whose
- *                              rank-2 tile expansion holds run on matrix cores,
- *                              the others (and RS with FP16X3) run the FP32
- *                              direct sum. */
+ *                              whose mask is their box (the range test holds
+ *                              on all of it) and whose rank-2 tile expansion
+ *                              holds run on matrix cores, the others (and RS
+ *                              with FP16X3) run the FP32 direct sum. */
 enum {
     OHX_MODE_GPU = 1,
     OHX_MODE_FLOAT = 2,

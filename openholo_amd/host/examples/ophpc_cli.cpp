@@ -9,7 +9,7 @@
 //   -o DIR            output directory (default Result)
 //   --vtx FILE        raw 56-byte Vertex records instead of -i
 //   --mfma bf16x1|bf16x3|fp16x3   matrix-core variant (Fresnel: all three; RS: the BF16
-//                                 ones, for the points whose mask is the whole hologram)
+//                                 ones, for the points whose tile expansion holds)
 //   --dump-field F    raw complex128 fields of all channels (row-major)
 //   --dump-enc F      raw float64 encoded buffers
 //   --dump-u8 F       raw uint8 normalised buffers

@@ -7,7 +7,7 @@ matrix-core variants:
   BF16 3-term : 1e-4
 ohx_last_launch_stats().gemm_points names how many points ran on the matrix
 cores; the tests check it against the eligibility each geometry implies
-(points whose mask is the whole hologram and whose tile expansion holds).
+(points whose mask is their box and whose tile expansion holds).
 """
 import numpy as np
 import pytest

@@ -103,11 +103,11 @@ def test_fp32_realistic_geometry_vs_exact(ctx, orc, diff):
 @pytest.mark.parametrize("diff", [0, 1])
 def test_fp64_realistic_geometry_vs_exact(ctx, orc, diff):
     """FP64 mode at the 10k-point RS workload's geometry (1920x1080 at 8 um,
-    532 nm, z ~ 0.5 m), 256 points: every (point, tile) pair takes the
+    532 nm, z ~ 0.5 m), 3,000 points: every (point, tile) pair takes the
     separable form (column factor x row factor, and for RS the short series
     correction).  Field vs the extended-precision oracle on 3000 pixels."""
     w = synth.WORKLOADS[1]
-    v = synth.points(w, 256)
+    v = synth.points(w, 3000)
     g = orc.make_geom(w.pn, w.pp)
     got = capi.generate(ctx, v, w.pn, w.pp, [w.lambdas[0]], diff, FP64)[0].ravel()
     pix = np.random.default_rng(13).choice(w.n_pixels, 3000, replace=False)

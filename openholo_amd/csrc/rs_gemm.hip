@@ -488,9 +488,11 @@ size_t scan_temp_bytes(int64_t n)
 constexpr size_t MAX_RS_PART = size_t(4) << 30;
 }  // namespace
 
-// [count words, 256 B] [key] [pos] [scan temp]
+// [count words, 256 B] [key] [pos] [scan temp]; beyond INT_MAX points (the
+// scan's item count) a size no budget admits
 size_t rs_partition_scratch_bytes(int64_t n)
 {
+    if (n > INT_MAX) return SIZE_MAX / 4;
     return 256 + 2 * al256((size_t)n * sizeof(unsigned long long)) + al256(scan_temp_bytes(n)) + 256;
 }
 

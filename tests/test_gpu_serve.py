@@ -172,3 +172,18 @@ def test_rs_matrix_core_jobs():
         res = [srv.wait(j, 300) for j in ids]
         assert all(r[0] == capi.OK for r in res)
         assert np.array_equal(res[1][2], alone)
+
+
+def test_rs_matrix_core_job_without_matrix_core_points():
+    """An RS job with MODE_MFMA_BF16X3 none of whose points qualify (all near,
+    partial masks): it runs as direct slices only and equals the FP32 direct
+    job to FP64 rounding of the slice partial sums."""
+    v = synth.random_cloud(73, 800, 2e-3, (2e-3, 4e-3))
+    pn, pp, lam = (640, 480), (8e-6, 8e-6), 473e-9
+    p = capi.params(pn, pp, lam)
+    with capi.Server(slice_pairs=2e8, slice_pairs_mfma=5e9) as srv:
+        st, stats, f_mfma, _ = srv.wait(srv.submit(v, p, 0, MFMA3, want_field=True), 120)
+        assert st == capi.OK and stats.iterations >= 2
+        st, _, f_direct, _ = srv.wait(srv.submit(v, p, 0, FP32, want_field=True), 120)
+        assert st == capi.OK
+    assert rel_l2(f_mfma, f_direct) < 1e-12

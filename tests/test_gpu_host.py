@@ -98,16 +98,22 @@ def test_example_program_ohc_and_bmp(tmp_path):
     assert np.array_equal(np.frombuffer(bmp[off:], np.uint8), np.fromfile(d / "u.bin", np.uint8))
 
 
-def test_cli_mfma_fresnel(tmp_path, orc):
+@pytest.mark.parametrize("diff", [0, 1])
+def test_cli_mfma(tmp_path, orc, diff):
+    """The drop-in with --mfma bf16x3 on the reference's RGB example, Fresnel
+    (-f 1) and RS (-f 0): every channel's field vs the oracle within the
+    BF16 3-term bound."""
     d = tmp_path
-    run(os.path.join(BIN, "ophpc_cli"), "-c", CFG_RGB, "-i", PLY, "-m", "0", "-p", "1", "-f", "1",
+    run(os.path.join(BIN, "ophpc_cli"), "-c", CFG_RGB, "-i", PLY, "-m", "0", "-p", "1", "-f", str(diff),
         "--mfma", "bf16x3", "-o", str(d / "R"), "--dump-points", str(d / "p.bin"), "--dump-field", str(d / "f.bin"))
     v = np.fromfile(d / "p.bin", VERTEX)
     f = np.fromfile(d / "f.bin", np.complex128).reshape(3, -1)
     g = orc.make_geom((1920, 1080), (10e-6, 10e-6), scale=(0.01, 0.01, 0.01), distance=0.5)
     pix = np.random.default_rng(9).choice(1920 * 1080, 1500, replace=False)
     for ch, lam in enumerate([638e-9, 520e-9, 450e-9]):
-        assert rel_l2(f[ch][pix], orc.field_exact(g, v, lam, ch, 1, pix)) < 1e-4
+        err = rel_l2(f[ch][pix], orc.field_exact(g, v, lam, ch, diff, pix))
+        print(f"diff={diff} ch={ch}: rel-L2 {err:.3e}")
+        assert err < 1e-4
 
 
 def test_cli_wrp_end_to_end(tmp_path, orc):

@@ -250,7 +250,9 @@ int ohx_encode_normalize_u8(ohx_ctx* ctx, const double* const* d_field, int n_ch
 /* Shape of the last ohx_pc_diffract on ctx (for throughput accounting). */
 typedef struct ohx_launch_stats {
     int64_t pairs_nominal;   /* n_points * pn_x * pn_y                        */
-    int64_t workgroups;      /* workgroups of the diffraction kernel          */
+    int64_t workgroups;      /* workgroups of the diffraction kernel (matrix-
+                                core Fresnel: of its unmasked instance, which
+                                runs when every box covers the hologram)      */
     int32_t kernel_kind;     /* 0 FP32 direct, 1 FP64 direct, 2 MFMA GEMM, 3 WRP */
     int32_t gemm_points;     /* RS with an MFMA mode: points that ran on the
                                 matrix cores (the others ran the FP32 direct

@@ -15,10 +15,22 @@ constexpr int KP = 16;            // points per stage
 constexpr int KS = 2 * KP;        // realified K per stage (32)
 constexpr int LDK = KS + 8;       // padded row length (bf16/f16 elements): 80 B rows
 constexpr int ROWW = LDK / 2;     // row length in dwords (20)
-constexpr int NGRP = NT / KP;     // thread groups of 16 lanes, one point per lane (16)
-constexpr int NVT = BM / NGRP;    // V rows per thread and stage (8): grp + NGRP i
-constexpr int NUT = BN / NGRP;    // U columns per thread and stage (8): grp + NGRP j
-constexpr int NE = NVT + NUT;     // elements per thread and stage
+
+// Tile shape of the Fresnel kernel: BM rows x W columns, W / 32 waves (wave w
+// owns columns 32w .. 32w+31 and all BM rows), 16 points per stage with one
+// point per lane of each 16-lane thread group.  W = 128: 4 waves, two
+// workgroups per CU; W = 256: 8 waves, one workgroup per CU, and a quarter
+// fewer generated elements per point-pixel pair.
+template <int W>
+struct Shape {
+    static constexpr int BN = W;
+    static constexpr int NT = 2 * W;             // 64 lanes per 32 columns
+    static constexpr int NGRP = NT / KP;         // thread groups of 16 lanes
+    static constexpr int NVT = BM / NGRP;        // V rows per thread and stage: grp + NGRP i
+    static constexpr int NUT = W / NGRP;         // U columns per thread and stage: grp + NGRP j
+    static constexpr int NE = NVT + NUT;         // elements per thread and stage
+    static constexpr int PLANE_U = W * ROWW;     // dwords of one U plane
+};
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
@@ -75,11 +87,11 @@ __device__ inline f32x16 mfma(u32x4 a, u32x4 b, f32x16 c)
 
 // LDS: V[y][p] and U[x][p] as packed (Re, Im) 16-bit pairs, hi and (3-term) lo
 // parts, double buffered; rows padded to 80 bytes (conflict-free ds_read_b128).
-template <bool THREE>
+template <bool THREE, int W = BN>
 struct Smem {
     static constexpr int NH = THREE ? 2 : 1;
     unsigned v[2][NH][BM][ROWW];
-    unsigned u[2][NH][BN][ROWW];
+    unsigned u[2][NH][W][ROWW];
 };
 
 __device__ inline double gpix_x(const Geometry& g, int col)
@@ -126,12 +138,12 @@ __device__ inline void load_a(const unsigned* rv, C& c, int g)
     c.A[g][0] = *reinterpret_cast<const u32x4*>(a);
     if constexpr (Traits<VAR>::three) c.A[g][1] = *reinterpret_cast<const u32x4*>(a + PLANE_V);
 }
-template <int VAR, class C>
+template <int VAR, int PU = PLANE_U, class C>
 __device__ inline void load_b(const unsigned* ru, C& c, int ks)
 {
     const unsigned* b = ru + c.ucol * ROWW + ks * 8 + c.koff;
     c.Ub[ks][0] = *reinterpret_cast<const u32x4*>(b);
-    if constexpr (Traits<VAR>::three) c.Ub[ks][1] = *reinterpret_cast<const u32x4*>(b + PLANE_U);
+    if constexpr (Traits<VAR>::three) c.Ub[ks][1] = *reinterpret_cast<const u32x4*>(b + PU);
 }
 template <class C>
 __device__ inline void xform_b(C& c, int ks, int h)
@@ -147,11 +159,12 @@ constexpr int mfma_per_stage()
     return Traits<VAR>::three ? 48 : 16;
 }
 
-// Operands of a stage's first MFMAs, loaded before slot 0.
-template <int VAR, class C>
+// Operands of a stage's first MFMAs, loaded before slot 0 (PU: dwords of one
+// U plane).
+template <int VAR, int PU = PLANE_U, class C>
 __device__ inline void stage_operands(const unsigned* rv, const unsigned* ru, C& c)
 {
-    load_b<VAR>(ru, c, 0);
+    load_b<VAR, PU>(ru, c, 0);
     load_a<VAR>(rv, c, 0);
     if constexpr (!Traits<VAR>::three) load_a<VAR>(rv, c, 1);
     xform_b(c, 0, 0);
@@ -159,7 +172,7 @@ __device__ inline void stage_operands(const unsigned* rv, const unsigned* ru, C&
 }
 
 // Slot S of a stage: MFMA S and the operand loads / transforms due there.
-template <int VAR, int S, class C>
+template <int VAR, int S, int PU = PLANE_U, class C>
 __device__ inline void mfma_slot(const unsigned* rv, const unsigned* ru, C& c)
 {
     constexpr bool THREE = Traits<VAR>::three;
@@ -169,7 +182,7 @@ __device__ inline void mfma_slot(const unsigned* rv, const unsigned* ru, C& c)
     c.acc[mb][t & 1] = mfma<VAR>(c.A[g][t < 4 ? 0 : 1], c.B[ks][t < 4 ? t : t - 4], c.acc[mb][t & 1]);
     if constexpr (t == 0 && THREE && g + 1 < 8) load_a<VAR>(rv, c, g + 1);
     if constexpr (t == 0 && !THREE && g + 2 < 8) load_a<VAR>(rv, c, g + 2);
-    if constexpr (S == NM / 4) load_b<VAR>(ru, c, 1);
+    if constexpr (S == NM / 4) load_b<VAR, PU>(ru, c, 1);
     if constexpr (S == NM * 3 / 8) xform_b(c, 1, 0);
     if constexpr (THREE && S == NM * 3 / 8 + 2) xform_b(c, 1, 1);
 }
@@ -198,20 +211,22 @@ struct GemmPlan {
     int tiles_x, tiles;
     int64_t splits, kslice, npx;
 };
-// kq: K granularity of a split in points (one stage)
-inline GemmPlan plan_gemm(const Geometry& g, int64_t npoints, int kq)
+// kq: K granularity of a split in points (one stage); bn: tile columns
+inline GemmPlan plan_gemm(const Geometry& g, int64_t npoints, int kq, int bn = gemm::BN)
 {
     using namespace gemm;
     GemmPlan p;
-    p.tiles_x = (g.pn_x + BN - 1) / BN;
+    p.tiles_x = (g.pn_x + bn - 1) / bn;
     p.tiles = p.tiles_x * ((g.pn_y + BM - 1) / BM);
     p.npx = (int64_t)g.pn_x * g.pn_y;
-    // Split K so that (a) every CU gets several workgroups (two reside per CU)
-    // and (b) no FP32 accumulator sums more than MAX_SLICE points: FP32
-    // rounding of the running sums grows like sqrt(points per slice).
+    // Split K so that (a) the 256 CUs get at least two rounds of workgroups
+    // (1024 of 128-column tiles, two resident per CU; 512 of 256-column
+    // tiles, one per CU) and (b) no FP32 accumulator sums more than MAX_SLICE
+    // points: FP32 rounding of the running sums grows like sqrt(points per slice).
     constexpr int64_t MAX_SLICE = 65536;
     const int64_t n = npoints > 1 ? npoints : 1;
-    int64_t splits = (1024 + p.tiles - 1) / p.tiles;
+    const int64_t min_wgs = bn > BN ? 512 : 1024;
+    int64_t splits = (min_wgs + p.tiles - 1) / p.tiles;
     if ((n + MAX_SLICE - 1) / MAX_SLICE > splits) splits = (n + MAX_SLICE - 1) / MAX_SLICE;
     int64_t max_by_k = (n + 4 * kq - 1) / (4 * kq);   // >= 4 stages per split
     if (max_by_k < 1) max_by_k = 1;

@@ -15,7 +15,8 @@
 // then does 8 points x 16 output columns' worth of complex MACs per instruction.
 //
 // Operands are never stored in HBM: per 16-point stage every thread generates
-// 8 V and 8 U elements into double-buffered LDS.  Along a thread's rows (and
+// its V rows and U columns into double-buffered LDS (4 and 8 on the 128 x 256
+// tiles of 8 waves, 8 and 8 on the 128 x 128 tiles of 4 waves).  Along a thread's rows (and
 // columns) the phase is a quadratic in the row index whose three coefficients
 // are computed in FP64 and reduced mod 1, so the FP32 evaluation (two FMAs,
 // then v_sin/v_cos) only sums terms below 16 cycles for any point depth.  Each
@@ -60,7 +61,8 @@ __device__ inline void load_raw(Raw& r, const PointSoA& s, const float4* __restr
 }
 
 // Per-(point, thread) constants of one stage.  Thread t generates V rows
-// grp + NGRP i (i < NVT) and U columns grp + NGRP j (j < NUT), grp = t / 16;
+// grp + NGRP i (i < NVT) and U columns grp + NGRP j (j < NUT), grp = t / 16
+// (NGRP, NVT, NUT of the tile shape);
 // with k = i - NVT/2 the phase (cycles) of its k-th row is exactly
 // v0 + k v1 + k^2 v2  (mod 1), a quadratic in the integer k, and likewise
 // u0 + k u1 + k^2 u2 (k = j - NUT/2) for its columns.  The three
@@ -90,21 +92,24 @@ __device__ inline void prep_x(Pt& P, const Raw& r, double xthr, double dx)
     P.u1 = (float)frac64(tx * (2.0 * dx));
     P.u2 = r.pc.y;
 }
-// amplitude and box; returns true when the box does not cover the whole tile
+// amplitude and box; returns true when the box does not cover the whole
+// BM x W tile
+template <int W>
 __device__ inline bool prep_rest(Pt& P, const Raw& r, int x0, int y0)
 {
     P.amp = r.valid ? r.pc.z : 0.f;
     P.rlo = r.valid ? r.box.z - y0 : 0;
     P.rhi = r.valid ? r.box.w - y0 : BM;
     P.clo = r.valid ? r.box.x - x0 : 0;
-    P.chi = r.valid ? r.box.y - x0 : BN;
-    return P.rlo > 0 || P.rhi < BM || P.clo > 0 || P.chi < BN;
+    P.chi = r.valid ? r.box.y - x0 : W;
+    return P.rlo > 0 || P.rhi < BM || P.clo > 0 || P.chi < W;
 }
+template <int W>
 __device__ inline bool prep(Pt& P, const Raw& r, double ythr, double xthr, double dy, double dx, int x0, int y0)
 {
     prep_y(P, r, ythr, dy);
     prep_x(P, r, xthr, dx);
-    return prep_rest(P, r, x0, y0);
+    return prep_rest<W>(P, r, x0, y0);
 }
 
 // The same work in NPIECE pieces of two or three FP64 operations, spread by the
@@ -157,64 +162,78 @@ __device__ inline void put(unsigned* hi_slot, unsigned* lo_slot, float re, float
 
 // A thread's NE elements per stage, V rows spread evenly among U columns:
 // element E is a V row when it raises the running V count.
-constexpr int v_before(int E) { return (E * NVT) / NE; }
-constexpr bool elem_is_v(int E) { return v_before(E + 1) > v_before(E); }
-constexpr int elem_index(int E) { return elem_is_v(E) ? v_before(E) : E - v_before(E); }
+template <int W>
+constexpr int v_before(int E)
+{
+    return (E * Shape<W>::NVT) / Shape<W>::NE;
+}
+template <int W>
+constexpr bool elem_is_v(int E)
+{
+    return v_before<W>(E + 1) > v_before<W>(E);
+}
+template <int W>
+constexpr int elem_index(int E)
+{
+    return elem_is_v<W>(E) ? v_before<W>(E) : E - v_before<W>(E);
+}
 
 // element E of the thread's stage: phase, sincos (V: times the amplitude)
-template <int E>
+template <int W, int E>
 __device__ inline float2 elem_value(const Pt& P)
 {
-    if constexpr (elem_is_v(E)) {
-        const float f = phase_at(elem_index(E) - NVT / 2, P.v0, P.v1, P.v2);
+    if constexpr (elem_is_v<W>(E)) {
+        const float f = phase_at(elem_index<W>(E) - Shape<W>::NVT / 2, P.v0, P.v1, P.v2);
         return make_float2(P.amp * cos_rev(f), P.amp * sin_rev(f));
     } else {
-        const float f = phase_at(elem_index(E) - NUT / 2, P.u0, P.u1, P.u2);
+        const float f = phase_at(elem_index<W>(E) - Shape<W>::NUT / 2, P.u0, P.u1, P.u2);
         return make_float2(cos_rev(f), sin_rev(f));
     }
 }
-template <int VAR, int E>
+template <int VAR, int W, int E>
 __device__ inline void elem_store(unsigned* wv, unsigned* wu, float2 cs, int p, int grp)
 {
-    const int e = (grp + NGRP * elem_index(E)) * ROWW + p;
-    if constexpr (elem_is_v(E)) put<VAR>(wv + e, wv + (Traits<VAR>::three ? PLANE_V : 0) + e, cs.x, cs.y);
-    else put<VAR>(wu + e, wu + (Traits<VAR>::three ? PLANE_U : 0) + e, cs.x, cs.y);
+    using Sh = Shape<W>;
+    const int e = (grp + Sh::NGRP * elem_index<W>(E)) * ROWW + p;
+    if constexpr (elem_is_v<W>(E)) put<VAR>(wv + e, wv + (Traits<VAR>::three ? PLANE_V : 0) + e, cs.x, cs.y);
+    else put<VAR>(wu + e, wu + (Traits<VAR>::three ? Sh::PLANE_U : 0) + e, cs.x, cs.y);
 }
 
-template <int VAR, int E>
+template <int VAR, int W, int E>
 __device__ inline void gen_elems(unsigned* wv, unsigned* wu, const Pt& P, int p, int grp)
 {
-    if constexpr (E < NE) {
-        elem_store<VAR, E>(wv, wu, elem_value<E>(P), p, grp);
-        gen_elems<VAR, E + 1>(wv, wu, P, p, grp);
+    if constexpr (E < Shape<W>::NE) {
+        elem_store<VAR, W, E>(wv, wu, elem_value<W, E>(P), p, grp);
+        gen_elems<VAR, W, E + 1>(wv, wu, P, p, grp);
     }
 }
 
 // Generation of one stage into buffer WB, ignoring the boxes.
-template <int VAR, int WB>
-__device__ inline void gen_stage(Smem<Traits<VAR>::three>& sm, const Pt& P, int p, int grp)
+template <int VAR, int W, int WB>
+__device__ inline void gen_stage(Smem<Traits<VAR>::three, W>& sm, const Pt& P, int p, int grp)
 {
-    gen_elems<VAR, 0>(&sm.v[WB][0][0][0], &sm.u[WB][0][0][0], P, p, grp);
+    gen_elems<VAR, W, 0>(&sm.v[WB][0][0][0], &sm.u[WB][0][0][0], P, p, grp);
 }
 
 // The boxes: zeroes the elements of rows / columns outside the point's box
 // (the same thread wrote them, so program order makes the zeros win).  Run
 // only by waves holding a point whose box does not cover the tile.
-template <int VAR, int WB>
-__device__ inline void mask_stage(Smem<Traits<VAR>::three>& sm, const Pt& P, int p, int grp)
+template <int VAR, int W, int WB>
+__device__ inline void mask_stage(Smem<Traits<VAR>::three, W>& sm, const Pt& P, int p, int grp)
 {
+    using Sh = Shape<W>;
     constexpr int H2 = Traits<VAR>::three ? 1 : 0;
 #pragma unroll
-    for (int i = 0; i < NVT; ++i) {
-        const int r = grp + NGRP * i;
+    for (int i = 0; i < Sh::NVT; ++i) {
+        const int r = grp + Sh::NGRP * i;
         if (r < P.rlo || r >= P.rhi) {
             sm.v[WB][0][r][p] = 0u;
             sm.v[WB][H2][r][p] = 0u;
         }
     }
 #pragma unroll
-    for (int j = 0; j < NUT; ++j) {
-        const int c = grp + NGRP * j;
+    for (int j = 0; j < Sh::NUT; ++j) {
+        const int c = grp + Sh::NGRP * j;
         if (c < P.clo || c >= P.chi) {
             sm.u[WB][0][c][p] = 0u;
             sm.u[WB][H2][c][p] = 0u;
@@ -223,7 +242,7 @@ __device__ inline void mask_stage(Smem<Traits<VAR>::three>& sm, const Pt& P, int
 }
 
 // State of one pipelined stage (see stage_body).
-template <int VAR>
+template <int VAR, int W>
 struct StageCtx {
     f32x16 (&acc)[4][2];
     Raw& R;
@@ -237,7 +256,7 @@ struct StageCtx {
     double t[4];   // FP64 temporaries shared by the prep pieces
     bool partial;
     u32x4 A[8][2], Ub[2][2], B[2][4];
-    float2 cs[NE];
+    float2 cs[Shape<W>::NE];
 };
 
 // The generation schedule: 3-term, half-element h (even: value of element
@@ -249,67 +268,75 @@ struct StageCtx {
 // per slot (slots 0 .. NPIECE/2 - 1), amplitude/box at NPIECE/2, elements from
 // NPIECE/2 + 1 to 15.
 constexpr int H0 = NPIECE + 1, E0 = NPIECE / 2 + 1;
+template <int W>
 constexpr int half_at(int S)
 {
+    constexpr int NE = Shape<W>::NE;
     for (int h = 0; h < 2 * NE; ++h)
         if (H0 + ((46 - H0) * h) / (2 * NE) == S) return h;
     return -1;
 }
+template <int W>
 constexpr int elem_at(int S, int n)   // n-th element at slot S (slots hold at most two)
 {
+    constexpr int NE = Shape<W>::NE;
     for (int e = 0; e < NE; ++e)
         if (E0 + ((16 - E0) * e) / NE == S && n-- == 0) return e;
     return -1;
 }
 // every element (1-term) and half-element (3-term) lands in exactly one slot
+template <int W>
 constexpr bool schedule_complete()
 {
+    constexpr int NE = Shape<W>::NE;
     for (int e = 0; e < NE; ++e) {
         int n = 0;
-        for (int S = 0; S < 16; ++S) n += (elem_at(S, 0) == e) + (elem_at(S, 1) == e) + (elem_at(S, 2) == e);
+        for (int S = 0; S < 16; ++S)
+            n += (elem_at<W>(S, 0) == e) + (elem_at<W>(S, 1) == e) + (elem_at<W>(S, 2) == e);
         if (n != 1) return false;
     }
     for (int S = 0; S < 16; ++S)
-        if (elem_at(S, 2) >= 0) return false;
+        if (elem_at<W>(S, 2) >= 0) return false;
     for (int h = 0; h < 2 * NE; ++h) {
         int n = 0;
-        for (int S = 0; S < 48; ++S) n += half_at(S) == h;
+        for (int S = 0; S < 48; ++S) n += half_at<W>(S) == h;
         if (n != 1) return false;
     }
     return true;
 }
-static_assert(schedule_complete(), "generation schedule must cover every element once");
+static_assert(schedule_complete<128>() && schedule_complete<256>(),
+              "generation schedule must cover every element once");
 
 // Slot S of the pipelined stage: MFMA S, operand loads / transforms due at
 // this slot, and this slot's share of the next stage's generation.
-template <int VAR, int S>
+template <int VAR, int W, int S>
 __device__ inline void slot(const unsigned* __restrict__ rv, const unsigned* __restrict__ ru,
-                            unsigned* __restrict__ wv, unsigned* __restrict__ wu, StageCtx<VAR>& c)
+                            unsigned* __restrict__ wv, unsigned* __restrict__ wu, StageCtx<VAR, W>& c)
 {
     constexpr bool THREE = Traits<VAR>::three;
     constexpr int NM = mfma_per_stage<VAR>();
     if constexpr (S < NM) {
-        mfma_slot<VAR, S>(rv, ru, c);
+        mfma_slot<VAR, S, Shape<W>::PLANE_U>(rv, ru, c);
         if constexpr (THREE && S < NPIECE) prep_piece<S>(c.P, c.t, c.R, c.ythr, c.xthr, c.dy, c.dx);
         if constexpr (!THREE && S < NPIECE / 2) {
             prep_piece<2 * S>(c.P, c.t, c.R, c.ythr, c.xthr, c.dy, c.dx);
             prep_piece<2 * S + 1>(c.P, c.t, c.R, c.ythr, c.xthr, c.dy, c.dx);
         }
         if constexpr (S == (THREE ? NPIECE : NPIECE / 2)) {
-            c.partial = prep_rest(c.P, c.R, c.x0, c.y0);
+            c.partial = prep_rest<W>(c.P, c.R, c.x0, c.y0);
             load_raw(c.R, c.s, c.pconst, c.kfirst, c.pnext, c.ke);
         }
         if constexpr (THREE) {
-            constexpr int h = half_at(S);
-            if constexpr (h >= 0 && (h & 1) == 0) c.cs[h >> 1] = elem_value<(h >> 1)>(c.P);
-            if constexpr (h >= 0 && (h & 1) == 1) elem_store<VAR, (h >> 1)>(wv, wu, c.cs[h >> 1], c.p, c.grp);
+            constexpr int h = half_at<W>(S);
+            if constexpr (h >= 0 && (h & 1) == 0) c.cs[h >> 1] = elem_value<W, (h >> 1)>(c.P);
+            if constexpr (h >= 0 && (h & 1) == 1) elem_store<VAR, W, (h >> 1)>(wv, wu, c.cs[h >> 1], c.p, c.grp);
         } else {
-            constexpr int e0 = elem_at(S, 0), e1 = elem_at(S, 1);
-            if constexpr (e0 >= 0) elem_store<VAR, e0>(wv, wu, elem_value<e0>(c.P), c.p, c.grp);
-            if constexpr (e1 >= 0) elem_store<VAR, e1>(wv, wu, elem_value<e1>(c.P), c.p, c.grp);
+            constexpr int e0 = elem_at<W>(S, 0), e1 = elem_at<W>(S, 1);
+            if constexpr (e0 >= 0) elem_store<VAR, W, e0>(wv, wu, elem_value<W, e0>(c.P), c.p, c.grp);
+            if constexpr (e1 >= 0) elem_store<VAR, W, e1>(wv, wu, elem_value<W, e1>(c.P), c.p, c.grp);
         }
         __builtin_amdgcn_sched_barrier(0);
-        slot<VAR, S + 1>(rv, ru, wv, wu, c);
+        slot<VAR, W, S + 1>(rv, ru, wv, wu, c);
     }
 }
 
@@ -321,17 +348,17 @@ __device__ inline void slot(const unsigned* __restrict__ rv, const unsigned* __r
 // generation's LDS writes never alias the operand reads.  Also loads the
 // point data of the stage after next into R; returns true when this lane's
 // point has a box that does not cover the tile (P then holds the box).
-template <int VAR>
+template <int VAR, int W>
 __device__ inline bool stage_body(const unsigned* __restrict__ rv, const unsigned* __restrict__ ru,
                                   unsigned* __restrict__ wv, unsigned* __restrict__ wu, f32x16 (&acc)[4][2],
                                   Pt& P, Raw& R, const PointSoA& s, const float4* pconst, int64_t kfirst,
                                   int64_t pnext, int64_t ke, double ythr, double xthr, double dy, double dx, int x0,
                                   int y0, int p, int grp, int lrow, int koff, int ucol)
 {
-    StageCtx<VAR> c{acc, R, s, pconst, kfirst, pnext, ke, ythr, xthr, dy, dx, x0, y0, p, grp, lrow, koff, ucol};
+    StageCtx<VAR, W> c{acc, R, s, pconst, kfirst, pnext, ke, ythr, xthr, dy, dx, x0, y0, p, grp, lrow, koff, ucol};
     c.partial = false;
-    stage_operands<VAR>(rv, ru, c);
-    slot<VAR, 0>(rv, ru, wv, wu, c);
+    stage_operands<VAR, Shape<W>::PLANE_U>(rv, ru, c);
+    slot<VAR, W, 0>(rv, ru, wv, wu, c);
     P = c.P;
     return c.partial;
 }
@@ -344,19 +371,19 @@ __device__ inline bool stage_body(const unsigned* __restrict__ rv, const unsigne
 // the launch has a box smaller than the hologram, so waves holding a point
 // whose box does not cover the tile zero its out-of-box elements (a ballot
 // and a branch per stage, ~5 % of the kernel, paid only by such launches).
-template <int VAR, int RB, bool MASK>
-__device__ inline void step(Smem<Traits<VAR>::three>& sm, f32x16 (&acc)[4][2], Raw& R, const PointSoA& s,
+template <int VAR, int W, int RB, bool MASK>
+__device__ inline void step(Smem<Traits<VAR>::three, W>& sm, f32x16 (&acc)[4][2], Raw& R, const PointSoA& s,
                             const float4* pconst, int64_t kfirst, int st, int64_t kb, int64_t ke, double ythr,
                             double xthr, double dy, double dx, int x0, int y0, int p, int grp, int lrow, int koff,
                             int ucolx)
 {
     Pt P;
-    const bool partial = stage_body<VAR>(&sm.v[RB][0][0][0], &sm.u[RB][0][0][0], &sm.v[RB ^ 1][0][0][0],
+    const bool partial = stage_body<VAR, W>(&sm.v[RB][0][0][0], &sm.u[RB][0][0][0], &sm.v[RB ^ 1][0][0][0],
                                          &sm.u[RB ^ 1][0][0][0], acc, P, R, s, pconst, kfirst,
                                          kb + (int64_t)(st + 2) * KP + p, ke, ythr, xthr, dy, dx, x0, y0, p, grp,
                                          lrow, koff, ucolx);
     if constexpr (MASK) {
-        if (__builtin_amdgcn_ballot_w64(partial)) mask_stage<VAR, RB ^ 1>(sm, P, p, grp);
+        if (__builtin_amdgcn_ballot_w64(partial)) mask_stage<VAR, W, RB ^ 1>(sm, P, p, grp);
     } else {
         (void)partial;
     }
@@ -364,8 +391,8 @@ __device__ inline void step(Smem<Traits<VAR>::three>& sm, f32x16 (&acc)[4][2], R
 }
 
 // One workgroup's tile and split: prologue, K loop, FP32 partials.
-template <int VAR, bool MASK>
-__device__ __attribute__((always_inline)) inline void fresnel_tile(Smem<Traits<VAR>::three>& sm, const Geometry& g,
+template <int VAR, int W, bool MASK>
+__device__ __attribute__((always_inline)) inline void fresnel_tile(Smem<Traits<VAR>::three, W>& sm, const Geometry& g,
                                                                    const PointSoA& s, const float4* pconst,
                                                                    float2* part, int tiles_x, int tiles,
                                                                    int64_t kfirst, int64_t klast, int64_t kslice)
@@ -373,13 +400,15 @@ __device__ __attribute__((always_inline)) inline void fresnel_tile(Smem<Traits<V
     const int tile = blockIdx.x % tiles;
     const int split = blockIdx.x / tiles;
     const int tx = tile % tiles_x, ty = tile / tiles_x;
-    const int x0 = tx * BN, y0 = ty * BM;
+    using Sh = Shape<W>;
+    const int x0 = tx * W, y0 = ty * BM;
     const int64_t kb = kfirst + (int64_t)split * kslice;
     const int64_t ke = min(klast, kb + kslice);
     const int nst = (int)((ke - kb + KP - 1) / KP);
     const int p = threadIdx.x & (KP - 1), grp = threadIdx.x >> 4;
-    const double ythr = gpix_y(g, y0 + grp + NGRP * (NVT / 2)), xthr = gpix_x(g, x0 + grp + NGRP * (NUT / 2));
-    const double dy = -NGRP * g.pp_y, dx = NGRP * g.pp_x;
+    const double ythr = gpix_y(g, y0 + grp + Sh::NGRP * (Sh::NVT / 2));
+    const double xthr = gpix_x(g, x0 + grp + Sh::NGRP * (Sh::NUT / 2));
+    const double dy = -Sh::NGRP * g.pp_y, dx = Sh::NGRP * g.pp_x;
 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int lrow = lane & 31, koff = (lane >> 5) * 4;   // dword offset of this lane's 8 values
@@ -391,10 +420,10 @@ __device__ __attribute__((always_inline)) inline void fresnel_tile(Smem<Traits<V
     if (nst > 0) {
         load_raw(R, s, pconst, kfirst, kb + p, ke);
         Pt P;
-        const bool partial = prep(P, R, ythr, xthr, dy, dx, x0, y0);
-        gen_stage<VAR, 0>(sm, P, p, grp);
+        const bool partial = prep<W>(P, R, ythr, xthr, dy, dx, x0, y0);
+        gen_stage<VAR, W, 0>(sm, P, p, grp);
         if constexpr (MASK) {
-            if (partial) mask_stage<VAR, 0>(sm, P, p, grp);
+            if (partial) mask_stage<VAR, W, 0>(sm, P, p, grp);
         } else {
             (void)partial;
         }
@@ -405,10 +434,10 @@ __device__ __attribute__((always_inline)) inline void fresnel_tile(Smem<Traits<V
 #pragma unroll
     for (int i = 0; i < 4; ++i) acc[i][0] = acc[i][1] = f32x16{};
     for (int st = 0; st < nst; st += 2) {
-        step<VAR, 0, MASK>(sm, acc, R, s, pconst, kfirst, st, kb, ke, ythr, xthr, dy, dx, x0, y0, p, grp, lrow,
+        step<VAR, W, 0, MASK>(sm, acc, R, s, pconst, kfirst, st, kb, ke, ythr, xthr, dy, dx, x0, y0, p, grp, lrow,
                            koff, ucolx);
         if (st + 1 >= nst) break;
-        step<VAR, 1, MASK>(sm, acc, R, s, pconst, kfirst, st + 1, kb, ke, ythr, xthr, dy, dx, x0, y0, p, grp, lrow,
+        step<VAR, W, 1, MASK>(sm, acc, R, s, pconst, kfirst, st + 1, kb, ke, ythr, xthr, dy, dx, x0, y0, p, grp, lrow,
                            koff, ucolx);
     }
 
@@ -419,15 +448,16 @@ __device__ __attribute__((always_inline)) inline void fresnel_tile(Smem<Traits<V
 // Both MASK instances are launched; the one that does not match
 // *partial_flag (!= 0: some point has a box smaller than the hologram) returns
 // at once, so each keeps its own register allocation.
-template <int VAR, bool MASK>
-__global__ __launch_bounds__(NT, 2) void k_fresnel_gemm(Geometry g, PointSoA s, const float4* __restrict__ pconst,
-                                                        const unsigned* __restrict__ partial_flag,
-                                                        float2* __restrict__ part, int tiles_x, int tiles,
-                                                        int64_t kfirst, int64_t klast, int64_t kslice)
+// Two waves per SIMD either way: two 4-wave workgroups (W = 128) or one
+// 8-wave workgroup (W = 256) per CU.
+template <int VAR, int W, bool MASK>
+__global__ __launch_bounds__(Shape<W>::NT, 512 / Shape<W>::NT) void k_fresnel_gemm(
+    Geometry g, PointSoA s, const float4* __restrict__ pconst, const unsigned* __restrict__ partial_flag,
+    float2* __restrict__ part, int tiles_x, int tiles, int64_t kfirst, int64_t klast, int64_t kslice)
 {
-    __shared__ __align__(16) Smem<Traits<VAR>::three> sm;
+    __shared__ __align__(16) Smem<Traits<VAR>::three, W> sm;
     if ((__builtin_amdgcn_readfirstlane(partial_flag[0]) != 0) != MASK) return;
-    fresnel_tile<VAR, MASK>(sm, g, s, pconst, part, tiles_x, tiles, kfirst, klast, kslice);
+    fresnel_tile<VAR, W, MASK>(sm, g, s, pconst, part, tiles_x, tiles, kfirst, klast, kslice);
 }
 
 // field += -j * amax * sum_split S   (FP64, fixed summation order)
@@ -473,23 +503,26 @@ __global__ void k_amp_init(float* amax)
 
 // Per-point constants of the stage generation, once per launch instead of once
 // per (point, tile, thread): the k^2 coefficients v2 = frac(q dy^2) and
-// u2 = frac(q dx^2) (dy = -NGRP pp_y, dx = NGRP pp_x; FP64 then reduced mod 1
-// and rounded) and the amplitude a/(lambda z), times 1/max|A| for FP16.  Also
+// u2 = frac(q dx^2) (dy = -ngrp pp_y, dx = ngrp pp_x, ngrp the tile shape's
+// thread groups, 16 into pc_narrow and 32 into pc_wide; FP64 then reduced
+// mod 1 and rounded) and the amplitude a/(lambda z), times 1/max|A| for FP16.  Also
 // raises *partial_flag when a point's box is smaller than the hologram (one
 // atomic per wave holding such a point, skipped once the flag reads as set).
 __global__ __launch_bounds__(256) void k_gemm_consts(Geometry g, PointSoA s, int64_t p0, int64_t p1,
                                                      const float* __restrict__ amax, int scaled,
-                                                     float4* __restrict__ pconst, unsigned* partial_flag)
+                                                     float4* __restrict__ pc_narrow, float4* __restrict__ pc_wide,
+                                                     unsigned* partial_flag)
 {
     const int64_t i = p0 + (int64_t)blockIdx.x * 256 + threadIdx.x;
     bool partial = false;
     if (i < p1) {
-        const double dy = -NGRP * g.pp_y, dx = NGRP * g.pp_x;
         const double q = s.q[i];
         const float am = amax[0];
-        const float vscale = (scaled && am > 0.f) ? 1.0f / am : 1.0f;
-        pconst[i - p0] = make_float4((float)frac64(q * (dy * dy)), (float)frac64(q * (dx * dx)),
-                                     (float)s.kamp[i] * vscale, 0.f);
+        const float a = (float)s.kamp[i] * ((scaled && am > 0.f) ? 1.0f / am : 1.0f);
+        const double dy = -Shape<128>::NGRP * g.pp_y, dx = Shape<128>::NGRP * g.pp_x;
+        const double dy2 = q * (dy * dy), dx2 = q * (dx * dx);
+        pc_narrow[i - p0] = make_float4((float)frac64(dy2), (float)frac64(dx2), a, 0.f);
+        pc_wide[i - p0] = make_float4((float)frac64(4.0 * dy2), (float)frac64(4.0 * dx2), a, 0.f);
         const int4 b = s.box[i];
         partial = b.x > 0 || b.y < g.pn_x || b.z > 0 || b.w < g.pn_y;
     }
@@ -501,14 +534,58 @@ __global__ __launch_bounds__(256) void k_gemm_consts(Geometry g, PointSoA s, int
 }  // namespace gemm
 
 
-// scratch: [amax, 256 B] [per-point float4 constants, padded to 256 B]
-// [split-K partial planes]
+// Tile width of each kernel instance: 256 columns (a quarter fewer generated
+// elements per point-pixel pair than 128), except the masked 3-term
+// instances, whose box pass does not fit the wide tile's registers; they keep
+// 128-column tiles and use the same split-K plan, so the partial planes and
+// their reduction do not depend on which instance runs.
+template <int VAR, bool MASK>
+constexpr int tile_width()
+{
+    return (MASK && gemm::Traits<VAR>::three) ? 128 : 256;
+}
+
+// scratch: [amax, 256 B] [per-point float4 constants for 128- and
+// 256-column tiles, each padded to 256 B] [split-K partial planes]
 static size_t consts_bytes(int64_t npoints) { return ((size_t)npoints * sizeof(float4) + 255) / 256 * 256; }
+
+// The split-K plan of a launch (that of the 256-column tiles).
+static GemmPlan fresnel_plan(const Geometry& g, int64_t npoints) { return plan_gemm(g, npoints, gemm::KP, 256); }
 
 size_t fresnel_gemm_scratch_bytes(const Geometry& g, int64_t npoints)
 {
-    const GemmPlan p = plan_gemm(g, npoints, gemm::KP);
-    return 256 + consts_bytes(npoints) + (size_t)p.splits * p.npx * sizeof(float2);
+    const GemmPlan p = fresnel_plan(g, npoints);
+    return 256 + 2 * consts_bytes(npoints) + (size_t)p.splits * p.npx * sizeof(float2);
+}
+
+template <int VAR, bool MASK>
+static void launch_instance(hipStream_t stream, const Geometry& g, const PointSoA& soa, const float4* pc_narrow,
+                            const float4* pc_wide, const unsigned* pf, float2* part, const GemmPlan& pl, int64_t p0,
+                            int64_t p1)
+{
+    using namespace gemm;
+    constexpr int W = tile_width<VAR, MASK>();
+    const int tiles_x = (g.pn_x + W - 1) / W, tiles = tiles_x * ((g.pn_y + BM - 1) / BM);
+    const dim3 grid((unsigned)((int64_t)tiles * pl.splits)), block(Shape<W>::NT);
+    hipLaunchKernelGGL((k_fresnel_gemm<VAR, W, MASK>), grid, block, 0, stream, g, soa,
+                       W == 256 ? pc_wide : pc_narrow, pf, part, tiles_x, tiles, p0, p1, pl.kslice);
+}
+
+// Both instances; the one that does not match the device's partial-box flag
+// returns at once.
+template <int VAR>
+static void launch_variant(hipStream_t stream, const Geometry& g, const PointSoA& soa, const float4* pc_narrow,
+                           const float4* pc_wide, const unsigned* pf, float2* part, const GemmPlan& pl, int64_t p0,
+                           int64_t p1)
+{
+    launch_instance<VAR, false>(stream, g, soa, pc_narrow, pc_wide, pf, part, pl, p0, p1);
+    launch_instance<VAR, true>(stream, g, soa, pc_narrow, pc_wide, pf, part, pl, p0, p1);
+}
+
+static int variant_of(uint32_t mode)
+{
+    using namespace gemm;
+    return (mode & OHX_MODE_MFMA_FP16X3) ? V_FP16X3 : (mode & OHX_MODE_MFMA_BF16X3) ? V_BF16X3 : V_BF16X1;
 }
 
 int launch_fresnel_gemm_range(hipStream_t stream, void* scratch, size_t scratch_bytes, const Geometry& g,
@@ -516,37 +593,28 @@ int launch_fresnel_gemm_range(hipStream_t stream, void* scratch, size_t scratch_
 {
     using namespace gemm;
     if (p1 <= p0) return OHX_OK;
-    const int var = (mode & OHX_MODE_MFMA_FP16X3) ? V_FP16X3 : (mode & OHX_MODE_MFMA_BF16X3) ? V_BF16X3 : V_BF16X1;
-    const GemmPlan pl = plan_gemm(g, p1 - p0, KP);
+    const int var = variant_of(mode);
+    const GemmPlan pl = fresnel_plan(g, p1 - p0);
     if (fresnel_gemm_scratch_bytes(g, p1 - p0) > scratch_bytes) return OHX_ERR_INVALID_ARG;
     float* amax = (float*)scratch;
-    float4* pconst = (float4*)((char*)scratch + 256);
-    float2* part = (float2*)((char*)scratch + 256 + consts_bytes(p1 - p0));
+    float4* pc_narrow = (float4*)((char*)scratch + 256);
+    float4* pc_wide = (float4*)((char*)scratch + 256 + consts_bytes(p1 - p0));
+    float2* part = (float2*)((char*)scratch + 256 + 2 * consts_bytes(p1 - p0));
     hipLaunchKernelGGL(k_amp_init, dim3(1), dim3(64), 0, stream, amax);
     hipLaunchKernelGGL(k_amp_max, dim3(256), dim3(256), 0, stream, (const double*)soa.kamp, p0, p1, amax);
     unsigned* partial_flag = reinterpret_cast<unsigned*>(amax) + 1;
     hipLaunchKernelGGL(k_gemm_consts, dim3((unsigned)((p1 - p0 + 255) / 256)), dim3(256), 0, stream, g, soa, p0,
-                       p1, (const float*)amax, var == V_FP16X3 ? 1 : 0, pconst, partial_flag);
-    const dim3 grid((unsigned)((int64_t)pl.tiles * pl.splits)), block(NT);
+                       p1, (const float*)amax, var == V_FP16X3 ? 1 : 0, pc_narrow, pc_wide, partial_flag);
     const unsigned* pf = partial_flag;
     switch (var) {
     case V_BF16X1:
-        hipLaunchKernelGGL((k_fresnel_gemm<V_BF16X1, false>), grid, block, 0, stream, g, soa, (const float4*)pconst,
-                           pf, part, pl.tiles_x, pl.tiles, p0, p1, pl.kslice);
-        hipLaunchKernelGGL((k_fresnel_gemm<V_BF16X1, true>), grid, block, 0, stream, g, soa, (const float4*)pconst,
-                           pf, part, pl.tiles_x, pl.tiles, p0, p1, pl.kslice);
+        launch_variant<V_BF16X1>(stream, g, soa, pc_narrow, pc_wide, pf, part, pl, p0, p1);
         break;
     case V_BF16X3:
-        hipLaunchKernelGGL((k_fresnel_gemm<V_BF16X3, false>), grid, block, 0, stream, g, soa, (const float4*)pconst,
-                           pf, part, pl.tiles_x, pl.tiles, p0, p1, pl.kslice);
-        hipLaunchKernelGGL((k_fresnel_gemm<V_BF16X3, true>), grid, block, 0, stream, g, soa, (const float4*)pconst,
-                           pf, part, pl.tiles_x, pl.tiles, p0, p1, pl.kslice);
+        launch_variant<V_BF16X3>(stream, g, soa, pc_narrow, pc_wide, pf, part, pl, p0, p1);
         break;
     default:
-        hipLaunchKernelGGL((k_fresnel_gemm<V_FP16X3, false>), grid, block, 0, stream, g, soa, (const float4*)pconst,
-                           pf, part, pl.tiles_x, pl.tiles, p0, p1, pl.kslice);
-        hipLaunchKernelGGL((k_fresnel_gemm<V_FP16X3, true>), grid, block, 0, stream, g, soa, (const float4*)pconst,
-                           pf, part, pl.tiles_x, pl.tiles, p0, p1, pl.kslice);
+        launch_variant<V_FP16X3>(stream, g, soa, pc_narrow, pc_wide, pf, part, pl, p0, p1);
         break;
     }
     if (hipGetLastError() != hipSuccess) return OHX_ERR_DEVICE;
@@ -586,7 +654,7 @@ int launch_fresnel_gemm(ohx_ctx* ctx, const Geometry& g, const PointSoA& soa, do
         const int64_t p1 = std::min<int64_t>(soa.n, p0 + chunk);
         st = launch_fresnel_gemm_range(ctx->stream, ctx->gemm_scratch, ctx->gemm_scratch_bytes, g, soa, p0, p1,
                                        d_field, mode);
-        const GemmPlan pl = plan_gemm(g, p1 - p0, gemm::KP);
+        const GemmPlan pl = fresnel_plan(g, p1 - p0);   // the 256-column instance's grid
         wgs += (int64_t)pl.tiles * pl.splits;
     }
     ctx->last.workgroups = wgs;

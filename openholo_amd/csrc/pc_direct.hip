@@ -469,9 +469,24 @@ __device__ __attribute__((always_inline)) inline void direct_tile(DirectSmem<DIF
 #pragma unroll
         for (int k = 0; k < RPT; ++k) pr[k] = pim[k] = 0.f;
         for (int j = 0; j < nF; ++j) {
+            // RS: the point's own constants are read first and all reads are
+            // issued before any arithmetic, so that the column factor (FP32
+            // chain, v_sin, v_cos) waits only for its own constants while the
+            // row factors are on their way (3 % faster; the lighter Fresnel
+            // loop is 2 % faster left to the scheduler)
             const float4 c = s_c0[j];
+            float4 d, n0, n1;   // RS only
+            if constexpr (DIFF == OHX_DIFF_RS) {
+                d = s_c1[j];
+                __builtin_amdgcn_sched_barrier(0);
+            }
             const float4* Rp = (const float4*)&s_R[wave][j][0];
             const float4 r0 = Rp[0], r1 = Rp[1], r2 = Rp[2], r3 = Rp[3];
+            if constexpr (DIFF == OHX_DIFF_RS) {
+                n0 = *(const float4*)&sm.Ny[wave][j][0];
+                n1 = *(const float4*)&sm.Ny[wave][j][4];
+                __builtin_amdgcn_sched_barrier(0);
+            }
             const float Rx[RPT] = {r0.x, r0.z, r1.x, r1.z, r2.x, r2.z, r3.x, r3.z};
             const float Ry[RPT] = {r0.y, r0.w, r1.y, r1.w, r2.y, r2.w, r3.y, r3.w};
             if (DIFF == OHX_DIFF_FRESNEL) {
@@ -483,8 +498,6 @@ __device__ __attribute__((always_inline)) inline void direct_tile(DirectSmem<DIF
                     pim[k] = fmaf(ca, Rx[k], fmaf(-sa, Ry[k], pim[k]));
                 }
             } else {
-                const float4 d = s_c1[j];
-                const float4 n0 = *(const float4*)&sm.Ny[wave][j][0], n1 = *(const float4*)&sm.Ny[wave][j][4];
                 const float Nv[RPT] = {n0.x, n0.y, n0.z, n0.w, n1.x, n1.y, n1.z, n1.w};
                 const float nx = fmaf(c.y, u, uu);
                 const float fa = fmaf(fmaf(c.z, nx, c.w), nx, c.x);

@@ -16,10 +16,11 @@
 //
 // Operands are never stored in HBM: per 16-point stage every thread generates
 // its V rows and U columns into double-buffered LDS (4 and 8 on the 128 x 256
-// tiles of 8 waves, 8 and 8 on the 128 x 128 tiles of 4 waves).  Along a thread's rows (and
-// columns) the phase is a quadratic in the row index whose three coefficients
-// are computed in FP64 and reduced mod 1, so the FP32 evaluation (two FMAs,
-// then v_sin/v_cos) only sums terms below 16 cycles for any point depth.  Each
+// tiles of 8 waves, 8 and 8 on the 128 x 128 tiles of 4 waves).  Along a
+// thread's rows (and columns) the phase is a quadratic in the row index whose
+// three coefficients are computed in FP64 and reduced mod 1, so the FP32
+// evaluation (two FMAs, then v_sin/v_cos) only sums terms below 16 cycles for
+// any point depth.  Each
 // wave's generation of stage s+1 is software-pipelined with its MFMAs of
 // stage s, slot by slot (a few VALU instructions after every MFMA), so that
 // the VALU work issues while the matrix core runs.  Precision variants:
@@ -434,11 +435,11 @@ __device__ __attribute__((always_inline)) inline void fresnel_tile(Smem<Traits<V
 #pragma unroll
     for (int i = 0; i < 4; ++i) acc[i][0] = acc[i][1] = f32x16{};
     for (int st = 0; st < nst; st += 2) {
-        step<VAR, W, 0, MASK>(sm, acc, R, s, pconst, kfirst, st, kb, ke, ythr, xthr, dy, dx, x0, y0, p, grp, lrow,
-                           koff, ucolx);
+        step<VAR, W, 0, MASK>(sm, acc, R, s, pconst, kfirst, st, kb, ke, ythr, xthr, dy, dx, x0, y0, p, grp,
+                              lrow, koff, ucolx);
         if (st + 1 >= nst) break;
-        step<VAR, W, 1, MASK>(sm, acc, R, s, pconst, kfirst, st + 1, kb, ke, ythr, xthr, dy, dx, x0, y0, p, grp, lrow,
-                           koff, ucolx);
+        step<VAR, W, 1, MASK>(sm, acc, R, s, pconst, kfirst, st + 1, kb, ke, ythr, xthr, dy, dx, x0, y0, p, grp,
+                              lrow, koff, ucolx);
     }
 
     store_partials(g, acc, part, split, x0, y0, ucolx, lane);

@@ -47,9 +47,11 @@ enum { OHX_DIFF_RS = 0, OHX_DIFF_FRESNEL = 1 };
  *   MFMA_*                   : Fresnel: H = U diag(a) V^T on matrix cores.
  *                              RS with BF16X1 / BF16X3 (and not FP16X3): points
  *                              whose mask is their box (the range test holds
- *                              on all of it) and whose rank-2 tile expansion
- *                              holds run on matrix cores, the others (and RS
- *                              with FP16X3) run the FP32 direct sum. */
+ *                              on all of it) run on matrix cores on the
+ *                              128x128 tiles where their rank-2 expansion
+ *                              holds and on the FP32 direct sum elsewhere; the
+ *                              other points (and RS with FP16X3) run the FP32
+ *                              direct sum. */
 enum {
     OHX_MODE_GPU = 1,
     OHX_MODE_FLOAT = 2,
@@ -255,8 +257,9 @@ typedef struct ohx_launch_stats {
                                 runs when every box covers the hologram)      */
     int32_t kernel_kind;     /* 0 FP32 direct, 1 FP64 direct, 2 MFMA GEMM, 3 WRP */
     int32_t gemm_points;     /* RS with an MFMA mode: points that ran on the
-                                matrix cores (the others ran the FP32 direct
-                                sum); 0 otherwise                            */
+                                matrix cores, on some or all of their tiles
+                                (the others ran the FP32 direct sum); 0
+                                otherwise                                    */
     int64_t pairs_in_box;    /* point-pixel pairs inside the points' anti-
                                 aliasing boxes (ophGen.cpp:384-407,467-490):
                                 the pairs the kernels evaluate; the RS range

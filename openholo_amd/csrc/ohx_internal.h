@@ -82,7 +82,65 @@ struct Geometry {
     double distance;
     int32_t channel, use_phase;
     int32_t diff;
+    // RS direct kernels only: > 0 skips the (point, tile) pairs whose 128 x 128
+    // matrix-core tile passes rs_bound_ok at this theta bound (the matrix-core
+    // kernel took them); 0 (the default) skips nothing
+    double rs_skip_theta;
 };
+
+// ---- RS on the matrix cores: the expansion bound on one tile ----------------
+// The rank-2 expansion of the RS term about the reference pixel (offset 64, 64)
+// of a 128 x 128 tile neglects terms that are small when, with dxc, dyc the
+// point's distances to that pixel and u, v <= 64 pitches, n = um (2 dxc + um),
+// N = vm (2 dyc + vm), e = (n + N) / z^2 and k = 1/(8 z^3 lambda) (r_c >= z):
+// third-order phase e^3 z/(16 lambda) <= RS_MAX_PHASE3 cycles, second-order
+// amplitude e^2 <= RS_MAX_AMP2 and cross term 4 pi k n N <= max_theta rad.
+// The classification bounds every tile of a box with the largest dxc, dyc;
+// the masked matrix-core kernel drops the tiles where the bound fails and the
+// direct kernel (Geometry::rs_skip_theta) takes exactly those: both evaluate
+// rs_tile_ok in FP64 without contraction, so they agree bit for bit.
+constexpr int RS_TILE = 128;
+constexpr double RS_MAX_PHASE3 = 2e-7;
+constexpr double RS_MAX_AMP2 = 1e-7;
+constexpr double RS_FOUR_PI = 12.566370614359172953850573533118;
+
+// um, vm: 64 pitches ((RS_TILE / 2) * pp); il: 1 / lambda.  FP32 (the
+// bounds hold with margin; what matters is that every caller rounds alike),
+// multiplied out: e^3 z il / 16 <= P3, e^2 <= A2 and 4 pi (il / (8 z^3)) n N
+// <= theta with e = (n + N) / z^2.
+__host__ __device__ inline bool rs_bound_ok(double um, double vm, double il, double dxc, double dyc, double z,
+                                            double max_theta)
+{
+#pragma clang fp contract(off)
+    const float u = (float)um, v = (float)vm, l = (float)il, zf = (float)z;
+    const float nx = u * (2.0f * (float)dxc + u), ny = v * (2.0f * (float)dyc + v);
+    const float zz = zf * zf, sn = nx + ny;
+    return sn * sn * sn * l <= (float)(16.0 * RS_MAX_PHASE3) * (zz * zz * zf) &&
+           sn * sn <= (float)RS_MAX_AMP2 * (zz * zz) &&
+           (float)(0.125 * RS_FOUR_PI) * l * nx * ny <= (float)max_theta * (zz * zf);
+}
+__host__ __device__ inline bool rs_bound_ok(const Geometry& g, double dxc, double dyc, double z, double max_theta)
+{
+    return rs_bound_ok((RS_TILE / 2) * g.pp_x, (RS_TILE / 2) * g.pp_y, g.inv_lambda, dxc, dyc, z, max_theta);
+}
+// x / y of the reference pixel of the 128 x 128 tile holding column c / row r
+// (pixel coordinates as ophGen.cpp:413,422)
+__host__ __device__ inline double rs_tile_xc(const Geometry& g, int c)
+{
+#pragma clang fp contract(off)
+    return g.x0 + (double)((c / RS_TILE) * RS_TILE + RS_TILE / 2 - 1 + g.off_x) * g.pp_x;
+}
+__host__ __device__ inline double rs_tile_yc(const Geometry& g, int r)
+{
+#pragma clang fp contract(off)
+    return g.y0 + (double)(g.pn_y - ((r / RS_TILE) * RS_TILE + RS_TILE / 2) + g.off_y) * g.pp_y;
+}
+// the bound on the tile holding pixel (c, r) for point (px, py, z)
+__host__ __device__ inline bool rs_tile_ok(const Geometry& g, int c, int r, double px, double py, double z,
+                                           double max_theta)
+{
+    return rs_bound_ok(g, fabs(rs_tile_xc(g, c) - px), fabs(rs_tile_yc(g, r) - py), z, max_theta);
+}
 
 // One job of the persistent multi-job direct kernel (serving engine).
 struct DirectJob {
@@ -145,11 +203,14 @@ int launch_rs_gemm(ohx_ctx* ctx, const Geometry& g, const PointSoA& soa, double*
 // soa into part (matrix-core points first, those whose box is the whole
 // hologram before the masked ones, each group in order; (*d_count)[0] = the
 // matrix-core points, (*d_count)[1] = the unmasked ones, device words inside
-// scratch) and the GEMM over part[p0, p1) into d_field (points from n_full
-// on masked; wgs, if given, adds the workgroups launched)
+// scratch; allow_split also puts points whose expansion bound fails on some
+// tiles in the masked group, whose instance drops those tiles for the direct
+// kernel with Geometry::rs_skip_theta) and the GEMM over part[p0, p1) into
+// d_field (points from n_full on masked; wgs, if given, adds the workgroups
+// launched)
 size_t rs_partition_scratch_bytes(int64_t n);
 int launch_rs_partition(hipStream_t st, void* scratch, size_t scratch_bytes, const Geometry& g, const PointSoA& soa,
-                        const PointSoA& part, uint32_t mode, unsigned long long** d_count);
+                        const PointSoA& part, uint32_t mode, bool allow_split, unsigned long long** d_count);
 size_t rs_gemm_scratch_bytes(const Geometry& g, int64_t npoints);
 int launch_rs_gemm_range(hipStream_t st, void* scratch, size_t scratch_bytes, const Geometry& g,
                          const PointSoA& part, int64_t p0, int64_t p1, int64_t n_full, double* d_field,

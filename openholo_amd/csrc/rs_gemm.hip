@@ -29,20 +29,22 @@
 // K entries of the Fresnel kernel's complex GEMM per point (S = sum U V^T,
 // field += -j S), 8 points per 16-entry stage.
 //
-// Which points: a point goes to the matrix cores when (a) the per-pixel range
-// test of ophGen.cpp:415-429 holds with a 1e-12 margin on every pixel of its
-// box (ophGen.cpp:384-407), so that its mask is the box, a product of a row
-// and a column interval, and (b) on every tile the box touches the neglected
-// terms are small: e^3 r_c/(16 lambda) <= 2e-7 cycles and e^2 <= 1e-7 (the
-// FP32 direct kernel's bounds), and theta_max <= 8e-3 rad for BF16x3
-// (theta_max^2/6 <= 1.1e-5 of the term, the size of the 3-term split's own
-// error), 3.5e-2 rad for BF16x1 (2e-4, a tenth of its BF16 rounding).
-// k_rs_classify bounds all three with the largest |dx_c|, |dy_c| over those
-// tiles and r_c >= z.  A stable three-way partition puts the points whose box
-// is the whole hologram first, then the other matrix-core points, which the
-// masked kernel instance takes (zero amplitude for rows and columns outside
-// the box), then the rest, which run through the FP32 direct kernels
-// (launch_direct_with) after the GEMMs.
+// Which points and tiles: a point goes to the matrix cores when the per-pixel
+// range test of ophGen.cpp:415-429 holds with a 1e-12 margin on every pixel of
+// its box (ophGen.cpp:384-407), so that its mask is the box, a product of a
+// row and a column interval.  On a tile, its expansion holds when the
+// neglected terms are small (rs_bound_ok, with r_c >= z): e^3 r_c/(16 lambda)
+// <= 2e-7 cycles and e^2 <= 1e-7 (the FP32 direct kernel's bounds), and
+// theta_max <= 8e-3 rad for BF16x3 (theta_max^2/6 <= 1.1e-5 of the term, the
+// size of the 3-term split's own error), 3.5e-2 rad for BF16x1 (2e-4, a tenth
+// of its BF16 rounding).  k_rs_classify bounds every tile of the box with the
+// largest |dx_c|, |dy_c| over them.  A stable three-way partition puts the
+// points whose box is the whole hologram and whose expansion holds on every
+// tile first (the unmasked instance), then the other matrix-core points (the
+// masked instance: zero amplitude for rows and columns outside the box and on
+// the tiles where the expansion fails), then the rest.  The FP32 direct
+// kernels then take the tiles the masked instance dropped
+// (Geometry::rs_skip_theta, the same test) and every tile of the rest.
 //
 // Operand generation, per stage and thread (point t % 8, rows and columns
 // grp + 32 i, grp = t / 8): the per-(point, tile) constants in FP64 (r_c from
@@ -72,10 +74,9 @@ constexpr int RC = BN / RG;      // U columns per thread and stage: grp + RG j
 constexpr int RU = RR + RC;      // work units (a row or a column) per thread and stage
 static_assert(RR == 4 && RC == 4 && PLANE_V == PLANE_U, "unit schedule assumes 4 rows and 4 columns");
 
-constexpr double FOUR_PI = 12.566370614359172953850573533118;
-// neglected terms on the matrix-core path (see the header comment)
-constexpr double MAX_PHASE3 = 2e-7;   // cycles
-constexpr double MAX_AMP2 = 1e-7;
+constexpr double FOUR_PI = RS_FOUR_PI;
+// neglected terms on the matrix-core path (see the header comment; the
+// phase and amplitude bounds are RS_MAX_PHASE3 and RS_MAX_AMP2)
 constexpr double MAX_THETA_X3 = 8e-3; // rad, BF16x3
 constexpr double MAX_THETA_X1 = 3.5e-2; // rad, BF16x1
 constexpr double RANGE_TOL = 1e-12;   // m, margin of the range test
@@ -120,13 +121,14 @@ struct Frame {
     double vb, dv, ub, du;    // v of row grp + RG i = vb + i dv; u of column grp + RG j = ub + j du
     double il, hl;            // 1/lambda, 1/(2 lambda)
     double um, vm;            // 64 pitches: the largest |u|, |v| on the tile
+    double theta;             // masked instance: tiles failing rs_bound_ok at this bound are dropped
     int x0, y0;               // tile origin
 };
 
 // The constants in NPR pieces of a few dependent FP64 operations each, spread
 // by the pipelined stage over its first MFMA slots.
 constexpr int NPR = 5;
-template <int K>
+template <int K, bool MASK>
 __device__ inline void prep_piece(Pt& P, Tmp& t, const Raw& r, const Frame& f)
 {
     if constexpr (K == 0) {
@@ -147,7 +149,11 @@ __device__ inline void prep_piece(Pt& P, Tmp& t, const Raw& r, const Frame& f)
         P.k = (P.h * t.irc2) * -0.25;
     } else if constexpr (K == 4) {
         P.mu = (float)(-t.irc2);
-        P.ac = r.valid ? (float)(r.kamp * t.irc2) : 0.f;
+        // masked instance: zero amplitude on tiles where the expansion bound
+        // fails (the direct kernel takes them)
+        bool keep = r.valid;
+        if constexpr (MASK) keep = keep && rs_bound_ok(f.um, f.vm, f.il, fabs(t.dxc), fabs(t.dyc), r.pz, f.theta);
+        P.ac = keep ? (float)(r.kamp * t.irc2) : 0.f;
         P.acmu = P.ac * P.mu;
         P.g = (float)(FOUR_PI * P.k);
         P.dy2 = 2.0 * t.dyc;
@@ -280,9 +286,9 @@ __device__ inline void slot(const unsigned* __restrict__ rv, const unsigned* __r
     constexpr int NM = mfma_per_stage<VAR>();
     if constexpr (S < NM) {
         mfma_slot<VAR, S>(rv, ru, c);
-        if constexpr (THREE && S < NPR) prep_piece<S>(c.P, c.t, c.R, c.f);
-        if constexpr (!THREE && 2 * S < NPR) prep_piece<2 * S>(c.P, c.t, c.R, c.f);
-        if constexpr (!THREE && 2 * S + 1 < NPR) prep_piece<2 * S + 1>(c.P, c.t, c.R, c.f);
+        if constexpr (THREE && S < NPR) prep_piece<S, MASK>(c.P, c.t, c.R, c.f);
+        if constexpr (!THREE && 2 * S < NPR) prep_piece<2 * S, MASK>(c.P, c.t, c.R, c.f);
+        if constexpr (!THREE && 2 * S + 1 < NPR) prep_piece<2 * S + 1, MASK>(c.P, c.t, c.R, c.f);
         if constexpr (S == (THREE ? NPR : (NPR + 1) / 2)) load_raw<MASK>(c.R, c.s, c.pnext, c.ke);
         if constexpr (THREE) {
             constexpr int h = half_at(S);
@@ -306,14 +312,15 @@ __device__ inline void gen_units(unsigned* wv, unsigned* wu, const Pt& P, const 
         gen_units<VAR, MASK, E + 1>(wv, wu, P, f, p, grp);
     }
 }
+template <bool MASK>
 __device__ inline void prep_all(Pt& P, const Raw& r, const Frame& f)
 {
     Tmp t;
-    prep_piece<0>(P, t, r, f);
-    prep_piece<1>(P, t, r, f);
-    prep_piece<2>(P, t, r, f);
-    prep_piece<3>(P, t, r, f);
-    prep_piece<4>(P, t, r, f);
+    prep_piece<0, MASK>(P, t, r, f);
+    prep_piece<1, MASK>(P, t, r, f);
+    prep_piece<2, MASK>(P, t, r, f);
+    prep_piece<3, MASK>(P, t, r, f);
+    prep_piece<4, MASK>(P, t, r, f);
 }
 
 // One pipeline step: MFMAs of stage st (buffer RB), generation of stage st+1
@@ -335,10 +342,12 @@ __device__ inline void step(Smem<Traits<VAR>::three>& sm, f32x16 (&acc)[4][2], R
 // S = sum_p U V^T over points kfirst .. klast-1 of s (all matrix-core
 // eligible), split-K: workgroup (tile, split) sums kslice points into plane
 // `split` of part.  MASK: the points' boxes may be smaller than the
-// hologram; out-of-box rows and columns get zero amplitude.
+// hologram, out-of-box rows and columns get zero amplitude, and so do the
+// tiles where the expansion bound at max_theta fails.
 template <int VAR, bool MASK>
 __global__ __launch_bounds__(NT, 2) void k_rs_gemm(Geometry g, PointSoA s, float2* __restrict__ part, int tiles_x,
-                                                   int tiles, int64_t kfirst, int64_t klast, int64_t kslice)
+                                                   int tiles, int64_t kfirst, int64_t klast, int64_t kslice,
+                                                   double max_theta)
 {
     constexpr bool THREE = Traits<VAR>::three;
     __shared__ __align__(16) Smem<THREE> sm;
@@ -361,6 +370,7 @@ __global__ __launch_bounds__(NT, 2) void k_rs_gemm(Geometry g, PointSoA s, float
     f.hl = 0.5 * g.inv_lambda;
     f.um = (BN / 2) * g.pp_x;
     f.vm = (BM / 2) * g.pp_y;
+    f.theta = max_theta;
     f.x0 = x0;
     f.y0 = y0;
 
@@ -372,7 +382,7 @@ __global__ __launch_bounds__(NT, 2) void k_rs_gemm(Geometry g, PointSoA s, float
     if (nst > 0) {
         load_raw<MASK>(R, s, kb + p, ke);
         Pt P;
-        prep_all(P, R, f);
+        prep_all<MASK>(P, R, f);
         gen_units<VAR, MASK, 0>(&sm.v[0][0][0][0], &sm.u[0][0][0][0], P, f, p, grp);
         load_raw<MASK>(R, s, kb + RP + p, ke);
     }
@@ -389,12 +399,15 @@ __global__ __launch_bounds__(NT, 2) void k_rs_gemm(Geometry g, PointSoA s, float
 }
 
 // Class of point i as a scan key: 1 (matrix cores, box = hologram), 1 << 32
-// (matrix cores, box smaller than the hologram: masked), 0 (direct kernel).
-// Matrix-core points meet the conditions of the header comment: the
-// reference's range test holds with margin on every pixel of the box (the
-// mask is then the box, a product of a row and a column interval), and the
-// neglected terms are small on every tile the box touches.
-__global__ __launch_bounds__(256) void k_rs_classify(Geometry g, PointSoA s, double max_theta,
+// (masked matrix-core instance), 0 (direct kernel only).  Matrix-core points
+// pass the reference's range test with margin on every pixel of the box (the
+// mask is then the box, a product of a row and a column interval); those whose
+// expansion bound (rs_bound_ok) holds on every tile the box touches and whose
+// box is the hologram take the unmasked instance.  With allow_split, a point
+// whose bound fails on some of its tiles but may hold on the nearest one takes
+// the masked instance too, which drops the failing tiles, and the direct
+// kernel takes those tiles (Geometry::rs_skip_theta).
+__global__ __launch_bounds__(256) void k_rs_classify(Geometry g, PointSoA s, double max_theta, int allow_split,
                                                      unsigned long long* __restrict__ key)
 {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -414,19 +427,17 @@ __global__ __launch_bounds__(256) void k_rs_classify(Geometry g, PointSoA s, dou
     const double zz = z * z;
     ok = ok && fabs(g.hx_coef) * sqrt(dymin * dymin + zz) - dxmax > RANGE_TOL &&
          fabs(g.hy_coef) * sqrt(dxmin * dxmin + zz) - dymax > RANGE_TOL;
-    // neglected terms on every tile the box touches: reference pixels at tile
-    // offset (64, 64), |u|, |v| <= 64 pitches, r_c >= z
-    const double um = (BN / 2) * g.pp_x, vm = (BM / 2) * g.pp_y;
-    const double xc0 = gpix_x(g, (bx0 / BN) * BN + BN / 2), xc1 = gpix_x(g, (bx1 / BN) * BN + BN / 2);
-    const double yc0 = gpix_y(g, (by0 / BM) * BM + BM / 2), yc1 = gpix_y(g, (by1 / BM) * BM + BM / 2);
+    // the expansion bound on the farthest and on the nearest tile reference
+    // pixels of the box (the bound grows with the distances)
+    const double xc0 = rs_tile_xc(g, bx0), xc1 = rs_tile_xc(g, bx1);
+    const double yc0 = rs_tile_yc(g, by0), yc1 = rs_tile_yc(g, by1);
     const double dxc = fmax(fabs(xc0 - px), fabs(xc1 - px));
     const double dyc = fmax(fabs(yc0 - py), fabs(yc1 - py));
-    const double nx = um * (2.0 * dxc + um), ny = vm * (2.0 * dyc + vm);
-    const double e = (nx + ny) / zz;
-    const double kmax = 0.125 * g.inv_lambda / (zz * z);
-    ok = ok && e * e * e * z * g.inv_lambda / 16.0 <= MAX_PHASE3 && e * e <= MAX_AMP2 &&
-         FOUR_PI * kmax * nx * ny <= max_theta;
-    key[i] = ok ? (full ? 1ull : 1ull << 32) : 0ull;
+    const bool all_tiles = rs_bound_ok(g, dxc, dyc, z, max_theta);
+    const double dxn = (px >= fmin(xc0, xc1) && px <= fmax(xc0, xc1)) ? 0.0 : fmin(fabs(xc0 - px), fabs(xc1 - px));
+    const double dyn = (py >= fmin(yc0, yc1) && py <= fmax(yc0, yc1)) ? 0.0 : fmin(fabs(yc0 - py), fabs(yc1 - py));
+    const bool some_tile = allow_split && rs_bound_ok(g, dxn, dyn, z, max_theta);
+    key[i] = ok && all_tiles && full ? 1ull : ok && (all_tiles || some_tile) ? 1ull << 32 : 0ull;
 }
 
 // Stable three-way partition: unmasked matrix-core points to o[0, n0), masked
@@ -496,8 +507,13 @@ size_t rs_partition_scratch_bytes(int64_t n)
     return 256 + 2 * al256((size_t)n * sizeof(unsigned long long)) + al256(scan_temp_bytes(n)) + 256;
 }
 
+static double max_theta_of(uint32_t mode)
+{
+    return (mode & OHX_MODE_MFMA_BF16X3) ? rsgemm::MAX_THETA_X3 : rsgemm::MAX_THETA_X1;
+}
+
 int launch_rs_partition(hipStream_t st, void* scratch, size_t scratch_bytes, const Geometry& g, const PointSoA& soa,
-                        const PointSoA& part, uint32_t mode, unsigned long long** d_count)
+                        const PointSoA& part, uint32_t mode, bool allow_split, unsigned long long** d_count)
 {
     using namespace rsgemm;
     const int64_t n = soa.n;
@@ -510,8 +526,8 @@ int launch_rs_partition(hipStream_t st, void* scratch, size_t scratch_bytes, con
     unsigned long long* pos = (unsigned long long*)(b + 256 + al256((size_t)n * sizeof(unsigned long long)));
     void* tmp = b + 256 + 2 * al256((size_t)n * sizeof(unsigned long long));
     const unsigned pgrid = (unsigned)((n + 255) / 256);
-    const double max_theta = (mode & OHX_MODE_MFMA_BF16X3) ? MAX_THETA_X3 : MAX_THETA_X1;
-    hipLaunchKernelGGL(k_rs_classify, dim3(pgrid), dim3(256), 0, st, g, soa, max_theta, key);
+    hipLaunchKernelGGL(k_rs_classify, dim3(pgrid), dim3(256), 0, st, g, soa, max_theta_of(mode), allow_split ? 1 : 0,
+                       key);
     if (hipcub::DeviceScan::ExclusiveSum(tmp, cub_bytes, (const unsigned long long*)key, pos, (int)n, st) !=
         hipSuccess)
         return OHX_ERR_DEVICE;
@@ -542,18 +558,19 @@ int rs_gemm_launch(hipStream_t st, void* scratch, size_t scratch_bytes, const Ge
     const dim3 grid((unsigned)((int64_t)pl.tiles * pl.splits)), block(NT);
     float2* pp = (float2*)scratch;
     const bool x3 = (mode & OHX_MODE_MFMA_BF16X3) != 0;
+    const double th = max_theta_of(mode);
     if (x3 && masked)
         hipLaunchKernelGGL((k_rs_gemm<V_BF16X3, true>), grid, block, 0, st, g, part, pp, pl.tiles_x, pl.tiles, p0, p1,
-                           pl.kslice);
+                           pl.kslice, th);
     else if (x3)
         hipLaunchKernelGGL((k_rs_gemm<V_BF16X3, false>), grid, block, 0, st, g, part, pp, pl.tiles_x, pl.tiles, p0,
-                           p1, pl.kslice);
+                           p1, pl.kslice, th);
     else if (masked)
         hipLaunchKernelGGL((k_rs_gemm<V_BF16X1, true>), grid, block, 0, st, g, part, pp, pl.tiles_x, pl.tiles, p0, p1,
-                           pl.kslice);
+                           pl.kslice, th);
     else
         hipLaunchKernelGGL((k_rs_gemm<V_BF16X1, false>), grid, block, 0, st, g, part, pp, pl.tiles_x, pl.tiles, p0,
-                           p1, pl.kslice);
+                           p1, pl.kslice, th);
     if (hipGetLastError() != hipSuccess) return OHX_ERR_DEVICE;
     if (wgs) *wgs += (int64_t)pl.tiles * pl.splits;
     return launch_gemm_reduce(st, pp, pl.splits, pl.npx, d_field);
@@ -595,7 +612,7 @@ int launch_rs_gemm(ohx_ctx* ctx, const Geometry& g, const PointSoA& soa, double*
     PointSoA part;
     carve_points((char*)ctx->rs_scratch + pbytes, n, &part);
     unsigned long long* d_count = nullptr;
-    int e = launch_rs_partition(st, ctx->rs_scratch, pbytes, g, soa, part, mode, &d_count);
+    int e = launch_rs_partition(st, ctx->rs_scratch, pbytes, g, soa, part, mode, true, &d_count);
     if (e) return set_error(ctx, e, "RS matrix-core partition failed");
     unsigned long long cnt[2] = {0, 0};
     e = check_hip(ctx, hipMemcpyAsync(cnt, d_count, sizeof(cnt), hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
@@ -614,6 +631,17 @@ int launch_rs_gemm(ohx_ctx* ctx, const Geometry& g, const PointSoA& soa, double*
                                      d_field, mode, &wgs);
             if (e) return set_error(ctx, e, "RS matrix-core launch failed");
         }
+    }
+    // the masked instance dropped the tiles where a point's expansion bound
+    // fails: the direct kernel takes exactly those (rs_skip_theta), then all
+    // tiles of the points outside the matrix-core classes
+    if (ne > n_full) {
+        Geometry gs = g;
+        gs.rs_skip_theta = max_theta_of(mode);
+        PointSoA masked = soa_tail(part, n_full);
+        masked.n = ne - n_full;
+        if ((e = launch_direct_with(ctx, gs, masked, d_field, false, ctx->minmax + 5, true))) return e;
+        wgs += ctx->last.workgroups;
     }
     if (ne < n) {
         if ((e = launch_direct_with(ctx, g, soa_tail(part, ne), d_field, false, ctx->minmax + 5, true))) return e;

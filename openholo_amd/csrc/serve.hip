@@ -314,7 +314,7 @@ static int setup_job(ohx_server* sv, SJob& j)
         if (!st) st = launch_prep_on(sv->s_setup, j.d_vtx, j.g, j.soa);
         if (!st && j.rsm) {
             unsigned long long* dc = nullptr;
-            st = launch_rs_partition(sv->s_setup, j.rs_tmp, j.rs_tmp_bytes, j.g, j.soa, j.part, j.mode, &dc);
+            st = launch_rs_partition(sv->s_setup, j.rs_tmp, j.rs_tmp_bytes, j.g, j.soa, j.part, j.mode, false, &dc);
             if (!st && hipMemcpyAsync(rs_count(j), dc, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                                       sv->s_setup) != hipSuccess)
                 st = OHX_ERR_DEVICE;

@@ -67,7 +67,8 @@ int geometry_from_params(ohx_ctx* ctx, const ohx_pc_params* p, uint32_t diff, Ge
     g->channel = p->channel;
     g->use_phase = p->use_phase ? 1 : 0;
     g->diff = (int32_t)diff;
-    g->rs_skip_theta = 0.0;
+    g->rs_skip_mask = nullptr;
+    g->rs_mask_words = 0;
     return OHX_OK;
 }
 

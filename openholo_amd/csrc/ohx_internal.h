@@ -82,10 +82,13 @@ struct Geometry {
     double distance;
     int32_t channel, use_phase;
     int32_t diff;
-    // RS direct kernels only: > 0 skips the (point, tile) pairs whose 128 x 128
-    // matrix-core tile passes rs_bound_ok at this theta bound (the matrix-core
-    // kernel took them); 0 (the default) skips nothing
-    double rs_skip_theta;
+    // RS direct kernels only: non-null skips the (point, tile) pairs whose bit
+    // is set in this mask (the masked matrix-core instance took them): row p
+    // (the launch's point index) of rs_mask_words words holds bit t for the
+    // 128 x 128 tile t (row-major over the hologram, rs_tile_index); null
+    // (the default) skips nothing
+    const unsigned* rs_skip_mask;
+    int32_t rs_mask_words;
 };
 
 // ---- RS on the matrix cores: the expansion bound on one tile ----------------
@@ -96,18 +99,17 @@ struct Geometry {
 // third-order phase e^3 z/(16 lambda) <= RS_MAX_PHASE3 cycles, second-order
 // amplitude e^2 <= RS_MAX_AMP2 and cross term 4 pi k n N <= max_theta rad.
 // The classification bounds every tile of a box with the largest dxc, dyc;
-// the masked matrix-core kernel drops the tiles where the bound fails and the
-// direct kernel (Geometry::rs_skip_theta) takes exactly those: both evaluate
-// rs_tile_ok in FP64 without contraction, so they agree bit for bit.
+// per tile, a mask (k_rs_tilemask) records where the bound and the range test
+// hold: the masked matrix-core kernel takes those tiles and the direct kernel
+// (Geometry::rs_skip_mask) the others.
 constexpr int RS_TILE = 128;
 constexpr double RS_MAX_PHASE3 = 2e-7;
 constexpr double RS_MAX_AMP2 = 1e-7;
 constexpr double RS_FOUR_PI = 12.566370614359172953850573533118;
 
-// um, vm: 64 pitches ((RS_TILE / 2) * pp); il: 1 / lambda.  FP32 (the
-// bounds hold with margin; what matters is that every caller rounds alike),
-// multiplied out: e^3 z il / 16 <= P3, e^2 <= A2 and 4 pi (il / (8 z^3)) n N
-// <= theta with e = (n + N) / z^2.
+// um, vm: 64 pitches ((RS_TILE / 2) * pp); il: 1 / lambda.  In FP32 (the
+// limits are far from the rounding), multiplied out: e^3 z il / 16 <= P3,
+// e^2 <= A2 and 4 pi (il / (8 z^3)) n N <= theta with e = (n + N) / z^2.
 __host__ __device__ inline bool rs_bound_ok(double um, double vm, double il, double dxc, double dyc, double z,
                                             double max_theta)
 {
@@ -135,11 +137,21 @@ __host__ __device__ inline double rs_tile_yc(const Geometry& g, int r)
 #pragma clang fp contract(off)
     return g.y0 + (double)(g.pn_y - ((r / RS_TILE) * RS_TILE + RS_TILE / 2) + g.off_y) * g.pp_y;
 }
-// the bound on the tile holding pixel (c, r) for point (px, py, z)
-__host__ __device__ inline bool rs_tile_ok(const Geometry& g, int c, int r, double px, double py, double z,
-                                           double max_theta)
+// 128 x 128 tiles along x, tile index of the tile holding pixel (c, r), and
+// 32-bit words of one mask row
+__host__ __device__ inline int rs_tiles_x(const Geometry& g) { return (g.pn_x + RS_TILE - 1) / RS_TILE; }
+__host__ __device__ inline int rs_tile_index(const Geometry& g, int c, int r)
 {
-    return rs_bound_ok(g, fabs(rs_tile_xc(g, c) - px), fabs(rs_tile_yc(g, r) - py), z, max_theta);
+    return (r / RS_TILE) * rs_tiles_x(g) + c / RS_TILE;
+}
+__host__ __device__ inline int rs_mask_words(const Geometry& g)
+{
+    return (rs_tiles_x(g) * ((g.pn_y + RS_TILE - 1) / RS_TILE) + 31) / 32;
+}
+// bit t of mask row `row`
+__device__ inline bool rs_mask_bit(const unsigned* __restrict__ row, int t)
+{
+    return (row[t >> 5] >> (t & 31)) & 1u;
 }
 
 // One job of the persistent multi-job direct kernel (serving engine).
@@ -203,18 +215,20 @@ int launch_rs_gemm(ohx_ctx* ctx, const Geometry& g, const PointSoA& soa, double*
 // soa into part (matrix-core points first, those whose box is the whole
 // hologram before the masked ones, each group in order; (*d_count)[0] = the
 // matrix-core points, (*d_count)[1] = the unmasked ones, device words inside
-// scratch; allow_split also puts points whose expansion bound fails on some
-// tiles in the masked group, whose instance drops those tiles for the direct
-// kernel with Geometry::rs_skip_theta) and the GEMM over part[p0, p1) into
-// d_field (points from n_full on masked; wgs, if given, adds the workgroups
-// launched)
+// scratch; split puts every point whose expansion may hold on a tile in the
+// masked group, whose instance then takes only the tiles set in its tile mask
+// (k_rs_tilemask), the direct kernel the others with Geometry::rs_skip_mask)
+// and the GEMM over part[p0, p1) into d_field (points from n_full on masked,
+// with mask rows from point n_full on if mask is given; wgs, if given, adds
+// the workgroups launched)
 size_t rs_partition_scratch_bytes(int64_t n);
 int launch_rs_partition(hipStream_t st, void* scratch, size_t scratch_bytes, const Geometry& g, const PointSoA& soa,
-                        const PointSoA& part, uint32_t mode, bool allow_split, unsigned long long** d_count);
+                        const PointSoA& part, uint32_t mode, bool split, unsigned long long** d_count);
 size_t rs_gemm_scratch_bytes(const Geometry& g, int64_t npoints);
 int launch_rs_gemm_range(hipStream_t st, void* scratch, size_t scratch_bytes, const Geometry& g,
                          const PointSoA& part, int64_t p0, int64_t p1, int64_t n_full, double* d_field,
-                         uint32_t mode, int64_t* wgs = nullptr);
+                         uint32_t mode, int64_t* wgs = nullptr, const unsigned* mask = nullptr,
+                         int mask_words = 0);
 int launch_probe(ohx_ctx* ctx, const Geometry& g, const PointSoA& soa, const int64_t* d_pix,
                  int64_t n_pix, double* d_out);
 

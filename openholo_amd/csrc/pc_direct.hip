@@ -297,21 +297,10 @@ __device__ __attribute__((always_inline)) inline void direct_tile(DirectSmem<DIF
         double px = 0, py = 0, z = 0, cyc0 = 0, kamp = 0, q = 0;
         const int64_t p = base + tid;
         // RS: the tiles that the masked matrix-core instance took for this
-        // point (rs_tile_ok, the test that instance applies) are skipped;
-        // tested ahead of the classification, whose registers it would
-        // otherwise add to
+        // point (its bit in Geometry::rs_skip_mask) are skipped
         bool skip = false;
-        if (DIFF == OHX_DIFF_RS && g.rs_skip_theta > 0.0 && tid < PCH && p < p_end) {
-            // the test's tile constants are recomputed per chunk from laundered
-            // kernel arguments: hoisted out of the loop, they would hold eight
-            // VGPRs through the pair loop
-            int lx0 = x0, ly0 = y0;
-            double ppx = g.pp_x, ppy = g.pp_y, il = g.inv_lambda, th = g.rs_skip_theta;
-            asm volatile("" : "+s"(lx0), "+s"(ly0), "+s"(ppx), "+s"(ppy), "+s"(il), "+s"(th));
-            skip = rs_bound_ok((RS_TILE / 2) * ppx, (RS_TILE / 2) * ppy, il, fabs(rs_tile_xc(g, lx0) - s.px[p]),
-                               fabs(rs_tile_yc(g, ly0) - s.py[p]), s.pz[p], th);
-        }
-        __builtin_amdgcn_sched_barrier(0);
+        if (DIFF == OHX_DIFF_RS && g.rs_skip_mask && tid < PCH && p < p_end)
+            skip = rs_mask_bit(g.rs_skip_mask + p * g.rs_mask_words, rs_tile_index(g, x0, y0));
         if (tid < PCH && p < p_end && !skip) {
             b = s.box[p];
             if (b.x < cx1 && b.y > x0 && b.z < ry1 && b.w > y0) {

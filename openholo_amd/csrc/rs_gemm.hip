@@ -29,22 +29,23 @@
 // K entries of the Fresnel kernel's complex GEMM per point (S = sum U V^T,
 // field += -j S), 8 points per 16-entry stage.
 //
-// Which points and tiles: a point goes to the matrix cores when the per-pixel
-// range test of ophGen.cpp:415-429 holds with a 1e-12 margin on every pixel of
-// its box (ophGen.cpp:384-407), so that its mask is the box, a product of a
-// row and a column interval.  On a tile, its expansion holds when the
+// Which points and tiles: on a tile, a point's expansion holds when the
 // neglected terms are small (rs_bound_ok, with r_c >= z): e^3 r_c/(16 lambda)
 // <= 2e-7 cycles and e^2 <= 1e-7 (the FP32 direct kernel's bounds), and
 // theta_max <= 8e-3 rad for BF16x3 (theta_max^2/6 <= 1.1e-5 of the term, the
 // size of the 3-term split's own error), 3.5e-2 rad for BF16x1 (2e-4, a tenth
-// of its BF16 rounding).  k_rs_classify bounds every tile of the box with the
-// largest |dx_c|, |dy_c| over them.  A stable three-way partition puts the
-// points whose box is the whole hologram and whose expansion holds on every
-// tile first (the unmasked instance), then the other matrix-core points (the
-// masked instance: zero amplitude for rows and columns outside the box and on
-// the tiles where the expansion fails), then the rest.  The FP32 direct
-// kernels then take the tiles the masked instance dropped
-// (Geometry::rs_skip_theta, the same test) and every tile of the rest.
+// of its BF16 rounding); the matrix cores may take the tile when, in
+// addition, the per-pixel range test of ophGen.cpp:415-429 holds with a
+// 1e-12 margin on every pixel of the tile inside the box (ophGen.cpp:384-407),
+// so that the mask there is the box, a product of a row and a column
+// interval.  A stable three-way partition (k_rs_classify) puts first the
+// points whose box is the hologram, on all of which the range test holds and
+// on every tile the expansion (the unmasked instance), then the points whose
+// expansion may hold on their nearest tile (the masked instance: zero
+// amplitude for rows and columns outside the box and on the tiles that
+// k_rs_tilemask leaves out), then the rest.  The FP32 direct kernels then
+// take the masked points' other tiles (Geometry::rs_skip_mask) and every tile
+// of the rest.
 //
 // Operand generation, per stage and thread (point t % 8, rows and columns
 // grp + 32 i, grp = t / 8): the per-(point, tile) constants in FP64 (r_c from
@@ -81,19 +82,31 @@ constexpr double MAX_THETA_X3 = 8e-3; // rad, BF16x3
 constexpr double MAX_THETA_X1 = 3.5e-2; // rad, BF16x1
 constexpr double RANGE_TOL = 1e-12;   // m, margin of the range test
 
+// Per-tile mask of the masked instance (k_rs_tilemask): point j's row starts
+// at bits + (j - base) * words; the workgroup's tile is bit `tile`.  Null
+// bits: every tile of every point.
+struct TileMask {
+    const unsigned* bits;
+    int64_t base;
+    int words, tile;
+};
+
 // Point data of one stage: thread t holds point t % RP.  Lanes past the slice
 // end read the slice's last point and are marked invalid (zero amplitude).
-// MASK: the points' boxes may be smaller than the hologram (loaded too).
+// MASK: the points' boxes may be smaller than the hologram (loaded too), and
+// the point's word of the tile mask (all ones without a mask).
 struct Raw {
     double px, py, pz, cyc0, kamp;
     int4 box;
+    unsigned mword;
     bool valid;
 };
 template <bool MASK>
-__device__ inline void load_raw(Raw& r, const PointSoA& s, int64_t pi, int64_t p1)
+__device__ inline void load_raw(Raw& r, const PointSoA& s, const TileMask& tm, int64_t pi, int64_t p1)
 {
     r.valid = pi < p1;
     const int64_t j = r.valid ? pi : p1 - 1;
+    if constexpr (MASK) r.mword = tm.bits ? tm.bits[(j - tm.base) * tm.words + (tm.tile >> 5)] : ~0u;
     r.px = s.px[j];
     r.py = s.py[j];
     r.pz = s.pz[j];
@@ -121,7 +134,7 @@ struct Frame {
     double vb, dv, ub, du;    // v of row grp + RG i = vb + i dv; u of column grp + RG j = ub + j du
     double il, hl;            // 1/lambda, 1/(2 lambda)
     double um, vm;            // 64 pitches: the largest |u|, |v| on the tile
-    double theta;             // masked instance: tiles failing rs_bound_ok at this bound are dropped
+    TileMask tm;              // masked instance: the tiles it takes per point
     int x0, y0;               // tile origin
 };
 
@@ -149,10 +162,10 @@ __device__ inline void prep_piece(Pt& P, Tmp& t, const Raw& r, const Frame& f)
         P.k = (P.h * t.irc2) * -0.25;
     } else if constexpr (K == 4) {
         P.mu = (float)(-t.irc2);
-        // masked instance: zero amplitude on tiles where the expansion bound
-        // fails (the direct kernel takes them)
+        // masked instance: zero amplitude on the tiles the mask leaves to the
+        // direct kernel
         bool keep = r.valid;
-        if constexpr (MASK) keep = keep && rs_bound_ok(f.um, f.vm, f.il, fabs(t.dxc), fabs(t.dyc), r.pz, f.theta);
+        if constexpr (MASK) keep = keep && ((r.mword >> (f.tm.tile & 31)) & 1u);
         P.ac = keep ? (float)(r.kamp * t.irc2) : 0.f;
         P.acmu = P.ac * P.mu;
         P.g = (float)(FOUR_PI * P.k);
@@ -289,7 +302,7 @@ __device__ inline void slot(const unsigned* __restrict__ rv, const unsigned* __r
         if constexpr (THREE && S < NPR) prep_piece<S, MASK>(c.P, c.t, c.R, c.f);
         if constexpr (!THREE && 2 * S < NPR) prep_piece<2 * S, MASK>(c.P, c.t, c.R, c.f);
         if constexpr (!THREE && 2 * S + 1 < NPR) prep_piece<2 * S + 1, MASK>(c.P, c.t, c.R, c.f);
-        if constexpr (S == (THREE ? NPR : (NPR + 1) / 2)) load_raw<MASK>(c.R, c.s, c.pnext, c.ke);
+        if constexpr (S == (THREE ? NPR : (NPR + 1) / 2)) load_raw<MASK>(c.R, c.s, c.f.tm, c.pnext, c.ke);
         if constexpr (THREE) {
             constexpr int h = half_at(S);
             if constexpr (h >= 0 && (h & 1) == 0) c.cs[h >> 1] = unit_value<(h >> 1), MASK>(c.P, c.f, c.grp);
@@ -343,11 +356,13 @@ __device__ inline void step(Smem<Traits<VAR>::three>& sm, f32x16 (&acc)[4][2], R
 // eligible), split-K: workgroup (tile, split) sums kslice points into plane
 // `split` of part.  MASK: the points' boxes may be smaller than the
 // hologram, out-of-box rows and columns get zero amplitude, and so do the
-// tiles where the expansion bound at max_theta fails.
+// tiles whose bit is clear in the point's row of mask (rows from point
+// mask_base on, mask_words words each; null mask: none).
 template <int VAR, bool MASK>
 __global__ __launch_bounds__(NT, 2) void k_rs_gemm(Geometry g, PointSoA s, float2* __restrict__ part, int tiles_x,
                                                    int tiles, int64_t kfirst, int64_t klast, int64_t kslice,
-                                                   double max_theta)
+                                                   const unsigned* __restrict__ mask, int64_t mask_base,
+                                                   int mask_words)
 {
     constexpr bool THREE = Traits<VAR>::three;
     __shared__ __align__(16) Smem<THREE> sm;
@@ -370,7 +385,7 @@ __global__ __launch_bounds__(NT, 2) void k_rs_gemm(Geometry g, PointSoA s, float
     f.hl = 0.5 * g.inv_lambda;
     f.um = (BN / 2) * g.pp_x;
     f.vm = (BM / 2) * g.pp_y;
-    f.theta = max_theta;
+    f.tm = TileMask{mask, mask_base, mask_words, rs_tile_index(g, x0, y0)};
     f.x0 = x0;
     f.y0 = y0;
 
@@ -380,11 +395,11 @@ __global__ __launch_bounds__(NT, 2) void k_rs_gemm(Geometry g, PointSoA s, float
 
     Raw R;
     if (nst > 0) {
-        load_raw<MASK>(R, s, kb + p, ke);
+        load_raw<MASK>(R, s, f.tm, kb + p, ke);
         Pt P;
         prep_all<MASK>(P, R, f);
         gen_units<VAR, MASK, 0>(&sm.v[0][0][0][0], &sm.u[0][0][0][0], P, f, p, grp);
-        load_raw<MASK>(R, s, kb + RP + p, ke);
+        load_raw<MASK>(R, s, f.tm, kb + RP + p, ke);
     }
     __syncthreads();
     f32x16 acc[4][2];
@@ -398,35 +413,43 @@ __global__ __launch_bounds__(NT, 2) void k_rs_gemm(Geometry g, PointSoA s, float
     store_partials(g, acc, part, split, x0, y0, ucol, lane);
 }
 
-// Class of point i as a scan key: 1 (matrix cores, box = hologram), 1 << 32
-// (masked matrix-core instance), 0 (direct kernel only).  Matrix-core points
-// pass the reference's range test with margin on every pixel of the box (the
-// mask is then the box, a product of a row and a column interval); those whose
-// expansion bound (rs_bound_ok) holds on every tile the box touches and whose
-// box is the hologram take the unmasked instance.  With allow_split, a point
-// whose bound fails on some of its tiles but may hold on the nearest one takes
-// the masked instance too, which drops the failing tiles, and the direct
-// kernel takes those tiles (Geometry::rs_skip_theta).
-__global__ __launch_bounds__(256) void k_rs_classify(Geometry g, PointSoA s, double max_theta, int allow_split,
+// The reference's range test (ophGen.cpp:415-429) with a RANGE_TOL margin on
+// every pixel of columns [c0, c1) x rows [r0, r1): bounded with the extreme
+// distances of the rectangle from the point.
+__device__ inline bool range_ok_rect(const Geometry& g, double px, double py, double z, int c0, int c1, int r0,
+                                     int r1)
+{
+    const double xl = gpix_x(g, c0), xh = gpix_x(g, c1 - 1);
+    const double yh = gpix_y(g, r0), yl = gpix_y(g, r1 - 1);
+    const double dxmax = fmax(fabs(xl - px), fabs(xh - px));
+    const double dymax = fmax(fabs(yl - py), fabs(yh - py));
+    const double dxmin = (px >= xl && px <= xh) ? 0.0 : fmin(fabs(xl - px), fabs(xh - px));
+    const double dymin = (py >= yl && py <= yh) ? 0.0 : fmin(fabs(yl - py), fabs(yh - py));
+    const double zz = z * z;
+    return fabs(g.hx_coef) * sqrt(dymin * dymin + zz) - dxmax > RANGE_TOL &&
+           fabs(g.hy_coef) * sqrt(dxmin * dxmin + zz) - dymax > RANGE_TOL;
+}
+
+// Class of point i as a scan key: 1 (unmasked matrix-core instance), 1 << 32
+// (masked instance), 0 (direct kernel only).  The unmasked instance takes the
+// points whose box is the hologram, on which the range test holds with margin
+// (the mask is then the box) and whose expansion bound (rs_bound_ok) holds on
+// every tile.  Without split, the masked instance takes the other points whose
+// range test and bound hold on the whole box (out-of-box rows and columns get
+// zero amplitude).  With split, it takes every point whose bound may hold on
+// its nearest tile, on the tiles where k_rs_tilemask finds the range test and
+// the bound both hold; the direct kernel takes the other tiles.
+__global__ __launch_bounds__(256) void k_rs_classify(Geometry g, PointSoA s, double max_theta, int split,
                                                      unsigned long long* __restrict__ key)
 {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= s.n) return;
     const double px = s.px[i], py = s.py[i], z = s.pz[i];
     const int4 b = s.box[i];
-    bool ok = z > 0.0 && b.x < b.y && b.z < b.w;
+    const bool some = z > 0.0 && b.x < b.y && b.z < b.w;
     const bool full = b.x == 0 && b.y == g.pn_x && b.z == 0 && b.w == g.pn_y;
-    const int bx1 = ok ? b.y - 1 : 0, by1 = ok ? b.w - 1 : 0, bx0 = ok ? b.x : 0, by0 = ok ? b.z : 0;
-    // the reference's range test on every pixel of the box, with margin
-    const double xl = gpix_x(g, bx0), xh = gpix_x(g, bx1);
-    const double yh = gpix_y(g, by0), yl = gpix_y(g, by1);
-    const double dxmax = fmax(fabs(xl - px), fabs(xh - px));
-    const double dymax = fmax(fabs(yl - py), fabs(yh - py));
-    const double dxmin = (px >= xl && px <= xh) ? 0.0 : fmin(fabs(xl - px), fabs(xh - px));
-    const double dymin = (py >= yl && py <= yh) ? 0.0 : fmin(fabs(yl - py), fabs(yh - py));
-    const double zz = z * z;
-    ok = ok && fabs(g.hx_coef) * sqrt(dymin * dymin + zz) - dxmax > RANGE_TOL &&
-         fabs(g.hy_coef) * sqrt(dxmin * dxmin + zz) - dymax > RANGE_TOL;
+    const int bx1 = some ? b.y - 1 : 0, by1 = some ? b.w - 1 : 0, bx0 = some ? b.x : 0, by0 = some ? b.z : 0;
+    const bool range = some && range_ok_rect(g, px, py, z, bx0, bx1 + 1, by0, by1 + 1);
     // the expansion bound on the farthest and on the nearest tile reference
     // pixels of the box (the bound grows with the distances)
     const double xc0 = rs_tile_xc(g, bx0), xc1 = rs_tile_xc(g, bx1);
@@ -436,8 +459,39 @@ __global__ __launch_bounds__(256) void k_rs_classify(Geometry g, PointSoA s, dou
     const bool all_tiles = rs_bound_ok(g, dxc, dyc, z, max_theta);
     const double dxn = (px >= fmin(xc0, xc1) && px <= fmax(xc0, xc1)) ? 0.0 : fmin(fabs(xc0 - px), fabs(xc1 - px));
     const double dyn = (py >= fmin(yc0, yc1) && py <= fmax(yc0, yc1)) ? 0.0 : fmin(fabs(yc0 - py), fabs(yc1 - py));
-    const bool some_tile = allow_split && rs_bound_ok(g, dxn, dyn, z, max_theta);
-    key[i] = ok && all_tiles && full ? 1ull : ok && (all_tiles || some_tile) ? 1ull << 32 : 0ull;
+    const bool near_tile = rs_bound_ok(g, dxn, dyn, z, max_theta);
+    key[i] = range && all_tiles && full ? 1ull
+             : (split ? some && near_tile : range && all_tiles) ? 1ull << 32
+                                                                : 0ull;
+}
+
+// Tile mask of the masked instance's points s[0, n): word w of row i holds
+// bits for tiles 32 w .. 32 w + 31 (rs_tile_index); a bit is set when the
+// tile meets the point's box and, on the part inside the box, the range test
+// holds with margin (the mask there is the box) and the expansion bound holds.
+__global__ __launch_bounds__(256) void k_rs_tilemask(Geometry g, PointSoA s, double max_theta, int words,
+                                                     unsigned* __restrict__ mask)
+{
+    const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= s.n * words) return;
+    const int64_t i = idx / words;
+    const int w = (int)(idx - i * words);
+    const double px = s.px[i], py = s.py[i], z = s.pz[i];
+    const int4 b = s.box[i];
+    const int tx_n = rs_tiles_x(g), tiles = tx_n * ((g.pn_y + RS_TILE - 1) / RS_TILE);
+    unsigned bits = 0u;
+    for (int k = 0; k < 32; ++k) {
+        const int t = w * 32 + k;
+        if (t >= tiles) break;
+        const int c = (t % tx_n) * RS_TILE, r = (t / tx_n) * RS_TILE;
+        const int c0 = max(c, b.x), c1 = min(min(c + RS_TILE, g.pn_x), b.y);
+        const int r0 = max(r, b.z), r1 = min(min(r + RS_TILE, g.pn_y), b.w);
+        if (c0 >= c1 || r0 >= r1 || !(z > 0.0)) continue;
+        if (rs_bound_ok(g, fabs(rs_tile_xc(g, c) - px), fabs(rs_tile_yc(g, r) - py), z, max_theta) &&
+            range_ok_rect(g, px, py, z, c0, c1, r0, r1))
+            bits |= 1u << k;
+    }
+    mask[idx] = bits;
 }
 
 // Stable three-way partition: unmasked matrix-core points to o[0, n0), masked
@@ -497,6 +551,7 @@ size_t scan_temp_bytes(int64_t n)
     return b;
 }
 constexpr size_t MAX_RS_PART = size_t(4) << 30;
+constexpr size_t MAX_RS_MASK = size_t(1) << 30;
 }  // namespace
 
 // [count words, 256 B] [key] [pos] [scan temp]; beyond INT_MAX points (the
@@ -513,7 +568,7 @@ static double max_theta_of(uint32_t mode)
 }
 
 int launch_rs_partition(hipStream_t st, void* scratch, size_t scratch_bytes, const Geometry& g, const PointSoA& soa,
-                        const PointSoA& part, uint32_t mode, bool allow_split, unsigned long long** d_count)
+                        const PointSoA& part, uint32_t mode, bool split, unsigned long long** d_count)
 {
     using namespace rsgemm;
     const int64_t n = soa.n;
@@ -526,8 +581,7 @@ int launch_rs_partition(hipStream_t st, void* scratch, size_t scratch_bytes, con
     unsigned long long* pos = (unsigned long long*)(b + 256 + al256((size_t)n * sizeof(unsigned long long)));
     void* tmp = b + 256 + 2 * al256((size_t)n * sizeof(unsigned long long));
     const unsigned pgrid = (unsigned)((n + 255) / 256);
-    hipLaunchKernelGGL(k_rs_classify, dim3(pgrid), dim3(256), 0, st, g, soa, max_theta_of(mode), allow_split ? 1 : 0,
-                       key);
+    hipLaunchKernelGGL(k_rs_classify, dim3(pgrid), dim3(256), 0, st, g, soa, max_theta_of(mode), split ? 1 : 0, key);
     if (hipcub::DeviceScan::ExclusiveSum(tmp, cub_bytes, (const unsigned long long*)key, pos, (int)n, st) !=
         hipSuccess)
         return OHX_ERR_DEVICE;
@@ -549,7 +603,8 @@ size_t rs_gemm_scratch_bytes(const Geometry& g, int64_t npoints)
 
 namespace {
 int rs_gemm_launch(hipStream_t st, void* scratch, size_t scratch_bytes, const Geometry& g, const PointSoA& part,
-                   int64_t p0, int64_t p1, bool masked, double* d_field, uint32_t mode, int64_t* wgs)
+                   int64_t p0, int64_t p1, bool masked, double* d_field, uint32_t mode, int64_t* wgs,
+                   const unsigned* mask, int64_t mask_base, int mask_words)
 {
     using namespace rsgemm;
     if (p1 <= p0) return OHX_OK;
@@ -558,19 +613,19 @@ int rs_gemm_launch(hipStream_t st, void* scratch, size_t scratch_bytes, const Ge
     const dim3 grid((unsigned)((int64_t)pl.tiles * pl.splits)), block(NT);
     float2* pp = (float2*)scratch;
     const bool x3 = (mode & OHX_MODE_MFMA_BF16X3) != 0;
-    const double th = max_theta_of(mode);
+    const unsigned* mk = masked ? mask : nullptr;
     if (x3 && masked)
         hipLaunchKernelGGL((k_rs_gemm<V_BF16X3, true>), grid, block, 0, st, g, part, pp, pl.tiles_x, pl.tiles, p0, p1,
-                           pl.kslice, th);
+                           pl.kslice, mk, mask_base, mask_words);
     else if (x3)
         hipLaunchKernelGGL((k_rs_gemm<V_BF16X3, false>), grid, block, 0, st, g, part, pp, pl.tiles_x, pl.tiles, p0,
-                           p1, pl.kslice, th);
+                           p1, pl.kslice, mk, mask_base, mask_words);
     else if (masked)
         hipLaunchKernelGGL((k_rs_gemm<V_BF16X1, true>), grid, block, 0, st, g, part, pp, pl.tiles_x, pl.tiles, p0, p1,
-                           pl.kslice, th);
+                           pl.kslice, mk, mask_base, mask_words);
     else
         hipLaunchKernelGGL((k_rs_gemm<V_BF16X1, false>), grid, block, 0, st, g, part, pp, pl.tiles_x, pl.tiles, p0,
-                           p1, pl.kslice, th);
+                           p1, pl.kslice, mk, mask_base, mask_words);
     if (hipGetLastError() != hipSuccess) return OHX_ERR_DEVICE;
     if (wgs) *wgs += (int64_t)pl.tiles * pl.splits;
     return launch_gemm_reduce(st, pp, pl.splits, pl.npx, d_field);
@@ -579,11 +634,13 @@ int rs_gemm_launch(hipStream_t st, void* scratch, size_t scratch_bytes, const Ge
 
 int launch_rs_gemm_range(hipStream_t st, void* scratch, size_t scratch_bytes, const Geometry& g,
                          const PointSoA& part, int64_t p0, int64_t p1, int64_t n_full, double* d_field, uint32_t mode,
-                         int64_t* wgs)
+                         int64_t* wgs, const unsigned* mask, int mask_words)
 {
     const int64_t m = std::min(std::max(n_full, p0), p1);
-    int e = rs_gemm_launch(st, scratch, scratch_bytes, g, part, p0, m, false, d_field, mode, wgs);
-    if (!e) e = rs_gemm_launch(st, scratch, scratch_bytes, g, part, m, p1, true, d_field, mode, wgs);
+    int e = rs_gemm_launch(st, scratch, scratch_bytes, g, part, p0, m, false, d_field, mode, wgs, nullptr, 0, 0);
+    if (!e)
+        e = rs_gemm_launch(st, scratch, scratch_bytes, g, part, m, p1, true, d_field, mode, wgs, mask, n_full,
+                           mask_words);
     return e;
 }
 
@@ -611,8 +668,12 @@ int launch_rs_gemm(ohx_ctx* ctx, const Geometry& g, const PointSoA& soa, double*
     }
     PointSoA part;
     carve_points((char*)ctx->rs_scratch + pbytes, n, &part);
+    // per-tile split between the masked instance and the direct kernel, unless
+    // its mask (one row per masked point) could exceed MAX_RS_MASK
+    const int words = rs_mask_words(g);
+    const bool split = (size_t)n * words * sizeof(unsigned) <= MAX_RS_MASK;
     unsigned long long* d_count = nullptr;
-    int e = launch_rs_partition(st, ctx->rs_scratch, pbytes, g, soa, part, mode, true, &d_count);
+    int e = launch_rs_partition(st, ctx->rs_scratch, pbytes, g, soa, part, mode, split, &d_count);
     if (e) return set_error(ctx, e, "RS matrix-core partition failed");
     unsigned long long cnt[2] = {0, 0};
     e = check_hip(ctx, hipMemcpyAsync(cnt, d_count, sizeof(cnt), hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
@@ -621,25 +682,39 @@ int launch_rs_gemm(ohx_ctx* ctx, const Geometry& g, const PointSoA& soa, double*
     const int64_t ne = (int64_t)cnt[0], n_full = (int64_t)cnt[1];
 
     int64_t wgs = 0;
+    const int64_t nm = split ? ne - n_full : 0;   // points of the masked instance with a tile mask
+    unsigned* mask = nullptr;
     if (ne > 0) {
         int64_t chunk = ne;
         while (chunk > RP && rs_gemm_scratch_bytes(g, chunk) > MAX_RS_PART) chunk = (chunk + 1) / 2;
-        if ((e = ensure_gemm_scratch(ctx, rs_gemm_scratch_bytes(g, chunk)))) return e;
+        // [split-K planes] [tile mask]
+        const size_t planes = al256(rs_gemm_scratch_bytes(g, chunk));
+        if ((e = ensure_gemm_scratch(ctx, planes + (size_t)nm * words * sizeof(unsigned)))) return e;
+        if (nm > 0) {
+            mask = (unsigned*)((char*)ctx->gemm_scratch + planes);
+            PointSoA masked = soa_tail(part, n_full);
+            masked.n = nm;
+            const int64_t items = nm * words;
+            hipLaunchKernelGGL(k_rs_tilemask, dim3((unsigned)((items + 255) / 256)), dim3(256), 0, st, g, masked,
+                               max_theta_of(mode), words, mask);
+            if ((e = check_hip(ctx, hipGetLastError(), "k_rs_tilemask"))) return e;
+        }
         for (int64_t p0 = 0; p0 < ne; p0 += chunk) {
             const int64_t p1 = std::min<int64_t>(ne, p0 + chunk);
-            e = launch_rs_gemm_range(st, ctx->gemm_scratch, ctx->gemm_scratch_bytes, g, part, p0, p1, n_full,
-                                     d_field, mode, &wgs);
+            e = launch_rs_gemm_range(st, ctx->gemm_scratch, planes, g, part, p0, p1, n_full, d_field, mode, &wgs,
+                                     mask, words);
             if (e) return set_error(ctx, e, "RS matrix-core launch failed");
         }
     }
-    // the masked instance dropped the tiles where a point's expansion bound
-    // fails: the direct kernel takes exactly those (rs_skip_theta), then all
-    // tiles of the points outside the matrix-core classes
-    if (ne > n_full) {
+    // the masked instance took the tiles set in the mask: the direct kernel
+    // takes its points' other tiles (rs_skip_mask), then every tile of the
+    // points outside the matrix-core classes
+    if (nm > 0) {
         Geometry gs = g;
-        gs.rs_skip_theta = max_theta_of(mode);
+        gs.rs_skip_mask = mask;
+        gs.rs_mask_words = words;
         PointSoA masked = soa_tail(part, n_full);
-        masked.n = ne - n_full;
+        masked.n = nm;
         if ((e = launch_direct_with(ctx, gs, masked, d_field, false, ctx->minmax + 5, true))) return e;
         wgs += ctx->last.workgroups;
     }

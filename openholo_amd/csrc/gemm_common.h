@@ -211,26 +211,52 @@ struct GemmPlan {
     int tiles_x, tiles;
     int64_t splits, kslice, npx;
 };
+// Split K so that (a) the workgroups fill the device's resident slots in whole
+// rounds (256 CUs x one 8-wave workgroup of a 256-column tile, or two 4-wave
+// ones of a 128-column tile): a launch's last round otherwise runs part-empty,
+// and (b) no FP32 accumulator sums more than GEMM_MAX_SLICE points: FP32
+// rounding of the running sums grows like sqrt(points per slice).  Among the
+// split counts from one round of workgroups to about four, the plan takes the
+// one with the least time, counted as rounds x points per split plus
+// GEMM_SPLIT_COST points for each split's partial plane (its write and its
+// reduction), the smaller count on ties; every split keeps >= 4 stages.
+constexpr int64_t GEMM_MAX_SLICE = 65536;
+constexpr int64_t GEMM_SPLIT_COST = 128;
+inline int64_t gemm_slots(int bn) { return bn > gemm::BN ? 256 : 512; }
+inline int64_t gemm_tiles(const Geometry& g, int bn)
+{
+    return (int64_t)((g.pn_x + bn - 1) / bn) * ((g.pn_y + gemm::BM - 1) / gemm::BM);
+}
+// the most splits plan_gemm chooses for any launch of at most npoints points
+// (each bound below grows with the point count)
+inline int64_t gemm_max_splits(const Geometry& g, int64_t npoints, int kq, int bn)
+{
+    const int64_t n = npoints > 1 ? npoints : 1, tiles = gemm_tiles(g, bn);
+    const int64_t round1 = (gemm_slots(bn) + tiles - 1) / tiles;
+    const int64_t max_by_k = std::max<int64_t>(1, (n + 4 * kq - 1) / (4 * kq));
+    return std::min(std::max<int64_t>((n + GEMM_MAX_SLICE - 1) / GEMM_MAX_SLICE, 4 * round1), max_by_k);
+}
 // kq: K granularity of a split in points (one stage); bn: tile columns
 inline GemmPlan plan_gemm(const Geometry& g, int64_t npoints, int kq, int bn = gemm::BN)
 {
-    using namespace gemm;
     GemmPlan p;
     p.tiles_x = (g.pn_x + bn - 1) / bn;
-    p.tiles = p.tiles_x * ((g.pn_y + BM - 1) / BM);
+    p.tiles = (int)gemm_tiles(g, bn);
     p.npx = (int64_t)g.pn_x * g.pn_y;
-    // Split K so that (a) the 256 CUs get at least two rounds of workgroups
-    // (1024 of 128-column tiles, two resident per CU; 512 of 256-column
-    // tiles, one per CU) and (b) no FP32 accumulator sums more than MAX_SLICE
-    // points: FP32 rounding of the running sums grows like sqrt(points per slice).
-    constexpr int64_t MAX_SLICE = 65536;
-    const int64_t n = npoints > 1 ? npoints : 1;
-    const int64_t min_wgs = bn > BN ? 512 : 1024;
-    int64_t splits = (min_wgs + p.tiles - 1) / p.tiles;
-    if ((n + MAX_SLICE - 1) / MAX_SLICE > splits) splits = (n + MAX_SLICE - 1) / MAX_SLICE;
-    int64_t max_by_k = (n + 4 * kq - 1) / (4 * kq);   // >= 4 stages per split
-    if (max_by_k < 1) max_by_k = 1;
-    if (splits > max_by_k) splits = max_by_k;
+    const int64_t n = npoints > 1 ? npoints : 1, slots = gemm_slots(bn);
+    const int64_t round1 = (slots + p.tiles - 1) / p.tiles;
+    const int64_t max_by_k = std::max<int64_t>(1, (n + 4 * kq - 1) / (4 * kq));   // >= 4 stages per split
+    const int64_t lo = std::min(std::max((n + GEMM_MAX_SLICE - 1) / GEMM_MAX_SLICE, round1), max_by_k);
+    const int64_t hi = std::min(std::max(lo, 4 * round1), max_by_k);
+    int64_t splits = lo, best = INT64_MAX;
+    for (int64_t s = lo; s <= hi; ++s) {
+        const int64_t rounds = (p.tiles * s + slots - 1) / slots;
+        const int64_t cost = rounds * ((n + s - 1) / s) + GEMM_SPLIT_COST * s;
+        if (cost < best) {
+            best = cost;
+            splits = s;
+        }
+    }
     int64_t kslice = (n + splits - 1) / splits;
     kslice = (kslice + kq - 1) / kq * kq;
     p.splits = (n + kslice - 1) / kslice;

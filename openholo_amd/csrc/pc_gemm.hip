@@ -553,10 +553,11 @@ static size_t consts_bytes(int64_t npoints) { return ((size_t)npoints * sizeof(f
 // The split-K plan of a launch (that of the 256-column tiles).
 static GemmPlan fresnel_plan(const Geometry& g, int64_t npoints) { return plan_gemm(g, npoints, gemm::KP, 256); }
 
+// Enough for any launch of at most npoints points.
 size_t fresnel_gemm_scratch_bytes(const Geometry& g, int64_t npoints)
 {
-    const GemmPlan p = fresnel_plan(g, npoints);
-    return 256 + 2 * consts_bytes(npoints) + (size_t)p.splits * p.npx * sizeof(float2);
+    const int64_t splits = gemm_max_splits(g, npoints, gemm::KP, 256);
+    return 256 + 2 * consts_bytes(npoints) + (size_t)splits * g.pn_x * g.pn_y * sizeof(float2);
 }
 
 template <int VAR, bool MASK>

@@ -594,11 +594,7 @@ int launch_rs_partition(hipStream_t st, void* scratch, size_t scratch_bytes, con
 // split count, before rounding the slice to whole stages, bounds them all).
 size_t rs_gemm_scratch_bytes(const Geometry& g, int64_t npoints)
 {
-    using namespace gemm;
-    const int64_t tiles = (int64_t)((g.pn_x + BN - 1) / BN) * ((g.pn_y + BM - 1) / BM);
-    const int64_t n = npoints > 1 ? npoints : 1;
-    const int64_t splits = std::max<int64_t>((1024 + tiles - 1) / tiles, (n + 65535) / 65536);
-    return (size_t)splits * g.pn_x * g.pn_y * sizeof(float2);
+    return (size_t)gemm_max_splits(g, npoints, rsgemm::RP, gemm::BN) * g.pn_x * g.pn_y * sizeof(float2);
 }
 
 namespace {

@@ -235,6 +235,7 @@ int ohx_ctx_destroy(ohx_ctx* c)
     if (c->minmax) (void)hipFree(c->minmax);
     if (c->gemm_scratch) (void)hipFree(c->gemm_scratch);
     if (c->rs_scratch) (void)hipFree(c->rs_scratch);
+    if (c->rs_mask) (void)hipFree(c->rs_mask);
     wrp_release(c);
     (void)hipStreamSynchronize(c->copy_stream);
     (void)hipEventDestroy(c->copy_ready);

@@ -32,6 +32,10 @@ struct ohx_ctx {
     // subsets' constants (grown on demand)
     void* rs_scratch = nullptr;
     size_t rs_scratch_bytes = 0;
+    // RS matrix-core path: the masked points' tile mask (grown on demand; apart
+    // from gemm_scratch, which the direct kernel's point splits reuse)
+    void* rs_mask = nullptr;
+    size_t rs_mask_bytes = 0;
     // hipFFT plans of the WRP propagation (hipfftHandle) for an L x M grid:
     // length-L transforms of ceil(M/4) contiguous rows of the transposed
     // grid, length-M transforms of all L rows

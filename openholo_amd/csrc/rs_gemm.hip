@@ -683,11 +683,20 @@ int launch_rs_gemm(ohx_ctx* ctx, const Geometry& g, const PointSoA& soa, double*
     if (ne > 0) {
         int64_t chunk = ne;
         while (chunk > RP && rs_gemm_scratch_bytes(g, chunk) > MAX_RS_PART) chunk = (chunk + 1) / 2;
-        // [split-K planes] [tile mask]
-        const size_t planes = al256(rs_gemm_scratch_bytes(g, chunk));
-        if ((e = ensure_gemm_scratch(ctx, planes + (size_t)nm * words * sizeof(unsigned)))) return e;
+        if ((e = ensure_gemm_scratch(ctx, rs_gemm_scratch_bytes(g, chunk)))) return e;
         if (nm > 0) {
-            mask = (unsigned*)((char*)ctx->gemm_scratch + planes);
+            const size_t mbytes = (size_t)nm * words * sizeof(unsigned);
+            if (mbytes > ctx->rs_mask_bytes) {
+                if (ctx->rs_mask) {
+                    if ((e = check_hip(ctx, hipStreamSynchronize(st), "hipStreamSynchronize"))) return e;
+                    (void)hipFree(ctx->rs_mask);
+                    ctx->rs_mask = nullptr;
+                    ctx->rs_mask_bytes = 0;
+                }
+                if ((e = check_hip(ctx, hipMalloc(&ctx->rs_mask, mbytes), "hipMalloc(RS tile mask)"))) return e;
+                ctx->rs_mask_bytes = mbytes;
+            }
+            mask = (unsigned*)ctx->rs_mask;
             PointSoA masked = soa_tail(part, n_full);
             masked.n = nm;
             const int64_t items = nm * words;
@@ -697,8 +706,8 @@ int launch_rs_gemm(ohx_ctx* ctx, const Geometry& g, const PointSoA& soa, double*
         }
         for (int64_t p0 = 0; p0 < ne; p0 += chunk) {
             const int64_t p1 = std::min<int64_t>(ne, p0 + chunk);
-            e = launch_rs_gemm_range(st, ctx->gemm_scratch, planes, g, part, p0, p1, n_full, d_field, mode, &wgs,
-                                     mask, words);
+            e = launch_rs_gemm_range(st, ctx->gemm_scratch, ctx->gemm_scratch_bytes, g, part, p0, p1, n_full,
+                                     d_field, mode, &wgs, mask, words);
             if (e) return set_error(ctx, e, "RS matrix-core launch failed");
         }
     }

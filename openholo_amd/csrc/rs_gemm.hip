@@ -50,8 +50,9 @@
 // Operand generation, per stage and thread (point t % 8, rows and columns
 // grp + 32 i, grp = t / 8): the per-(point, tile) constants in FP64 (r_c from
 // an FP32 v_rsq seed and one Newton step, relative error ~2e-14), then per row
-// N and Y in FP64, Y reduced mod 1 before rounding to FP32, one v_sin/v_cos
-// pair for both V1 and V2 (and for U1, U2).  Precision: V1 and U1 in the
+// N and Y in FP64, Y reduced mod 1 before rounding to FP32 (BF16x1: N and Y
+// in FP32, then reduced), one v_sin/v_cos pair for both V1 and V2 (and for
+// U1, U2).  Precision: V1 and U1 in the
 // 3-term split (hi hi + hi lo + lo hi), V2 and U2 hi only (lo part zero): the
 // rank-2 correction is below theta of the term, so its 2^-9 BF16 rounding
 // costs below 2e-5 of the term (theta <= 8e-3).  BF16 only: A_c reaches ~1e7,
@@ -132,6 +133,7 @@ struct Tmp {
 struct Frame {
     double yc, xc;            // reference pixel (x0 + 64, y0 + 64)
     double vb, dv, ub, du;    // v of row grp + RG i = vb + i dv; u of column grp + RG j = ub + j du
+    float vbf, dvf, ubf, duf; // the same in FP32 (1-term units)
     double il, hl;            // 1/lambda, 1/(2 lambda)
     double um, vm;            // 64 pitches: the largest |u|, |v| on the tile
     TileMask tm;              // masked instance: the tiles it takes per point
@@ -190,11 +192,39 @@ __device__ inline void prep_piece(Pt& P, Tmp& t, const Raw& r, const Frame& f)
 // Unit E of a thread's stage: even E -> V row E/2, odd E -> U column E/2.
 // Value: (Re, Im) of the rank-1 and the rank-2 entry.
 // MASK: rows / columns outside the point's box get zero amplitude.
-template <int E, bool MASK>
+// F32 (the 1-term variant): the offsets, N and the phase in FP32, reduced by
+// v_fract_f32.  Where the range test holds on a tile the local frequency is
+// below half a cycle per pixel, so the phase stays below ~35 cycles and
+// carries below 1e-5 cycles of FP32 rounding, far below that variant's BF16
+// rounding of the entries (2^-9 of the term).
+template <int E, bool MASK, bool F32>
 __device__ inline float4 unit_value(const Pt& P, const Frame& f, int grp)
 {
     constexpr int I = E >> 1;
-    if constexpr ((E & 1) == 0) {
+    if constexpr (F32 && (E & 1) == 0) {
+        const float v = fmaf((float)I, f.dvf, f.vbf);
+        const float N = v * ((float)P.dy2 + v);
+        const float ph = __builtin_amdgcn_fractf(N * fmaf((float)P.k, N, (float)P.h));
+        float a = fmaf(fmaf(-P.beta, N, P.mu), N, 1.f);
+        if constexpr (MASK) {
+            const unsigned row = grp + RG * I;
+            a = (row >= __builtin_amdgcn_ubfe(P.box, 0, 8) && row < __builtin_amdgcn_ubfe(P.box, 8, 8)) ? a : 0.f;
+        }
+        const float c = a * cos_rev(ph), s = a * sin_rev(ph);
+        return make_float4(c, s, N * c, N * s);
+    } else if constexpr (F32) {
+        const float u = fmaf((float)I, f.duf, f.ubf);
+        const float n = u * ((float)P.dx2 + u);
+        const float ph = __builtin_amdgcn_fractf(fmaf(n, fmaf((float)P.k, n, (float)P.h), (float)P.phic));
+        float a = fmaf(fmaf(-P.acal, n, P.acmu), n, P.ac);
+        if constexpr (MASK) {
+            const unsigned col = grp + RG * I;
+            a = (col >= __builtin_amdgcn_ubfe(P.box, 16, 8) && col < __builtin_amdgcn_ubfe(P.box, 24, 8)) ? a : 0.f;
+        }
+        const float c = a * cos_rev(ph), s = a * sin_rev(ph);
+        const float gn = P.g * n;
+        return make_float4(c, s, -gn * s, gn * c);
+    } else if constexpr ((E & 1) == 0) {
         // the row offsets are recomputed per stage instead of being hoisted
         // out of the K loop, which would hold eight FP64 values in registers
         double vb = f.vb;
@@ -305,11 +335,11 @@ __device__ inline void slot(const unsigned* __restrict__ rv, const unsigned* __r
         if constexpr (S == (THREE ? NPR : (NPR + 1) / 2)) load_raw<MASK>(c.R, c.s, c.f.tm, c.pnext, c.ke);
         if constexpr (THREE) {
             constexpr int h = half_at(S);
-            if constexpr (h >= 0 && (h & 1) == 0) c.cs[h >> 1] = unit_value<(h >> 1), MASK>(c.P, c.f, c.grp);
+            if constexpr (h >= 0 && (h & 1) == 0) c.cs[h >> 1] = unit_value<(h >> 1), MASK, false>(c.P, c.f, c.grp);
             if constexpr (h >= 0 && (h & 1) == 1) unit_store<VAR, (h >> 1)>(wv, wu, c.cs[h >> 1], c.p, c.grp);
         } else {
             constexpr int e = unit_at(S);
-            if constexpr (e >= 0) unit_store<VAR, e>(wv, wu, unit_value<e, MASK>(c.P, c.f, c.grp), c.p, c.grp);
+            if constexpr (e >= 0) unit_store<VAR, e>(wv, wu, unit_value<e, MASK, true>(c.P, c.f, c.grp), c.p, c.grp);
         }
         __builtin_amdgcn_sched_barrier(0);
         slot<VAR, MASK, S + 1>(rv, ru, wv, wu, c);
@@ -321,7 +351,7 @@ template <int VAR, bool MASK, int E>
 __device__ inline void gen_units(unsigned* wv, unsigned* wu, const Pt& P, const Frame& f, int p, int grp)
 {
     if constexpr (E < RU) {
-        unit_store<VAR, E>(wv, wu, unit_value<E, MASK>(P, f, grp), p, grp);
+        unit_store<VAR, E>(wv, wu, unit_value<E, MASK, !Traits<VAR>::three>(P, f, grp), p, grp);
         gen_units<VAR, MASK, E + 1>(wv, wu, P, f, p, grp);
     }
 }
@@ -381,6 +411,10 @@ __global__ __launch_bounds__(NT, 2) void k_rs_gemm(Geometry g, PointSoA s, float
     f.dv = -(double)RG * g.pp_y;
     f.ub = (double)(grp - BN / 2) * g.pp_x;
     f.du = (double)RG * g.pp_x;
+    f.vbf = (float)f.vb;
+    f.dvf = (float)f.dv;
+    f.ubf = (float)f.ub;
+    f.duf = (float)f.du;
     f.il = g.inv_lambda;
     f.hl = 0.5 * g.inv_lambda;
     f.um = (BN / 2) * g.pp_x;

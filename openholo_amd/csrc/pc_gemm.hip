@@ -37,28 +37,53 @@ namespace ohx {
 
 namespace gemm {
 
-// Point data of one stage as read from HBM: thread t holds point t % 16.
-// pc = (v2, u2, amplitude, 0) from k_gemm_consts.
-struct Raw {
+// One point's stage data, written per launch by k_gemm_consts for one tile
+// shape: position, 1/(2 lambda z), z/lambda cycles, the k^2 coefficients
+// (v2, u2), the amplitude and the box; 64 bytes, read as 16-byte buffer
+// loads at one 32-bit offset (the box only by the masked instances).
+struct __align__(16) PointRec {
     double px, py, q, cyc0;
-    float4 pc;
+    float v2, u2, amp, pad;
     int4 box;
+};
+static_assert(sizeof(PointRec) == 64, "16-byte loads at a 64-byte stride");
+// Records of one launch: at most MAX_REC points, so that every byte offset
+// fits the 32-bit buffer offset.
+constexpr int64_t MAX_REC = int64_t(1) << 24;
+using Rsrc = __amdgpu_buffer_rsrc_t;
+__device__ inline Rsrc rec_rsrc(const PointRec* rec, int n)
+{
+    // raw (untyped) buffer of n records; 0x20000: the 32-bit data format,
+    // which raw loads on gfx9 need to be non-zero
+    return __builtin_amdgcn_make_buffer_rsrc((void*)rec, (short)0, n * (int)sizeof(PointRec), 0x00020000);
+}
+
+// Point data of one stage as read from HBM: thread t holds point t % 16.
+struct Raw {
+    PointRec c;
     bool valid;
 };
 
-// Lanes past the slice end read the slice's last point and are marked invalid
+// Point pi of the launch's records (pi relative to the first).  Lanes past
+// the slice end (pi >= p1) read the slice's last point and are marked invalid
 // (zero amplitude, full box), so the loads are unconditional and in bounds.
-__device__ inline void load_raw(Raw& r, const PointSoA& s, const float4* __restrict__ pconst, int64_t kfirst,
-                                int64_t pi, int64_t p1)
+// (The unmasked instances never read the box, and its load is dropped.)
+__device__ inline void load_raw(Raw& r, Rsrc rs, int pi, int p1)
 {
     r.valid = pi < p1;
-    const int64_t j = r.valid ? pi : p1 - 1;
-    r.px = s.px[j];
-    r.py = s.py[j];
-    r.q = s.q[j];
-    r.cyc0 = s.cyc0[j];
-    r.pc = pconst[j - kfirst];
-    r.box = s.box[j];
+    const int off = min(pi, p1 - 1) * (int)sizeof(PointRec);
+    const u32x4 a = __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0);
+    const u32x4 b = __builtin_amdgcn_raw_buffer_load_b128(rs, off + 16, 0, 0);
+    const u32x4 c = __builtin_amdgcn_raw_buffer_load_b128(rs, off + 32, 0, 0);
+    r.c.px = __builtin_bit_cast(double, make_uint2(a.x, a.y));
+    r.c.py = __builtin_bit_cast(double, make_uint2(a.z, a.w));
+    r.c.q = __builtin_bit_cast(double, make_uint2(b.x, b.y));
+    r.c.cyc0 = __builtin_bit_cast(double, make_uint2(b.z, b.w));
+    r.c.v2 = __uint_as_float(c.x);
+    r.c.u2 = __uint_as_float(c.y);
+    r.c.amp = __uint_as_float(c.z);
+    const u32x4 d = __builtin_amdgcn_raw_buffer_load_b128(rs, off + 48, 0, 0);
+    r.c.box = make_int4((int)d.x, (int)d.y, (int)d.z, (int)d.w);
 }
 
 // Per-(point, thread) constants of one stage.  Thread t generates V rows
@@ -81,28 +106,28 @@ struct Pt {
 // dx = NGRP pp_x (row index grows toward -y).
 __device__ inline void prep_y(Pt& P, const Raw& r, double ythr, double dy)
 {
-    const double ey = ythr - r.py, ty = r.q * ey;
-    P.v0 = (float)frac64(ty * ey + r.cyc0);
+    const double ey = ythr - r.c.py, ty = r.c.q * ey;
+    P.v0 = (float)frac64(ty * ey + r.c.cyc0);
     P.v1 = (float)frac64(ty * (2.0 * dy));
-    P.v2 = r.pc.x;
+    P.v2 = r.c.v2;
 }
 __device__ inline void prep_x(Pt& P, const Raw& r, double xthr, double dx)
 {
-    const double ex = xthr - r.px, tx = r.q * ex;
+    const double ex = xthr - r.c.px, tx = r.c.q * ex;
     P.u0 = (float)frac64(tx * ex);
     P.u1 = (float)frac64(tx * (2.0 * dx));
-    P.u2 = r.pc.y;
+    P.u2 = r.c.u2;
 }
 // amplitude and box; returns true when the box does not cover the whole
 // BM x W tile
 template <int W>
 __device__ inline bool prep_rest(Pt& P, const Raw& r, int x0, int y0)
 {
-    P.amp = r.valid ? r.pc.z : 0.f;
-    P.rlo = r.valid ? r.box.z - y0 : 0;
-    P.rhi = r.valid ? r.box.w - y0 : BM;
-    P.clo = r.valid ? r.box.x - x0 : 0;
-    P.chi = r.valid ? r.box.y - x0 : W;
+    P.amp = r.valid ? r.c.amp : 0.f;
+    P.rlo = r.valid ? r.c.box.z - y0 : 0;
+    P.rhi = r.valid ? r.c.box.w - y0 : BM;
+    P.clo = r.valid ? r.c.box.x - x0 : 0;
+    P.chi = r.valid ? r.c.box.y - x0 : W;
     return P.rlo > 0 || P.rhi < BM || P.clo > 0 || P.chi < W;
 }
 template <int W>
@@ -121,21 +146,21 @@ __device__ inline void prep_piece(Pt& P, double (&t)[4], const Raw& r, double yt
                                   double dx)
 {
     if constexpr (K == 0) {
-        t[0] = ythr - r.py;
-        t[1] = r.q * t[0];
+        t[0] = ythr - r.c.py;
+        t[1] = r.c.q * t[0];
     } else if constexpr (K == 1) {
-        P.v0 = (float)frac64(t[1] * t[0] + r.cyc0);
+        P.v0 = (float)frac64(t[1] * t[0] + r.c.cyc0);
     } else if constexpr (K == 2) {
         P.v1 = (float)frac64(t[1] * (2.0 * dy));
-        P.v2 = r.pc.x;
+        P.v2 = r.c.v2;
     } else if constexpr (K == 3) {
-        t[2] = xthr - r.px;
-        t[3] = r.q * t[2];
+        t[2] = xthr - r.c.px;
+        t[3] = r.c.q * t[2];
     } else if constexpr (K == 4) {
         P.u0 = (float)frac64(t[3] * t[2]);
     } else if constexpr (K == 5) {
         P.u1 = (float)frac64(t[3] * (2.0 * dx));
-        P.u2 = r.pc.y;
+        P.u2 = r.c.u2;
     }
 }
 
@@ -247,9 +272,8 @@ template <int VAR, int W>
 struct StageCtx {
     f32x16 (&acc)[4][2];
     Raw& R;
-    const PointSoA& s;
-    const float4* pconst;
-    int64_t kfirst, pnext, ke;
+    Rsrc rs;
+    int pnext, ke;
     double ythr, xthr, dy, dx;
     int x0, y0;
     int p, grp, lrow, koff, ucol;
@@ -325,7 +349,7 @@ __device__ inline void slot(const unsigned* __restrict__ rv, const unsigned* __r
         }
         if constexpr (S == (THREE ? NPIECE : NPIECE / 2)) {
             c.partial = prep_rest<W>(c.P, c.R, c.x0, c.y0);
-            load_raw(c.R, c.s, c.pconst, c.kfirst, c.pnext, c.ke);
+            load_raw(c.R, c.rs, c.pnext, c.ke);
         }
         if constexpr (THREE) {
             constexpr int h = half_at<W>(S);
@@ -352,11 +376,10 @@ __device__ inline void slot(const unsigned* __restrict__ rv, const unsigned* __r
 template <int VAR, int W>
 __device__ inline bool stage_body(const unsigned* __restrict__ rv, const unsigned* __restrict__ ru,
                                   unsigned* __restrict__ wv, unsigned* __restrict__ wu, f32x16 (&acc)[4][2],
-                                  Pt& P, Raw& R, const PointSoA& s, const float4* pconst, int64_t kfirst,
-                                  int64_t pnext, int64_t ke, double ythr, double xthr, double dy, double dx, int x0,
-                                  int y0, int p, int grp, int lrow, int koff, int ucol)
+                                  Pt& P, Raw& R, Rsrc rs, int pnext, int ke, double ythr, double xthr, double dy,
+                                  double dx, int x0, int y0, int p, int grp, int lrow, int koff, int ucol)
 {
-    StageCtx<VAR, W> c{acc, R, s, pconst, kfirst, pnext, ke, ythr, xthr, dy, dx, x0, y0, p, grp, lrow, koff, ucol};
+    StageCtx<VAR, W> c{acc, R, rs, pnext, ke, ythr, xthr, dy, dx, x0, y0, p, grp, lrow, koff, ucol};
     c.partial = false;
     stage_operands<VAR, Shape<W>::PLANE_U>(rv, ru, c);
     slot<VAR, W, 0>(rv, ru, wv, wu, c);
@@ -373,16 +396,14 @@ __device__ inline bool stage_body(const unsigned* __restrict__ rv, const unsigne
 // whose box does not cover the tile zero its out-of-box elements (a ballot
 // and a branch per stage, ~5 % of the kernel, paid only by such launches).
 template <int VAR, int W, int RB, bool MASK>
-__device__ inline void step(Smem<Traits<VAR>::three, W>& sm, f32x16 (&acc)[4][2], Raw& R, const PointSoA& s,
-                            const float4* pconst, int64_t kfirst, int st, int64_t kb, int64_t ke, double ythr,
-                            double xthr, double dy, double dx, int x0, int y0, int p, int grp, int lrow, int koff,
-                            int ucolx)
+__device__ inline void step(Smem<Traits<VAR>::three, W>& sm, f32x16 (&acc)[4][2], Raw& R, Rsrc rs, int st,
+                            int kb, int ke, double ythr, double xthr, double dy, double dx, int x0, int y0, int p,
+                            int grp, int lrow, int koff, int ucolx)
 {
     Pt P;
     const bool partial = stage_body<VAR, W>(&sm.v[RB][0][0][0], &sm.u[RB][0][0][0], &sm.v[RB ^ 1][0][0][0],
-                                         &sm.u[RB ^ 1][0][0][0], acc, P, R, s, pconst, kfirst,
-                                         kb + (int64_t)(st + 2) * KP + p, ke, ythr, xthr, dy, dx, x0, y0, p, grp,
-                                         lrow, koff, ucolx);
+                                         &sm.u[RB ^ 1][0][0][0], acc, P, R, rs, kb + (st + 2) * KP + p, ke, ythr,
+                                         xthr, dy, dx, x0, y0, p, grp, lrow, koff, ucolx);
     if constexpr (MASK) {
         if (__builtin_amdgcn_ballot_w64(partial)) mask_stage<VAR, W, RB ^ 1>(sm, P, p, grp);
     } else {
@@ -394,7 +415,7 @@ __device__ inline void step(Smem<Traits<VAR>::three, W>& sm, f32x16 (&acc)[4][2]
 // One workgroup's tile and split: prologue, K loop, FP32 partials.
 template <int VAR, int W, bool MASK>
 __device__ __attribute__((always_inline)) inline void fresnel_tile(Smem<Traits<VAR>::three, W>& sm, const Geometry& g,
-                                                                   const PointSoA& s, const float4* pconst,
+                                                                   const PointRec* pconst,
                                                                    float2* part, int tiles_x, int tiles,
                                                                    int64_t kfirst, int64_t klast, int64_t kslice)
 {
@@ -403,9 +424,12 @@ __device__ __attribute__((always_inline)) inline void fresnel_tile(Smem<Traits<V
     const int tx = tile % tiles_x, ty = tile / tiles_x;
     using Sh = Shape<W>;
     const int x0 = tx * W, y0 = ty * BM;
-    const int64_t kb = kfirst + (int64_t)split * kslice;
-    const int64_t ke = min(klast, kb + kslice);
-    const int nst = (int)((ke - kb + KP - 1) / KP);
+    // point indices relative to kfirst (fewer than MAX_REC per launch)
+    const int np = (int)(klast - kfirst);
+    const int kb = (int)(split * kslice);
+    const int ke = min(np, (int)(kb + kslice));
+    const int nst = (ke - kb + KP - 1) / KP;
+    const Rsrc rs = rec_rsrc(pconst, np);
     const int p = threadIdx.x & (KP - 1), grp = threadIdx.x >> 4;
     const double ythr = gpix_y(g, y0 + grp + Sh::NGRP * (Sh::NVT / 2));
     const double xthr = gpix_x(g, x0 + grp + Sh::NGRP * (Sh::NUT / 2));
@@ -419,7 +443,7 @@ __device__ __attribute__((always_inline)) inline void fresnel_tile(Smem<Traits<V
     // split, which the planner never makes, would write zeros)
     Raw R;
     if (nst > 0) {
-        load_raw(R, s, pconst, kfirst, kb + p, ke);
+        load_raw(R, rs, kb + p, ke);
         Pt P;
         const bool partial = prep<W>(P, R, ythr, xthr, dy, dx, x0, y0);
         gen_stage<VAR, W, 0>(sm, P, p, grp);
@@ -428,18 +452,17 @@ __device__ __attribute__((always_inline)) inline void fresnel_tile(Smem<Traits<V
         } else {
             (void)partial;
         }
-        load_raw(R, s, pconst, kfirst, kb + KP + p, ke);
+        load_raw(R, rs, kb + KP + p, ke);
     }
     __syncthreads();
     f32x16 acc[4][2];
 #pragma unroll
     for (int i = 0; i < 4; ++i) acc[i][0] = acc[i][1] = f32x16{};
     for (int st = 0; st < nst; st += 2) {
-        step<VAR, W, 0, MASK>(sm, acc, R, s, pconst, kfirst, st, kb, ke, ythr, xthr, dy, dx, x0, y0, p, grp,
-                              lrow, koff, ucolx);
+        step<VAR, W, 0, MASK>(sm, acc, R, rs, st, kb, ke, ythr, xthr, dy, dx, x0, y0, p, grp, lrow, koff, ucolx);
         if (st + 1 >= nst) break;
-        step<VAR, W, 1, MASK>(sm, acc, R, s, pconst, kfirst, st + 1, kb, ke, ythr, xthr, dy, dx, x0, y0, p, grp,
-                              lrow, koff, ucolx);
+        step<VAR, W, 1, MASK>(sm, acc, R, rs, st + 1, kb, ke, ythr, xthr, dy, dx, x0, y0, p, grp, lrow, koff,
+                              ucolx);
     }
 
     store_partials(g, acc, part, split, x0, y0, ucolx, lane);
@@ -453,12 +476,12 @@ __device__ __attribute__((always_inline)) inline void fresnel_tile(Smem<Traits<V
 // 8-wave workgroup (W = 256) per CU.
 template <int VAR, int W, bool MASK>
 __global__ __launch_bounds__(Shape<W>::NT, 512 / Shape<W>::NT) void k_fresnel_gemm(
-    Geometry g, PointSoA s, const float4* __restrict__ pconst, const unsigned* __restrict__ partial_flag,
+    Geometry g, const PointRec* __restrict__ pconst, const unsigned* __restrict__ partial_flag,
     float2* __restrict__ part, int tiles_x, int tiles, int64_t kfirst, int64_t klast, int64_t kslice)
 {
     __shared__ __align__(16) Smem<Traits<VAR>::three, W> sm;
     if ((__builtin_amdgcn_readfirstlane(partial_flag[0]) != 0) != MASK) return;
-    fresnel_tile<VAR, W, MASK>(sm, g, s, pconst, part, tiles_x, tiles, kfirst, klast, kslice);
+    fresnel_tile<VAR, W, MASK>(sm, g, pconst, part, tiles_x, tiles, kfirst, klast, kslice);
 }
 
 // field += -j * amax * sum_split S   (FP64, fixed summation order)
@@ -502,17 +525,18 @@ __global__ void k_amp_init(float* amax)
     }
 }
 
-// Per-point constants of the stage generation, once per launch instead of once
-// per (point, tile, thread): the k^2 coefficients v2 = frac(q dy^2) and
-// u2 = frac(q dx^2) (dy = -ngrp pp_y, dx = ngrp pp_x, ngrp the tile shape's
-// thread groups, 16 into pc_narrow and 32 into pc_wide; FP64 then reduced
-// mod 1 and rounded) and the amplitude a/(lambda z), times 1/max|A| for FP16.  Also
+// Per-point records of the stage generation, once per launch instead of once
+// per (point, tile, thread): the point's position, q and cycles, the k^2
+// coefficients v2 = frac(q dy^2) and u2 = frac(q dx^2) (dy = -ngrp pp_y,
+// dx = ngrp pp_x, ngrp the tile shape's thread groups, 16 into pc_narrow and
+// 32 into pc_wide; FP64 then reduced mod 1 and rounded) and the amplitude
+// a/(lambda z), times 1/max|A| for FP16.  Also
 // raises *partial_flag when a point's box is smaller than the hologram (one
 // atomic per wave holding such a point, skipped once the flag reads as set).
 __global__ __launch_bounds__(256) void k_gemm_consts(Geometry g, PointSoA s, int64_t p0, int64_t p1,
                                                      const float* __restrict__ amax, int scaled,
-                                                     float4* __restrict__ pc_narrow, float4* __restrict__ pc_wide,
-                                                     unsigned* partial_flag)
+                                                     PointRec* __restrict__ pc_narrow,
+                                                     PointRec* __restrict__ pc_wide, unsigned* partial_flag)
 {
     const int64_t i = p0 + (int64_t)blockIdx.x * 256 + threadIdx.x;
     bool partial = false;
@@ -522,9 +546,10 @@ __global__ __launch_bounds__(256) void k_gemm_consts(Geometry g, PointSoA s, int
         const float a = (float)s.kamp[i] * ((scaled && am > 0.f) ? 1.0f / am : 1.0f);
         const double dy = -Shape<128>::NGRP * g.pp_y, dx = Shape<128>::NGRP * g.pp_x;
         const double dy2 = q * (dy * dy), dx2 = q * (dx * dx);
-        pc_narrow[i - p0] = make_float4((float)frac64(dy2), (float)frac64(dx2), a, 0.f);
-        pc_wide[i - p0] = make_float4((float)frac64(4.0 * dy2), (float)frac64(4.0 * dx2), a, 0.f);
+        const double px = s.px[i], py = s.py[i], cyc0 = s.cyc0[i];
         const int4 b = s.box[i];
+        pc_narrow[i - p0] = PointRec{px, py, q, cyc0, (float)frac64(dy2), (float)frac64(dx2), a, 0.f, b};
+        pc_wide[i - p0] = PointRec{px, py, q, cyc0, (float)frac64(4.0 * dy2), (float)frac64(4.0 * dx2), a, 0.f, b};
         partial = b.x > 0 || b.y < g.pn_x || b.z > 0 || b.w < g.pn_y;
     }
     if (__builtin_amdgcn_ballot_w64(partial) && (threadIdx.x & 63) == 0 &&
@@ -546,9 +571,12 @@ constexpr int tile_width()
     return (MASK && gemm::Traits<VAR>::three) ? 128 : 256;
 }
 
-// scratch: [amax, 256 B] [per-point float4 constants for 128- and
-// 256-column tiles, each padded to 256 B] [split-K partial planes]
-static size_t consts_bytes(int64_t npoints) { return ((size_t)npoints * sizeof(float4) + 255) / 256 * 256; }
+// scratch: [amax, 256 B] [per-point records for 128- and 256-column tiles,
+// each padded to 256 B] [split-K partial planes]
+static size_t consts_bytes(int64_t npoints)
+{
+    return ((size_t)npoints * sizeof(gemm::PointRec) + 255) / 256 * 256;
+}
 
 // The split-K plan of a launch (that of the 256-column tiles).
 static GemmPlan fresnel_plan(const Geometry& g, int64_t npoints) { return plan_gemm(g, npoints, gemm::KP, 256); }
@@ -561,27 +589,27 @@ size_t fresnel_gemm_scratch_bytes(const Geometry& g, int64_t npoints)
 }
 
 template <int VAR, bool MASK>
-static void launch_instance(hipStream_t stream, const Geometry& g, const PointSoA& soa, const float4* pc_narrow,
-                            const float4* pc_wide, const unsigned* pf, float2* part, const GemmPlan& pl, int64_t p0,
-                            int64_t p1)
+static void launch_instance(hipStream_t stream, const Geometry& g, const gemm::PointRec* pc_narrow,
+                            const gemm::PointRec* pc_wide, const unsigned* pf, float2* part, const GemmPlan& pl,
+                            int64_t p0, int64_t p1)
 {
     using namespace gemm;
     constexpr int W = tile_width<VAR, MASK>();
     const int tiles_x = (g.pn_x + W - 1) / W, tiles = tiles_x * ((g.pn_y + BM - 1) / BM);
     const dim3 grid((unsigned)((int64_t)tiles * pl.splits)), block(Shape<W>::NT);
-    hipLaunchKernelGGL((k_fresnel_gemm<VAR, W, MASK>), grid, block, 0, stream, g, soa,
-                       W == 256 ? pc_wide : pc_narrow, pf, part, tiles_x, tiles, p0, p1, pl.kslice);
+    hipLaunchKernelGGL((k_fresnel_gemm<VAR, W, MASK>), grid, block, 0, stream, g, W == 256 ? pc_wide : pc_narrow,
+                       pf, part, tiles_x, tiles, p0, p1, pl.kslice);
 }
 
 // Both instances; the one that does not match the device's partial-box flag
 // returns at once.
 template <int VAR>
-static void launch_variant(hipStream_t stream, const Geometry& g, const PointSoA& soa, const float4* pc_narrow,
-                           const float4* pc_wide, const unsigned* pf, float2* part, const GemmPlan& pl, int64_t p0,
-                           int64_t p1)
+static void launch_variant(hipStream_t stream, const Geometry& g, const gemm::PointRec* pc_narrow,
+                           const gemm::PointRec* pc_wide, const unsigned* pf, float2* part, const GemmPlan& pl,
+                           int64_t p0, int64_t p1)
 {
-    launch_instance<VAR, false>(stream, g, soa, pc_narrow, pc_wide, pf, part, pl, p0, p1);
-    launch_instance<VAR, true>(stream, g, soa, pc_narrow, pc_wide, pf, part, pl, p0, p1);
+    launch_instance<VAR, false>(stream, g, pc_narrow, pc_wide, pf, part, pl, p0, p1);
+    launch_instance<VAR, true>(stream, g, pc_narrow, pc_wide, pf, part, pl, p0, p1);
 }
 
 static int variant_of(uint32_t mode)
@@ -590,17 +618,17 @@ static int variant_of(uint32_t mode)
     return (mode & OHX_MODE_MFMA_FP16X3) ? V_FP16X3 : (mode & OHX_MODE_MFMA_BF16X3) ? V_BF16X3 : V_BF16X1;
 }
 
-int launch_fresnel_gemm_range(hipStream_t stream, void* scratch, size_t scratch_bytes, const Geometry& g,
-                              const PointSoA& soa, int64_t p0, int64_t p1, double* d_field, uint32_t mode)
+// One launch sequence over at most MAX_REC points.
+static int launch_piece(hipStream_t stream, void* scratch, size_t scratch_bytes, const Geometry& g,
+                        const PointSoA& soa, int64_t p0, int64_t p1, double* d_field, uint32_t mode)
 {
     using namespace gemm;
-    if (p1 <= p0) return OHX_OK;
     const int var = variant_of(mode);
     const GemmPlan pl = fresnel_plan(g, p1 - p0);
     if (fresnel_gemm_scratch_bytes(g, p1 - p0) > scratch_bytes) return OHX_ERR_INVALID_ARG;
     float* amax = (float*)scratch;
-    float4* pc_narrow = (float4*)((char*)scratch + 256);
-    float4* pc_wide = (float4*)((char*)scratch + 256 + consts_bytes(p1 - p0));
+    auto* pc_narrow = (gemm::PointRec*)((char*)scratch + 256);
+    auto* pc_wide = (gemm::PointRec*)((char*)scratch + 256 + consts_bytes(p1 - p0));
     float2* part = (float2*)((char*)scratch + 256 + 2 * consts_bytes(p1 - p0));
     hipLaunchKernelGGL(k_amp_init, dim3(1), dim3(64), 0, stream, amax);
     hipLaunchKernelGGL(k_amp_max, dim3(256), dim3(256), 0, stream, (const double*)soa.kamp, p0, p1, amax);
@@ -610,13 +638,13 @@ int launch_fresnel_gemm_range(hipStream_t stream, void* scratch, size_t scratch_
     const unsigned* pf = partial_flag;
     switch (var) {
     case V_BF16X1:
-        launch_variant<V_BF16X1>(stream, g, soa, pc_narrow, pc_wide, pf, part, pl, p0, p1);
+        launch_variant<V_BF16X1>(stream, g, pc_narrow, pc_wide, pf, part, pl, p0, p1);
         break;
     case V_BF16X3:
-        launch_variant<V_BF16X3>(stream, g, soa, pc_narrow, pc_wide, pf, part, pl, p0, p1);
+        launch_variant<V_BF16X3>(stream, g, pc_narrow, pc_wide, pf, part, pl, p0, p1);
         break;
     default:
-        launch_variant<V_FP16X3>(stream, g, soa, pc_narrow, pc_wide, pf, part, pl, p0, p1);
+        launch_variant<V_FP16X3>(stream, g, pc_narrow, pc_wide, pf, part, pl, p0, p1);
         break;
     }
     if (hipGetLastError() != hipSuccess) return OHX_ERR_DEVICE;
@@ -625,6 +653,18 @@ int launch_fresnel_gemm_range(hipStream_t stream, void* scratch, size_t scratch_
     hipLaunchKernelGGL(k_gemm_reduce, dim3((unsigned)rg), dim3(256), 0, stream, (const float2*)part, (int)pl.splits,
                        pl.npx, (double2*)d_field, (const float*)amax, var == V_FP16X3 ? 1 : 0);
     return hipGetLastError() == hipSuccess ? OHX_OK : OHX_ERR_DEVICE;
+}
+// Scratch sized for p1 - p0 points holds any shorter piece's.
+int launch_fresnel_gemm_range(hipStream_t stream, void* scratch, size_t scratch_bytes, const Geometry& g,
+                              const PointSoA& soa, int64_t p0, int64_t p1, double* d_field, uint32_t mode)
+{
+    if (p1 <= p0) return OHX_OK;
+    if (fresnel_gemm_scratch_bytes(g, p1 - p0) > scratch_bytes) return OHX_ERR_INVALID_ARG;
+    int st = OHX_OK;
+    for (int64_t q0 = p0; q0 < p1 && st == OHX_OK; q0 += gemm::MAX_REC)
+        st = launch_piece(stream, scratch, scratch_bytes, g, soa, q0, std::min(p1, q0 + gemm::MAX_REC), d_field,
+                          mode);
+    return st;
 }
 
 int launch_gemm_reduce(hipStream_t stream, const void* part, int64_t splits, int64_t npx, double* d_field)
@@ -646,7 +686,7 @@ constexpr size_t MAX_GEMM_SCRATCH = size_t(4) << 30;
 int launch_fresnel_gemm(ohx_ctx* ctx, const Geometry& g, const PointSoA& soa, double* d_field, uint32_t mode)
 {
     int64_t chunk = soa.n;
-    while (chunk > gemm::KP && fresnel_gemm_scratch_bytes(g, chunk) > MAX_GEMM_SCRATCH)
+    while (chunk > gemm::KP && (fresnel_gemm_scratch_bytes(g, chunk) > MAX_GEMM_SCRATCH || chunk > gemm::MAX_REC))
         chunk = (chunk + 1) / 2;
     const size_t bytes = fresnel_gemm_scratch_bytes(g, chunk);
     int st = ensure_gemm_scratch(ctx, bytes);

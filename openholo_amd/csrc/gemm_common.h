@@ -108,6 +108,18 @@ __device__ inline double gpix_y(const Geometry& g, int row)
 // x - floor(x) in one v_fract_f64 (clamped below 1.0)
 __device__ inline double frac64(double x) { return __builtin_amdgcn_fract(x); }
 
+// Per-point records of a launch (64 bytes each) are read with buffer loads at
+// a 32-bit byte offset: at most MAX_REC records per launch.
+constexpr int64_t MAX_REC = int64_t(1) << 24;
+using Rsrc = __amdgpu_buffer_rsrc_t;
+__device__ inline Rsrc rec_rsrc(const void* base, int bytes)
+{
+    // raw (untyped) buffer; 0x20000: the 32-bit data format, which raw
+    // loads on gfx9 need to be non-zero
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, bytes, 0x00020000);
+}
+__device__ inline u32x4 rec_load16(Rsrc rs, int off) { return __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0); }
+
 // LDS plane layout: plane h (hi / lo part) of one buffer's V block is BM rows
 // of ROWW dwords (U block: BN rows); element (row, p) at h*PLANE + row*ROWW + p.
 constexpr int PLANE_V = BM * ROWW;

@@ -222,15 +222,19 @@ int launch_rs_gemm(ohx_ctx* ctx, const Geometry& g, const PointSoA& soa, double*
 // scratch; split puts every point whose expansion may hold on a tile in the
 // masked group, whose instance then takes only the tiles set in its tile mask
 // (k_rs_tilemask), the direct kernel the others with Geometry::rs_skip_mask)
-// and the GEMM over part[p0, p1) into d_field (points from n_full on masked,
+// and the GEMM over the partitioned points [p0, p1), read from their records
+// rec (rs_partition_records), into d_field (points from n_full on masked,
 // with mask rows from point n_full on if mask is given; wgs, if given, adds
 // the workgroups launched)
 size_t rs_partition_scratch_bytes(int64_t n);
+// the partitioned points' records (the matrix-core kernels' input) inside
+// the partition scratch of n points
+const void* rs_partition_records(const void* scratch, int64_t n);
 int launch_rs_partition(hipStream_t st, void* scratch, size_t scratch_bytes, const Geometry& g, const PointSoA& soa,
                         const PointSoA& part, uint32_t mode, bool split, unsigned long long** d_count);
 size_t rs_gemm_scratch_bytes(const Geometry& g, int64_t npoints);
 int launch_rs_gemm_range(hipStream_t st, void* scratch, size_t scratch_bytes, const Geometry& g,
-                         const PointSoA& part, int64_t p0, int64_t p1, int64_t n_full, double* d_field,
+                         const void* rec, int64_t p0, int64_t p1, int64_t n_full, double* d_field,
                          uint32_t mode, int64_t* wgs = nullptr, const unsigned* mask = nullptr,
                          int mask_words = 0);
 int launch_probe(ohx_ctx* ctx, const Geometry& g, const PointSoA& soa, const int64_t* d_pix,

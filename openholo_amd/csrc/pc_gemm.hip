@@ -47,16 +47,6 @@ struct __align__(16) PointRec {
     int4 box;
 };
 static_assert(sizeof(PointRec) == 64, "16-byte loads at a 64-byte stride");
-// Records of one launch: at most MAX_REC points, so that every byte offset
-// fits the 32-bit buffer offset.
-constexpr int64_t MAX_REC = int64_t(1) << 24;
-using Rsrc = __amdgpu_buffer_rsrc_t;
-__device__ inline Rsrc rec_rsrc(const PointRec* rec, int n)
-{
-    // raw (untyped) buffer of n records; 0x20000: the 32-bit data format,
-    // which raw loads on gfx9 need to be non-zero
-    return __builtin_amdgcn_make_buffer_rsrc((void*)rec, (short)0, n * (int)sizeof(PointRec), 0x00020000);
-}
 
 // Point data of one stage as read from HBM: thread t holds point t % 16.
 struct Raw {
@@ -72,9 +62,7 @@ __device__ inline void load_raw(Raw& r, Rsrc rs, int pi, int p1)
 {
     r.valid = pi < p1;
     const int off = min(pi, p1 - 1) * (int)sizeof(PointRec);
-    const u32x4 a = __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0);
-    const u32x4 b = __builtin_amdgcn_raw_buffer_load_b128(rs, off + 16, 0, 0);
-    const u32x4 c = __builtin_amdgcn_raw_buffer_load_b128(rs, off + 32, 0, 0);
+    const u32x4 a = rec_load16(rs, off), b = rec_load16(rs, off + 16), c = rec_load16(rs, off + 32);
     r.c.px = __builtin_bit_cast(double, make_uint2(a.x, a.y));
     r.c.py = __builtin_bit_cast(double, make_uint2(a.z, a.w));
     r.c.q = __builtin_bit_cast(double, make_uint2(b.x, b.y));
@@ -82,7 +70,7 @@ __device__ inline void load_raw(Raw& r, Rsrc rs, int pi, int p1)
     r.c.v2 = __uint_as_float(c.x);
     r.c.u2 = __uint_as_float(c.y);
     r.c.amp = __uint_as_float(c.z);
-    const u32x4 d = __builtin_amdgcn_raw_buffer_load_b128(rs, off + 48, 0, 0);
+    const u32x4 d = rec_load16(rs, off + 48);
     r.c.box = make_int4((int)d.x, (int)d.y, (int)d.z, (int)d.w);
 }
 
@@ -429,7 +417,7 @@ __device__ __attribute__((always_inline)) inline void fresnel_tile(Smem<Traits<V
     const int kb = (int)(split * kslice);
     const int ke = min(np, (int)(kb + kslice));
     const int nst = (ke - kb + KP - 1) / KP;
-    const Rsrc rs = rec_rsrc(pconst, np);
+    const Rsrc rs = rec_rsrc(pconst, np * (int)sizeof(PointRec));
     const int p = threadIdx.x & (KP - 1), grp = threadIdx.x >> 4;
     const double ythr = gpix_y(g, y0 + grp + Sh::NGRP * (Sh::NVT / 2));
     const double xthr = gpix_x(g, x0 + grp + Sh::NGRP * (Sh::NUT / 2));

@@ -83,14 +83,21 @@ constexpr double MAX_THETA_X3 = 8e-3; // rad, BF16x3
 constexpr double MAX_THETA_X1 = 3.5e-2; // rad, BF16x1
 constexpr double RANGE_TOL = 1e-12;   // m, margin of the range test
 
-// Per-tile mask of the masked instance (k_rs_tilemask): point j's row starts
-// at bits + (j - base) * words; the workgroup's tile is bit `tile`.  Null
-// bits: every tile of every point.
+// Per-tile mask of the masked instance (k_rs_tilemask): the launch's point jr
+// (relative to its first) has its row at bits + jr * words; the workgroup's
+// tile is bit `tile`.  Null bits: every tile of every point.
 struct TileMask {
     const unsigned* bits;
-    int64_t base;
     int words, tile;
 };
+
+// One partitioned point (k_rs_gather): position, z, cycles, amplitude and
+// box, 64 bytes, read as buffer loads at one 32-bit offset.
+struct __align__(16) RsRec {
+    double px, py, pz, cyc0, kamp, pad;
+    int4 box;
+};
+static_assert(sizeof(RsRec) == 64, "16-byte loads at a 64-byte stride");
 
 // Point data of one stage: thread t holds point t % RP.  Lanes past the slice
 // end read the slice's last point and are marked invalid (zero amplitude).
@@ -102,18 +109,26 @@ struct Raw {
     unsigned mword;
     bool valid;
 };
+// (pi, p1 relative to the launch's first record; the tile mask holds at most
+// 2^28 words, so jr * words fits 32 bits.)
 template <bool MASK>
-__device__ inline void load_raw(Raw& r, const PointSoA& s, const TileMask& tm, int64_t pi, int64_t p1)
+__device__ inline void load_raw(Raw& r, Rsrc rs, const TileMask& tm, int pi, int p1)
 {
     r.valid = pi < p1;
-    const int64_t j = r.valid ? pi : p1 - 1;
-    if constexpr (MASK) r.mword = tm.bits ? tm.bits[(j - tm.base) * tm.words + (tm.tile >> 5)] : ~0u;
-    r.px = s.px[j];
-    r.py = s.py[j];
-    r.pz = s.pz[j];
-    r.cyc0 = s.cyc0[j];
-    r.kamp = s.kamp[j];
-    if constexpr (MASK) r.box = s.box[j];
+    const int jr = min(pi, p1 - 1);
+    if constexpr (MASK) r.mword = tm.bits ? tm.bits[jr * tm.words + (tm.tile >> 5)] : ~0u;
+    const int off = jr * (int)sizeof(RsRec);
+    const u32x4 a = rec_load16(rs, off), b = rec_load16(rs, off + 16);
+    const uint2 c = __builtin_bit_cast(uint2, __builtin_amdgcn_raw_buffer_load_b64(rs, off + 32, 0, 0));
+    r.px = __builtin_bit_cast(double, make_uint2(a.x, a.y));
+    r.py = __builtin_bit_cast(double, make_uint2(a.z, a.w));
+    r.pz = __builtin_bit_cast(double, make_uint2(b.x, b.y));
+    r.cyc0 = __builtin_bit_cast(double, make_uint2(b.z, b.w));
+    r.kamp = __builtin_bit_cast(double, c);
+    if constexpr (MASK) {
+        const u32x4 d = rec_load16(rs, off + 48);
+        r.box = make_int4((int)d.x, (int)d.y, (int)d.z, (int)d.w);
+    }
 }
 
 // Per-(point, tile) constants of a stage
@@ -311,9 +326,9 @@ template <int VAR>
 struct StageCtx {
     f32x16 (&acc)[4][2];
     Raw& R;
-    const PointSoA& s;
+    Rsrc rs;
     const Frame& f;
-    int64_t pnext, ke;
+    int pnext, ke;
     int p, grp, lrow, koff, ucol;
     Pt P;
     Tmp t;
@@ -332,7 +347,7 @@ __device__ inline void slot(const unsigned* __restrict__ rv, const unsigned* __r
         if constexpr (THREE && S < NPR) prep_piece<S, MASK>(c.P, c.t, c.R, c.f);
         if constexpr (!THREE && 2 * S < NPR) prep_piece<2 * S, MASK>(c.P, c.t, c.R, c.f);
         if constexpr (!THREE && 2 * S + 1 < NPR) prep_piece<2 * S + 1, MASK>(c.P, c.t, c.R, c.f);
-        if constexpr (S == (THREE ? NPR : (NPR + 1) / 2)) load_raw<MASK>(c.R, c.s, c.f.tm, c.pnext, c.ke);
+        if constexpr (S == (THREE ? NPR : (NPR + 1) / 2)) load_raw<MASK>(c.R, c.rs, c.f.tm, c.pnext, c.ke);
         if constexpr (THREE) {
             constexpr int h = half_at(S);
             if constexpr (h >= 0 && (h & 1) == 0) c.cs[h >> 1] = unit_value<(h >> 1), MASK, false>(c.P, c.f, c.grp);
@@ -371,28 +386,27 @@ __device__ inline void prep_all(Pt& P, const Raw& r, const Frame& f)
 // stage the generation runs on clamped, zero-amplitude data into the unused
 // buffer, which keeps the step branch-free.
 template <int VAR, bool MASK, int RB>
-__device__ inline void step(Smem<Traits<VAR>::three>& sm, f32x16 (&acc)[4][2], Raw& R, const PointSoA& s,
-                            const Frame& f, int64_t pnext, int64_t ke, int p, int grp, int lrow, int koff, int ucol)
+__device__ inline void step(Smem<Traits<VAR>::three>& sm, f32x16 (&acc)[4][2], Raw& R, Rsrc rs, const Frame& f,
+                            int pnext, int ke, int p, int grp, int lrow, int koff, int ucol)
 {
     const unsigned* __restrict__ rv = &sm.v[RB][0][0][0];
     const unsigned* __restrict__ ru = &sm.u[RB][0][0][0];
-    StageCtx<VAR> c{acc, R, s, f, pnext, ke, p, grp, lrow, koff, ucol};
+    StageCtx<VAR> c{acc, R, rs, f, pnext, ke, p, grp, lrow, koff, ucol};
     stage_operands<VAR>(rv, ru, c);
     slot<VAR, MASK, 0>(rv, ru, &sm.v[RB ^ 1][0][0][0], &sm.u[RB ^ 1][0][0][0], c);
     __syncthreads();
 }
 
-// S = sum_p U V^T over points kfirst .. klast-1 of s (all matrix-core
-// eligible), split-K: workgroup (tile, split) sums kslice points into plane
-// `split` of part.  MASK: the points' boxes may be smaller than the
+// S = sum_p U V^T over the np records at rec (all matrix-core eligible, at
+// most MAX_REC), split-K: workgroup (tile, split) sums kslice points into
+// plane `split` of part.  MASK: the points' boxes may be smaller than the
 // hologram, out-of-box rows and columns get zero amplitude, and so do the
-// tiles whose bit is clear in the point's row of mask (rows from point
-// mask_base on, mask_words words each; null mask: none).
+// tiles whose bit is clear in the point's row of mask (one row of
+// mask_words words per record; null mask: none).
 template <int VAR, bool MASK>
-__global__ __launch_bounds__(NT, 2) void k_rs_gemm(Geometry g, PointSoA s, float2* __restrict__ part, int tiles_x,
-                                                   int tiles, int64_t kfirst, int64_t klast, int64_t kslice,
-                                                   const unsigned* __restrict__ mask, int64_t mask_base,
-                                                   int mask_words)
+__global__ __launch_bounds__(NT, 2) void k_rs_gemm(Geometry g, const RsRec* __restrict__ rec,
+                                                   float2* __restrict__ part, int tiles_x, int tiles, int np,
+                                                   int kslice, const unsigned* __restrict__ mask, int mask_words)
 {
     constexpr bool THREE = Traits<VAR>::three;
     __shared__ __align__(16) Smem<THREE> sm;
@@ -400,10 +414,11 @@ __global__ __launch_bounds__(NT, 2) void k_rs_gemm(Geometry g, PointSoA s, float
     const int tile = blockIdx.x % tiles;
     const int split = blockIdx.x / tiles;
     const int x0 = (tile % tiles_x) * BN, y0 = (tile / tiles_x) * BM;
-    const int64_t kb = kfirst + (int64_t)split * kslice;
-    const int64_t ke = min(klast, kb + kslice);
-    const int nst = (int)((ke - kb + RP - 1) / RP);
+    const int kb = split * kslice;
+    const int ke = min(np, kb + kslice);
+    const int nst = (ke - kb + RP - 1) / RP;
     const int p = threadIdx.x & (RP - 1), grp = threadIdx.x / RP;
+    const Rsrc rs = rec_rsrc(rec, np * (int)sizeof(RsRec));
     Frame f;
     f.yc = gpix_y(g, y0 + BM / 2);
     f.xc = gpix_x(g, x0 + BN / 2);
@@ -419,7 +434,7 @@ __global__ __launch_bounds__(NT, 2) void k_rs_gemm(Geometry g, PointSoA s, float
     f.hl = 0.5 * g.inv_lambda;
     f.um = (BN / 2) * g.pp_x;
     f.vm = (BM / 2) * g.pp_y;
-    f.tm = TileMask{mask, mask_base, mask_words, rs_tile_index(g, x0, y0)};
+    f.tm = TileMask{mask, mask_words, rs_tile_index(g, x0, y0)};
     f.x0 = x0;
     f.y0 = y0;
 
@@ -429,20 +444,20 @@ __global__ __launch_bounds__(NT, 2) void k_rs_gemm(Geometry g, PointSoA s, float
 
     Raw R;
     if (nst > 0) {
-        load_raw<MASK>(R, s, f.tm, kb + p, ke);
+        load_raw<MASK>(R, rs, f.tm, kb + p, ke);
         Pt P;
         prep_all<MASK>(P, R, f);
         gen_units<VAR, MASK, 0>(&sm.v[0][0][0][0], &sm.u[0][0][0][0], P, f, p, grp);
-        load_raw<MASK>(R, s, f.tm, kb + RP + p, ke);
+        load_raw<MASK>(R, rs, f.tm, kb + RP + p, ke);
     }
     __syncthreads();
     f32x16 acc[4][2];
 #pragma unroll
     for (int i = 0; i < 4; ++i) acc[i][0] = acc[i][1] = f32x16{};
     for (int st = 0; st < nst; st += 2) {
-        step<VAR, MASK, 0>(sm, acc, R, s, f, kb + (int64_t)(st + 2) * RP + p, ke, p, grp, lrow, koff, ucol);
+        step<VAR, MASK, 0>(sm, acc, R, rs, f, kb + (st + 2) * RP + p, ke, p, grp, lrow, koff, ucol);
         if (st + 1 >= nst) break;
-        step<VAR, MASK, 1>(sm, acc, R, s, f, kb + (int64_t)(st + 3) * RP + p, ke, p, grp, lrow, koff, ucol);
+        step<VAR, MASK, 1>(sm, acc, R, rs, f, kb + (st + 3) * RP + p, ke, p, grp, lrow, koff, ucol);
     }
     store_partials(g, acc, part, split, x0, y0, ucol, lane);
 }
@@ -531,10 +546,11 @@ __global__ __launch_bounds__(256) void k_rs_tilemask(Geometry g, PointSoA s, dou
 // Stable three-way partition: unmasked matrix-core points to o[0, n0), masked
 // ones to o[n0, n0 + n1), the others after them, each in point order; pos =
 // exclusive prefix sum of key (low word: n0 before i, high word: n1 before i).
+// The matrix-core kernels' records go to rec in the same order.
 // count[0] = n0 + n1, count[1] = n0.
 __global__ __launch_bounds__(256) void k_rs_gather(PointSoA s, const unsigned long long* __restrict__ key,
                                                    const unsigned long long* __restrict__ pos, PointSoA o,
-                                                   unsigned long long* __restrict__ count)
+                                                   RsRec* __restrict__ rec, unsigned long long* __restrict__ count)
 {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     const int64_t n = s.n;
@@ -551,6 +567,7 @@ __global__ __launch_bounds__(256) void k_rs_gather(PointSoA s, const unsigned lo
     o.kamp[j] = s.kamp[i];
     o.q[j] = s.q[i];
     o.box[j] = s.box[i];
+    rec[j] = RsRec{s.px[i], s.py[i], s.pz[i], s.cyc0[i], s.kamp[i], 0.0, s.box[i]};
     if (i == 0) {
         count[0] = (unsigned long long)(n0 + n1);
         count[1] = (unsigned long long)n0;
@@ -588,12 +605,20 @@ constexpr size_t MAX_RS_PART = size_t(4) << 30;
 constexpr size_t MAX_RS_MASK = size_t(1) << 30;
 }  // namespace
 
-// [count words, 256 B] [key] [pos] [scan temp]; beyond INT_MAX points (the
-// scan's item count) a size no budget admits
+// [count words, 256 B] [key] [pos] [scan temp] [records]; beyond INT_MAX
+// points (the scan's item count) a size no budget admits
+static size_t records_offset(int64_t n)
+{
+    return 256 + 2 * al256((size_t)n * sizeof(unsigned long long)) + al256(scan_temp_bytes(n));
+}
 size_t rs_partition_scratch_bytes(int64_t n)
 {
     if (n > INT_MAX) return SIZE_MAX / 4;
-    return 256 + 2 * al256((size_t)n * sizeof(unsigned long long)) + al256(scan_temp_bytes(n)) + 256;
+    return records_offset(n) + al256((size_t)n * sizeof(rsgemm::RsRec)) + 256;
+}
+const void* rs_partition_records(const void* scratch, int64_t n)
+{
+    return (const char*)scratch + records_offset(n);
 }
 
 static double max_theta_of(uint32_t mode)
@@ -620,7 +645,7 @@ int launch_rs_partition(hipStream_t st, void* scratch, size_t scratch_bytes, con
         hipSuccess)
         return OHX_ERR_DEVICE;
     hipLaunchKernelGGL(k_rs_gather, dim3(pgrid), dim3(256), 0, st, soa, (const unsigned long long*)key,
-                       (const unsigned long long*)pos, part, *d_count);
+                       (const unsigned long long*)pos, part, (RsRec*)(b + records_offset(n)), *d_count);
     return hipGetLastError() == hipSuccess ? OHX_OK : OHX_ERR_DEVICE;
 }
 
@@ -632,44 +657,58 @@ size_t rs_gemm_scratch_bytes(const Geometry& g, int64_t npoints)
 }
 
 namespace {
-int rs_gemm_launch(hipStream_t st, void* scratch, size_t scratch_bytes, const Geometry& g, const PointSoA& part,
-                   int64_t p0, int64_t p1, bool masked, double* d_field, uint32_t mode, int64_t* wgs,
-                   const unsigned* mask, int64_t mask_base, int mask_words)
+// One GEMM + reduce over records [p0, p1) (fewer than MAX_REC); mask: the
+// tile-mask row of record mask_base.
+int rs_gemm_piece(hipStream_t st, void* scratch, size_t scratch_bytes, const Geometry& g, const rsgemm::RsRec* rec,
+                  int64_t p0, int64_t p1, bool masked, double* d_field, uint32_t mode, int64_t* wgs,
+                  const unsigned* mask, int64_t mask_base, int mask_words)
 {
     using namespace rsgemm;
-    if (p1 <= p0) return OHX_OK;
     const GemmPlan pl = plan_gemm(g, p1 - p0, RP);
     if ((size_t)pl.splits * pl.npx * sizeof(float2) > scratch_bytes) return OHX_ERR_INVALID_ARG;
     const dim3 grid((unsigned)((int64_t)pl.tiles * pl.splits)), block(NT);
     float2* pp = (float2*)scratch;
     const bool x3 = (mode & OHX_MODE_MFMA_BF16X3) != 0;
-    const unsigned* mk = masked ? mask : nullptr;
+    const unsigned* mk = (masked && mask) ? mask + (p0 - mask_base) * mask_words : nullptr;
+    const RsRec* r = rec + p0;
+    const int np = (int)(p1 - p0), ks = (int)pl.kslice;
     if (x3 && masked)
-        hipLaunchKernelGGL((k_rs_gemm<V_BF16X3, true>), grid, block, 0, st, g, part, pp, pl.tiles_x, pl.tiles, p0, p1,
-                           pl.kslice, mk, mask_base, mask_words);
+        hipLaunchKernelGGL((k_rs_gemm<V_BF16X3, true>), grid, block, 0, st, g, r, pp, pl.tiles_x, pl.tiles, np, ks,
+                           mk, mask_words);
     else if (x3)
-        hipLaunchKernelGGL((k_rs_gemm<V_BF16X3, false>), grid, block, 0, st, g, part, pp, pl.tiles_x, pl.tiles, p0,
-                           p1, pl.kslice, mk, mask_base, mask_words);
+        hipLaunchKernelGGL((k_rs_gemm<V_BF16X3, false>), grid, block, 0, st, g, r, pp, pl.tiles_x, pl.tiles, np, ks,
+                           mk, mask_words);
     else if (masked)
-        hipLaunchKernelGGL((k_rs_gemm<V_BF16X1, true>), grid, block, 0, st, g, part, pp, pl.tiles_x, pl.tiles, p0, p1,
-                           pl.kslice, mk, mask_base, mask_words);
+        hipLaunchKernelGGL((k_rs_gemm<V_BF16X1, true>), grid, block, 0, st, g, r, pp, pl.tiles_x, pl.tiles, np, ks,
+                           mk, mask_words);
     else
-        hipLaunchKernelGGL((k_rs_gemm<V_BF16X1, false>), grid, block, 0, st, g, part, pp, pl.tiles_x, pl.tiles, p0,
-                           p1, pl.kslice, mk, mask_base, mask_words);
+        hipLaunchKernelGGL((k_rs_gemm<V_BF16X1, false>), grid, block, 0, st, g, r, pp, pl.tiles_x, pl.tiles, np, ks,
+                           mk, mask_words);
     if (hipGetLastError() != hipSuccess) return OHX_ERR_DEVICE;
     if (wgs) *wgs += (int64_t)pl.tiles * pl.splits;
     return launch_gemm_reduce(st, pp, pl.splits, pl.npx, d_field);
 }
+int rs_gemm_launch(hipStream_t st, void* scratch, size_t scratch_bytes, const Geometry& g, const void* rec,
+                   int64_t p0, int64_t p1, bool masked, double* d_field, uint32_t mode, int64_t* wgs,
+                   const unsigned* mask, int64_t mask_base, int mask_words)
+{
+    int e = OHX_OK;
+    for (int64_t q0 = p0; q0 < p1 && !e; q0 += gemm::MAX_REC)
+        e = rs_gemm_piece(st, scratch, scratch_bytes, g, (const rsgemm::RsRec*)rec, q0,
+                          std::min(p1, q0 + gemm::MAX_REC), masked, d_field, mode, wgs, mask, mask_base,
+                          mask_words);
+    return e;
+}
 }  // namespace
 
-int launch_rs_gemm_range(hipStream_t st, void* scratch, size_t scratch_bytes, const Geometry& g,
-                         const PointSoA& part, int64_t p0, int64_t p1, int64_t n_full, double* d_field, uint32_t mode,
-                         int64_t* wgs, const unsigned* mask, int mask_words)
+int launch_rs_gemm_range(hipStream_t st, void* scratch, size_t scratch_bytes, const Geometry& g, const void* rec,
+                         int64_t p0, int64_t p1, int64_t n_full, double* d_field, uint32_t mode, int64_t* wgs,
+                         const unsigned* mask, int mask_words)
 {
     const int64_t m = std::min(std::max(n_full, p0), p1);
-    int e = rs_gemm_launch(st, scratch, scratch_bytes, g, part, p0, m, false, d_field, mode, wgs, nullptr, 0, 0);
+    int e = rs_gemm_launch(st, scratch, scratch_bytes, g, rec, p0, m, false, d_field, mode, wgs, nullptr, 0, 0);
     if (!e)
-        e = rs_gemm_launch(st, scratch, scratch_bytes, g, part, m, p1, true, d_field, mode, wgs, mask, n_full,
+        e = rs_gemm_launch(st, scratch, scratch_bytes, g, rec, m, p1, true, d_field, mode, wgs, mask, n_full,
                            mask_words);
     return e;
 }
@@ -716,7 +755,8 @@ int launch_rs_gemm(ohx_ctx* ctx, const Geometry& g, const PointSoA& soa, double*
     unsigned* mask = nullptr;
     if (ne > 0) {
         int64_t chunk = ne;
-        while (chunk > RP && rs_gemm_scratch_bytes(g, chunk) > MAX_RS_PART) chunk = (chunk + 1) / 2;
+        while (chunk > RP && (rs_gemm_scratch_bytes(g, chunk) > MAX_RS_PART || chunk > gemm::MAX_REC))
+            chunk = (chunk + 1) / 2;
         if ((e = ensure_gemm_scratch(ctx, rs_gemm_scratch_bytes(g, chunk)))) return e;
         if (nm > 0) {
             const size_t mbytes = (size_t)nm * words * sizeof(unsigned);
@@ -740,8 +780,9 @@ int launch_rs_gemm(ohx_ctx* ctx, const Geometry& g, const PointSoA& soa, double*
         }
         for (int64_t p0 = 0; p0 < ne; p0 += chunk) {
             const int64_t p1 = std::min<int64_t>(ne, p0 + chunk);
-            e = launch_rs_gemm_range(st, ctx->gemm_scratch, ctx->gemm_scratch_bytes, g, part, p0, p1, n_full,
-                                     d_field, mode, &wgs, mask, words);
+            e = launch_rs_gemm_range(st, ctx->gemm_scratch, ctx->gemm_scratch_bytes, g,
+                                     rs_partition_records(ctx->rs_scratch, n), p0, p1, n_full, d_field, mode, &wgs,
+                                     mask, words);
             if (e) return set_error(ctx, e, "RS matrix-core launch failed");
         }
     }

@@ -485,8 +485,8 @@ static bool enqueue_iteration(ohx_server* sv, const std::vector<SJob*>& act, int
         const int64_t p0 = j->next, p1 = slice_end(*j);
         if (!st)
             st = j->mfma ? launch_fresnel_gemm_range(s, j->gemm, j->gemm_bytes, j->g, j->soa, p0, p1, j->field, j->mode)
-                         : launch_rs_gemm_range(s, j->gemm, j->gemm_bytes, j->g, j->part, p0, p1, j->n_full, j->field,
-                                                j->mode);
+                         : launch_rs_gemm_range(s, j->gemm, j->gemm_bytes, j->g, rs_partition_records(j->rs_tmp, j->n),
+                                                p0, p1, j->n_full, j->field, j->mode);
         if (!st && p1 >= j->n) {
             hipEvent_t e = get_event(sv);
             if (!e || hipEventRecord(e, s) != hipSuccess) st = OHX_ERR_DEVICE;

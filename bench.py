@@ -150,12 +150,12 @@ def cpu_baseline(w, target_s: float):
 # FETCH_SIZE and WRITE_SIZE summed over the launch's dispatches, each scaled by
 # the factor the same counter reads on a copy kernel of known size;
 # profiles/traffic_1m_2160p.md gives the counts, the factors and the commit.
-# k_fresnel_gemm<BF16x3> alone (128 x 256 tiles): 661.5 MB read (per-stage
-# point records re-read per tile, mostly L2 hits) + 1.062 GB written (the
-# split-K planes); the whole launch (prep, constants, GEMM, split-K reduce):
-# 3.25 GB.
-TRAFFIC = {"bf16x3": 1.72e9}
-TRAFFIC_LAUNCH = {"bf16x3": 3.25e9}
+# k_fresnel_gemm<BF16x3> alone (128 x 256 tiles, 64-byte point records):
+# 883.8 MB read (the per-stage point records re-read per tile, mostly L2
+# hits) + 1.062 GB written (the split-K planes); the whole launch (prep,
+# records, GEMM, split-K reduce): 3.59 GB.
+TRAFFIC = {"bf16x3": 1.95e9}
+TRAFFIC_LAUNCH = {"bf16x3": 3.59e9}
 # Speed of light of the direct kernels' separable pair loops (per lane and point,
 # 8 rows: column factor + 8 pairs) on this
 # hardware (openholo_amd/bin/valu_rates); profiles/valu_mfma_rates_gfx950.md.

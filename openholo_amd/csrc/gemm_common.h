@@ -199,6 +199,14 @@ __device__ inline void mfma_slot(const unsigned* rv, const unsigned* ru, C& c)
     if constexpr (THREE && S == NM * 3 / 8 + 2) xform_b(c, 1, 1);
 }
 
+// Sign of split s's accumulation: the matrix cores' FP32 accumulation rounds
+// with a bias toward -inf (measured: the error of long sums has a constant
+// negative part in both components, the same in every split), so odd splits
+// accumulate the negated products (their amplitudes negated), the split-K
+// reduction subtracts their planes, and the bias of each pair of splits
+// cancels.
+__device__ inline float split_sign(int split) { return (split & 1) ? -1.f : 1.f; }
+
 // FP32 partial sums of one split, interleaved (Re, Im) per pixel, into plane
 // `split` of part (pn_y x pn_x float2 per plane).
 __device__ inline void store_partials(const Geometry& g, const f32x16 (&acc)[4][2], float2* __restrict__ part,

@@ -28,7 +28,9 @@
 //   BF16x3 : hi hi + hi lo + lo hi   (lo = bf16(x - hi): ~2^-17 per factor)
 //   FP16x3 : same split in FP16 (~2^-22), V scaled by 1/max|A| to stay in range
 // Split-K slices write FP32 partials; k_gemm_reduce sums them in FP64 in a fixed
-// order (deterministic) and adds -j S into the FP64 field.
+// order (deterministic) and adds -j S into the FP64 field.  Odd splits
+// accumulate the negated products and are subtracted, which cancels the
+// matrix cores' accumulation bias between pairs of splits (split_sign).
 #include <algorithm>
 
 #include "gemm_common.h"
@@ -106,12 +108,12 @@ __device__ inline void prep_x(Pt& P, const Raw& r, double xthr, double dx)
     P.u1 = (float)frac64(tx * (2.0 * dx));
     P.u2 = r.c.u2;
 }
-// amplitude and box; returns true when the box does not cover the whole
-// BM x W tile
+// amplitude (times the split's sign sg) and box; returns true when the box
+// does not cover the whole BM x W tile
 template <int W>
-__device__ inline bool prep_rest(Pt& P, const Raw& r, int x0, int y0)
+__device__ inline bool prep_rest(Pt& P, const Raw& r, int x0, int y0, float sg)
 {
-    P.amp = r.valid ? r.c.amp : 0.f;
+    P.amp = r.valid ? sg * r.c.amp : 0.f;
     P.rlo = r.valid ? r.c.box.z - y0 : 0;
     P.rhi = r.valid ? r.c.box.w - y0 : BM;
     P.clo = r.valid ? r.c.box.x - x0 : 0;
@@ -119,11 +121,12 @@ __device__ inline bool prep_rest(Pt& P, const Raw& r, int x0, int y0)
     return P.rlo > 0 || P.rhi < BM || P.clo > 0 || P.chi < W;
 }
 template <int W>
-__device__ inline bool prep(Pt& P, const Raw& r, double ythr, double xthr, double dy, double dx, int x0, int y0)
+__device__ inline bool prep(Pt& P, const Raw& r, double ythr, double xthr, double dy, double dx, int x0, int y0,
+                            float sg)
 {
     prep_y(P, r, ythr, dy);
     prep_x(P, r, xthr, dx);
-    return prep_rest<W>(P, r, x0, y0);
+    return prep_rest<W>(P, r, x0, y0, sg);
 }
 
 // The same work in NPIECE pieces of two or three FP64 operations, spread by the
@@ -265,6 +268,7 @@ struct StageCtx {
     double ythr, xthr, dy, dx;
     int x0, y0;
     int p, grp, lrow, koff, ucol;
+    float sg;      // split_sign of the workgroup's split
     Pt P;
     double t[4];   // FP64 temporaries shared by the prep pieces
     bool partial;
@@ -336,7 +340,7 @@ __device__ inline void slot(const unsigned* __restrict__ rv, const unsigned* __r
             prep_piece<2 * S + 1>(c.P, c.t, c.R, c.ythr, c.xthr, c.dy, c.dx);
         }
         if constexpr (S == (THREE ? NPIECE : NPIECE / 2)) {
-            c.partial = prep_rest<W>(c.P, c.R, c.x0, c.y0);
+            c.partial = prep_rest<W>(c.P, c.R, c.x0, c.y0, c.sg);
             load_raw(c.R, c.rs, c.pnext, c.ke);
         }
         if constexpr (THREE) {
@@ -365,9 +369,9 @@ template <int VAR, int W>
 __device__ inline bool stage_body(const unsigned* __restrict__ rv, const unsigned* __restrict__ ru,
                                   unsigned* __restrict__ wv, unsigned* __restrict__ wu, f32x16 (&acc)[4][2],
                                   Pt& P, Raw& R, Rsrc rs, int pnext, int ke, double ythr, double xthr, double dy,
-                                  double dx, int x0, int y0, int p, int grp, int lrow, int koff, int ucol)
+                                  double dx, int x0, int y0, int p, int grp, int lrow, int koff, int ucol, float sg)
 {
-    StageCtx<VAR, W> c{acc, R, rs, pnext, ke, ythr, xthr, dy, dx, x0, y0, p, grp, lrow, koff, ucol};
+    StageCtx<VAR, W> c{acc, R, rs, pnext, ke, ythr, xthr, dy, dx, x0, y0, p, grp, lrow, koff, ucol, sg};
     c.partial = false;
     stage_operands<VAR, Shape<W>::PLANE_U>(rv, ru, c);
     slot<VAR, W, 0>(rv, ru, wv, wu, c);
@@ -386,12 +390,12 @@ __device__ inline bool stage_body(const unsigned* __restrict__ rv, const unsigne
 template <int VAR, int W, int RB, bool MASK>
 __device__ inline void step(Smem<Traits<VAR>::three, W>& sm, f32x16 (&acc)[4][2], Raw& R, Rsrc rs, int st,
                             int kb, int ke, double ythr, double xthr, double dy, double dx, int x0, int y0, int p,
-                            int grp, int lrow, int koff, int ucolx)
+                            int grp, int lrow, int koff, int ucolx, float sg)
 {
     Pt P;
     const bool partial = stage_body<VAR, W>(&sm.v[RB][0][0][0], &sm.u[RB][0][0][0], &sm.v[RB ^ 1][0][0][0],
                                          &sm.u[RB ^ 1][0][0][0], acc, P, R, rs, kb + (st + 2) * KP + p, ke, ythr,
-                                         xthr, dy, dx, x0, y0, p, grp, lrow, koff, ucolx);
+                                         xthr, dy, dx, x0, y0, p, grp, lrow, koff, ucolx, sg);
     if constexpr (MASK) {
         if (__builtin_amdgcn_ballot_w64(partial)) mask_stage<VAR, W, RB ^ 1>(sm, P, p, grp);
     } else {
@@ -418,6 +422,7 @@ __device__ __attribute__((always_inline)) inline void fresnel_tile(Smem<Traits<V
     const int ke = min(np, (int)(kb + kslice));
     const int nst = (ke - kb + KP - 1) / KP;
     const Rsrc rs = rec_rsrc(pconst, np * (int)sizeof(PointRec));
+    const float sg = split_sign(split);
     const int p = threadIdx.x & (KP - 1), grp = threadIdx.x >> 4;
     const double ythr = gpix_y(g, y0 + grp + Sh::NGRP * (Sh::NVT / 2));
     const double xthr = gpix_x(g, x0 + grp + Sh::NGRP * (Sh::NUT / 2));
@@ -433,7 +438,7 @@ __device__ __attribute__((always_inline)) inline void fresnel_tile(Smem<Traits<V
     if (nst > 0) {
         load_raw(R, rs, kb + p, ke);
         Pt P;
-        const bool partial = prep<W>(P, R, ythr, xthr, dy, dx, x0, y0);
+        const bool partial = prep<W>(P, R, ythr, xthr, dy, dx, x0, y0, sg);
         gen_stage<VAR, W, 0>(sm, P, p, grp);
         if constexpr (MASK) {
             if (partial) mask_stage<VAR, W, 0>(sm, P, p, grp);
@@ -447,10 +452,11 @@ __device__ __attribute__((always_inline)) inline void fresnel_tile(Smem<Traits<V
 #pragma unroll
     for (int i = 0; i < 4; ++i) acc[i][0] = acc[i][1] = f32x16{};
     for (int st = 0; st < nst; st += 2) {
-        step<VAR, W, 0, MASK>(sm, acc, R, rs, st, kb, ke, ythr, xthr, dy, dx, x0, y0, p, grp, lrow, koff, ucolx);
+        step<VAR, W, 0, MASK>(sm, acc, R, rs, st, kb, ke, ythr, xthr, dy, dx, x0, y0, p, grp, lrow, koff, ucolx,
+                              sg);
         if (st + 1 >= nst) break;
         step<VAR, W, 1, MASK>(sm, acc, R, rs, st + 1, kb, ke, ythr, xthr, dy, dx, x0, y0, p, grp, lrow, koff,
-                              ucolx);
+                              ucolx, sg);
     }
 
     store_partials(g, acc, part, split, x0, y0, ucolx, lane);
@@ -472,7 +478,8 @@ __global__ __launch_bounds__(Shape<W>::NT, 512 / Shape<W>::NT) void k_fresnel_ge
     fresnel_tile<VAR, W, MASK>(sm, g, pconst, part, tiles_x, tiles, kfirst, klast, kslice);
 }
 
-// field += -j * amax * sum_split S   (FP64, fixed summation order)
+// field += -j * amax * sum_split split_sign(split) S_split   (FP64, fixed
+// summation order; odd splits accumulated the negated products)
 __global__ __launch_bounds__(256) void k_gemm_reduce(const float2* __restrict__ part, int splits, int64_t npx,
                                                      double2* __restrict__ field, const float* __restrict__ amax,
                                                      int scaled)
@@ -484,8 +491,9 @@ __global__ __launch_bounds__(256) void k_gemm_reduce(const float2* __restrict__ 
         double sr = 0, si = 0;
         for (int k = 0; k < splits; ++k) {
             const float2 v = part[(int64_t)k * npx + i];
-            sr += v.x;
-            si += v.y;
+            const double sg = split_sign(k);   // odd splits hold the negated sums
+            sr = fma(sg, (double)v.x, sr);
+            si = fma(sg, (double)v.y, si);
         }
         double2 f = field[i];
         f.x += si * sc;

@@ -153,6 +153,7 @@ struct Frame {
     double um, vm;            // 64 pitches: the largest |u|, |v| on the tile
     TileMask tm;              // masked instance: the tiles it takes per point
     int x0, y0;               // tile origin
+    float sg;                 // split_sign of the workgroup's split
 };
 
 // The constants in NPR pieces of a few dependent FP64 operations each, spread
@@ -183,7 +184,7 @@ __device__ inline void prep_piece(Pt& P, Tmp& t, const Raw& r, const Frame& f)
         // direct kernel
         bool keep = r.valid;
         if constexpr (MASK) keep = keep && ((r.mword >> (f.tm.tile & 31)) & 1u);
-        P.ac = keep ? (float)(r.kamp * t.irc2) : 0.f;
+        P.ac = keep ? f.sg * (float)(r.kamp * t.irc2) : 0.f;
         P.acmu = P.ac * P.mu;
         P.g = (float)(FOUR_PI * P.k);
         P.dy2 = 2.0 * t.dyc;
@@ -437,6 +438,7 @@ __global__ __launch_bounds__(NT, 2) void k_rs_gemm(Geometry g, const RsRec* __re
     f.tm = TileMask{mask, mask_words, rs_tile_index(g, x0, y0)};
     f.x0 = x0;
     f.y0 = y0;
+    f.sg = split_sign(split);
 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int lrow = lane & 31, koff = (lane >> 5) * 4;

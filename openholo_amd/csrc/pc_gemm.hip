@@ -284,55 +284,73 @@ struct StageCtx {
 // NPIECE, half-elements from slot NPIECE + 1 to 46.  1-term: prep pieces two
 // per slot (slots 0 .. NPIECE/2 - 1), amplitude/box at NPIECE/2, elements from
 // NPIECE/2 + 1 to 15.
+//
+// LATE (the unmasked instances): the stage's barrier comes before its last
+// MFMA group (TAIL MFMAs), which then runs on operands already in registers
+// while the next stage's first operands are read; the generation ends
+// before that group.
 constexpr int H0 = NPIECE + 1, E0 = NPIECE / 2 + 1;
-template <int W>
+template <int VAR>
+constexpr int tail_mfmas()
+{
+    return Traits<VAR>::three ? 6 : 2;   // the last (ks, mb) group
+}
+template <int W, bool LATE>
 constexpr int half_at(int S)
 {
-    constexpr int NE = Shape<W>::NE;
+    constexpr int NE = Shape<W>::NE, END = LATE ? 48 - 6 : 46;
     for (int h = 0; h < 2 * NE; ++h)
-        if (H0 + ((46 - H0) * h) / (2 * NE) == S) return h;
+        if (H0 + ((END - H0) * h) / (2 * NE) == S) return h;
     return -1;
 }
-template <int W>
+template <int W, bool LATE>
 constexpr int elem_at(int S, int n)   // n-th element at slot S (slots hold at most two)
 {
-    constexpr int NE = Shape<W>::NE;
+    constexpr int NE = Shape<W>::NE, END = LATE ? 16 - 2 : 16;
     for (int e = 0; e < NE; ++e)
-        if (E0 + ((16 - E0) * e) / NE == S && n-- == 0) return e;
+        if (E0 + ((END - E0) * e) / NE == S && n-- == 0) return e;
     return -1;
 }
-// every element (1-term) and half-element (3-term) lands in exactly one slot
-template <int W>
+// every element (1-term) and half-element (3-term) lands in exactly one slot,
+// before the tail group when LATE
+template <int W, bool LATE>
 constexpr bool schedule_complete()
 {
     constexpr int NE = Shape<W>::NE;
     for (int e = 0; e < NE; ++e) {
         int n = 0;
         for (int S = 0; S < 16; ++S)
-            n += (elem_at<W>(S, 0) == e) + (elem_at<W>(S, 1) == e) + (elem_at<W>(S, 2) == e);
+            n += (elem_at<W, LATE>(S, 0) == e) + (elem_at<W, LATE>(S, 1) == e) + (elem_at<W, LATE>(S, 2) == e);
         if (n != 1) return false;
     }
     for (int S = 0; S < 16; ++S)
-        if (elem_at<W>(S, 2) >= 0) return false;
+        if (elem_at<W, LATE>(S, 2) >= 0 || (LATE && S >= 16 - 2 && elem_at<W, LATE>(S, 0) >= 0)) return false;
     for (int h = 0; h < 2 * NE; ++h) {
         int n = 0;
-        for (int S = 0; S < 48; ++S) n += half_at<W>(S) == h;
+        for (int S = 0; S < 48; ++S) n += half_at<W, LATE>(S) == h && !(LATE && S >= 48 - 6);
         if (n != 1) return false;
     }
     return true;
 }
-static_assert(schedule_complete<128>() && schedule_complete<256>(),
+static_assert(schedule_complete<128, false>() && schedule_complete<256, false>() &&
+                  schedule_complete<128, true>() && schedule_complete<256, true>(),
               "generation schedule must cover every element once");
+
+// The first operands of a stage, read by the previous stage right after its
+// barrier (LATE pipeline); the fields load_a / load_b use.
+struct Carry {
+    u32x4 A[8][2], Ub[2][2];
+    int lrow, koff, ucol;
+};
 
 // Slot S of the pipelined stage: MFMA S, operand loads / transforms due at
 // this slot, and this slot's share of the next stage's generation.
-template <int VAR, int W, int S>
+template <int VAR, int W, bool LATE, int S, int END>
 __device__ inline void slot(const unsigned* __restrict__ rv, const unsigned* __restrict__ ru,
                             unsigned* __restrict__ wv, unsigned* __restrict__ wu, StageCtx<VAR, W>& c)
 {
     constexpr bool THREE = Traits<VAR>::three;
-    constexpr int NM = mfma_per_stage<VAR>();
-    if constexpr (S < NM) {
+    if constexpr (S < END) {
         mfma_slot<VAR, S, Shape<W>::PLANE_U>(rv, ru, c);
         if constexpr (THREE && S < NPIECE) prep_piece<S>(c.P, c.t, c.R, c.ythr, c.xthr, c.dy, c.dx);
         if constexpr (!THREE && S < NPIECE / 2) {
@@ -344,16 +362,16 @@ __device__ inline void slot(const unsigned* __restrict__ rv, const unsigned* __r
             load_raw(c.R, c.rs, c.pnext, c.ke);
         }
         if constexpr (THREE) {
-            constexpr int h = half_at<W>(S);
+            constexpr int h = half_at<W, LATE>(S);
             if constexpr (h >= 0 && (h & 1) == 0) c.cs[h >> 1] = elem_value<W, (h >> 1)>(c.P);
             if constexpr (h >= 0 && (h & 1) == 1) elem_store<VAR, W, (h >> 1)>(wv, wu, c.cs[h >> 1], c.p, c.grp);
         } else {
-            constexpr int e0 = elem_at<W>(S, 0), e1 = elem_at<W>(S, 1);
+            constexpr int e0 = elem_at<W, LATE>(S, 0), e1 = elem_at<W, LATE>(S, 1);
             if constexpr (e0 >= 0) elem_store<VAR, W, e0>(wv, wu, elem_value<W, e0>(c.P), c.p, c.grp);
             if constexpr (e1 >= 0) elem_store<VAR, W, e1>(wv, wu, elem_value<W, e1>(c.P), c.p, c.grp);
         }
         __builtin_amdgcn_sched_barrier(0);
-        slot<VAR, W, S + 1>(rv, ru, wv, wu, c);
+        slot<VAR, W, LATE, S + 1, END>(rv, ru, wv, wu, c);
     }
 }
 
@@ -365,16 +383,44 @@ __device__ inline void slot(const unsigned* __restrict__ rv, const unsigned* __r
 // generation's LDS writes never alias the operand reads.  Also loads the
 // point data of the stage after next into R; returns true when this lane's
 // point has a box that does not cover the tile (P then holds the box).
-template <int VAR, int W>
+// LATE: the stage's first operands come from cy (read by the previous stage);
+// the barrier comes before the last MFMA group, after it the next stage's
+// first operands are read into cy from wv / wu, which all waves have written
+// by then, while the last group runs.  Otherwise the caller's barrier follows.
+template <int VAR, int W, bool LATE>
 __device__ inline bool stage_body(const unsigned* __restrict__ rv, const unsigned* __restrict__ ru,
                                   unsigned* __restrict__ wv, unsigned* __restrict__ wu, f32x16 (&acc)[4][2],
                                   Pt& P, Raw& R, Rsrc rs, int pnext, int ke, double ythr, double xthr, double dy,
-                                  double dx, int x0, int y0, int p, int grp, int lrow, int koff, int ucol, float sg)
+                                  double dx, int x0, int y0, int p, int grp, int lrow, int koff, int ucol, float sg,
+                                  Carry& cy)
 {
+    constexpr bool THREE = Traits<VAR>::three;
+    constexpr int NM = mfma_per_stage<VAR>(), PU = Shape<W>::PLANE_U;
     StageCtx<VAR, W> c{acc, R, rs, pnext, ke, ythr, xthr, dy, dx, x0, y0, p, grp, lrow, koff, ucol, sg};
     c.partial = false;
-    stage_operands<VAR, Shape<W>::PLANE_U>(rv, ru, c);
-    slot<VAR, W, 0>(rv, ru, wv, wu, c);
+    if constexpr (LATE) {
+        constexpr int T = NM - tail_mfmas<VAR>();
+        c.A[0][0] = cy.A[0][0];
+        c.Ub[0][0] = cy.Ub[0][0];
+        if constexpr (THREE) {
+            c.A[0][1] = cy.A[0][1];
+            c.Ub[0][1] = cy.Ub[0][1];
+        } else {
+            c.A[1][0] = cy.A[1][0];
+        }
+        xform_b(c, 0, 0);
+        if constexpr (THREE) xform_b(c, 0, 1);
+        slot<VAR, W, true, 0, T>(rv, ru, wv, wu, c);
+        __syncthreads();
+        load_b<VAR, PU>(wu, cy, 0);
+        load_a<VAR>(wv, cy, 0);
+        if constexpr (!THREE) load_a<VAR>(wv, cy, 1);
+        __builtin_amdgcn_sched_barrier(0);
+        slot<VAR, W, true, T, NM>(rv, ru, wv, wu, c);
+    } else {
+        stage_operands<VAR, PU>(rv, ru, c);
+        slot<VAR, W, false, 0, NM>(rv, ru, wv, wu, c);
+    }
     P = c.P;
     return c.partial;
 }
@@ -390,18 +436,18 @@ __device__ inline bool stage_body(const unsigned* __restrict__ rv, const unsigne
 template <int VAR, int W, int RB, bool MASK>
 __device__ inline void step(Smem<Traits<VAR>::three, W>& sm, f32x16 (&acc)[4][2], Raw& R, Rsrc rs, int st,
                             int kb, int ke, double ythr, double xthr, double dy, double dx, int x0, int y0, int p,
-                            int grp, int lrow, int koff, int ucolx, float sg)
+                            int grp, int lrow, int koff, int ucolx, float sg, Carry& cy)
 {
     Pt P;
-    const bool partial = stage_body<VAR, W>(&sm.v[RB][0][0][0], &sm.u[RB][0][0][0], &sm.v[RB ^ 1][0][0][0],
-                                         &sm.u[RB ^ 1][0][0][0], acc, P, R, rs, kb + (st + 2) * KP + p, ke, ythr,
-                                         xthr, dy, dx, x0, y0, p, grp, lrow, koff, ucolx, sg);
+    const bool partial = stage_body<VAR, W, !MASK>(&sm.v[RB][0][0][0], &sm.u[RB][0][0][0], &sm.v[RB ^ 1][0][0][0],
+                                                &sm.u[RB ^ 1][0][0][0], acc, P, R, rs, kb + (st + 2) * KP + p, ke,
+                                                ythr, xthr, dy, dx, x0, y0, p, grp, lrow, koff, ucolx, sg, cy);
     if constexpr (MASK) {
         if (__builtin_amdgcn_ballot_w64(partial)) mask_stage<VAR, W, RB ^ 1>(sm, P, p, grp);
+        __syncthreads();
     } else {
-        (void)partial;
+        (void)partial;   // the barrier was inside stage_body
     }
-    __syncthreads();
 }
 
 // One workgroup's tile and split: prologue, K loop, FP32 partials.
@@ -448,15 +494,30 @@ __device__ __attribute__((always_inline)) inline void fresnel_tile(Smem<Traits<V
         load_raw(R, rs, kb + KP + p, ke);
     }
     __syncthreads();
+    Carry cy;
+    cy.lrow = lrow;
+    cy.koff = koff;
+    cy.ucol = ucolx;
+    if constexpr (!MASK) {
+        load_b<VAR, Sh::PLANE_U>(&sm.u[0][0][0][0], cy, 0);
+        load_a<VAR>(&sm.v[0][0][0][0], cy, 0);
+        if constexpr (!Traits<VAR>::three) load_a<VAR>(&sm.v[0][0][0][0], cy, 1);
+    }
     f32x16 acc[4][2];
 #pragma unroll
     for (int i = 0; i < 4; ++i) acc[i][0] = acc[i][1] = f32x16{};
+    // LATE: no branch between the two steps, so that the next operands' reads
+    // stay ahead of the last MFMA group; for an odd stage count the second step
+    // then runs one stage past the end, whose operands (generated from clamped,
+    // zero-amplitude point data) contribute exact zeros.
     for (int st = 0; st < nst; st += 2) {
         step<VAR, W, 0, MASK>(sm, acc, R, rs, st, kb, ke, ythr, xthr, dy, dx, x0, y0, p, grp, lrow, koff, ucolx,
-                              sg);
-        if (st + 1 >= nst) break;
+                              sg, cy);
+        if constexpr (MASK) {
+            if (st + 1 >= nst) break;
+        }
         step<VAR, W, 1, MASK>(sm, acc, R, rs, st + 1, kb, ke, ythr, xthr, dy, dx, x0, y0, p, grp, lrow, koff,
-                              ucolx, sg);
+                              ucolx, sg, cy);
     }
 
     store_partials(g, acc, part, split, x0, y0, ucolx, lane);

@@ -285,10 +285,10 @@ struct StageCtx {
 // per slot (slots 0 .. NPIECE/2 - 1), amplitude/box at NPIECE/2, elements from
 // NPIECE/2 + 1 to 15.
 //
-// LATE (the unmasked instances): the stage's barrier comes before its last
-// MFMA group (TAIL MFMAs), which then runs on operands already in registers
-// while the next stage's first operands are read; the generation ends
-// before that group.
+// LATE (the unmasked instances): the stage's barrier comes within its last
+// MFMA group (tail_mfmas), which runs on operands already in registers while
+// the next stage's first operands are read; the generation ends before that
+// group.
 constexpr int H0 = NPIECE + 1, E0 = NPIECE / 2 + 1;
 template <int VAR>
 constexpr int tail_mfmas()
@@ -384,9 +384,11 @@ __device__ inline void slot(const unsigned* __restrict__ rv, const unsigned* __r
 // point data of the stage after next into R; returns true when this lane's
 // point has a box that does not cover the tile (P then holds the box).
 // LATE: the stage's first operands come from cy (read by the previous stage);
-// the barrier comes before the last MFMA group, after it the next stage's
-// first operands are read into cy from wv / wu, which all waves have written
-// by then, while the last group runs.  Otherwise the caller's barrier follows.
+// the barrier comes in the middle of the last MFMA group (whose operands are in
+// registers, and whose first half keeps the matrix pipe busy while the wave
+// waits there); after it the next stage's first operands are read into cy from
+// wv / wu, which all waves have written by then, while the rest of the group
+// runs.  Otherwise the caller's barrier follows.
 template <int VAR, int W, bool LATE>
 __device__ inline bool stage_body(const unsigned* __restrict__ rv, const unsigned* __restrict__ ru,
                                   unsigned* __restrict__ wv, unsigned* __restrict__ wu, f32x16 (&acc)[4][2],
@@ -399,7 +401,7 @@ __device__ inline bool stage_body(const unsigned* __restrict__ rv, const unsigne
     StageCtx<VAR, W> c{acc, R, rs, pnext, ke, ythr, xthr, dy, dx, x0, y0, p, grp, lrow, koff, ucol, sg};
     c.partial = false;
     if constexpr (LATE) {
-        constexpr int T = NM - tail_mfmas<VAR>();
+        constexpr int T = NM - tail_mfmas<VAR>() / 2;   // half the tail group before the barrier
         c.A[0][0] = cy.A[0][0];
         c.Ub[0][0] = cy.Ub[0][0];
         if constexpr (THREE) {

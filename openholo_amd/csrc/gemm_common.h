@@ -170,6 +170,20 @@ constexpr int mfma_per_stage()
 {
     return Traits<VAR>::three ? 48 : 16;
 }
+// MFMAs of a stage's last (ks, mb) group, which load no operands
+template <int VAR>
+constexpr int tail_mfmas()
+{
+    return Traits<VAR>::three ? 6 : 2;
+}
+
+// The first operands of a stage, read by the previous stage right after its
+// barrier, which comes within that stage's last MFMA group (late-barrier
+// pipeline); the fields load_a / load_b use.
+struct Carry {
+    u32x4 A[8][2], Ub[2][2];
+    int lrow, koff, ucol;
+};
 
 // Operands of a stage's first MFMAs, loaded before slot 0 (PU: dwords of one
 // U plane).

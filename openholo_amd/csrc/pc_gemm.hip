@@ -290,11 +290,6 @@ struct StageCtx {
 // the next stage's first operands are read; the generation ends before that
 // group.
 constexpr int H0 = NPIECE + 1, E0 = NPIECE / 2 + 1;
-template <int VAR>
-constexpr int tail_mfmas()
-{
-    return Traits<VAR>::three ? 6 : 2;   // the last (ks, mb) group
-}
 template <int W, bool LATE>
 constexpr int half_at(int S)
 {
@@ -336,12 +331,6 @@ static_assert(schedule_complete<128, false>() && schedule_complete<256, false>()
                   schedule_complete<128, true>() && schedule_complete<256, true>(),
               "generation schedule must cover every element once");
 
-// The first operands of a stage, read by the previous stage right after its
-// barrier (LATE pipeline); the fields load_a / load_b use.
-struct Carry {
-    u32x4 A[8][2], Ub[2][2];
-    int lrow, koff, ucol;
-};
 
 // Slot S of the pipelined stage: MFMA S, operand loads / transforms due at
 // this slot, and this slot's share of the next stage's generation.

@@ -291,32 +291,33 @@ __device__ inline void unit_store(unsigned* wv, unsigned* wu, float4 w, int p, i
 
 // Generation schedule.  3-term (48 MFMA slots): prep piece K at slot K, the
 // next stage's point loads at slot NPR, half-unit h (even: value of unit h/2,
-// odd: its store) at slot H0 + (46 - H0) h / (2 RU).  1-term (16 slots): prep
+// odd: its store) at slot H0 + (42 - H0) h / (2 RU).  1-term (16 slots): prep
 // pieces two per slot, loads at slot (NPR + 1) / 2, unit E at slot
-// E0 + (16 - E0) E / RU.
+// E0 + (14 - E0) E / RU.  Both end before the stage's last MFMA group, within
+// which the barrier comes (see step).
 constexpr int H0 = NPR + 1, E0 = (NPR + 1) / 2 + 1;
 constexpr int half_at(int S)
 {
     for (int h = 0; h < 2 * RU; ++h)
-        if (H0 + ((46 - H0) * h) / (2 * RU) == S) return h;
+        if (H0 + ((42 - H0) * h) / (2 * RU) == S) return h;
     return -1;
 }
 constexpr int unit_at(int S)
 {
     for (int e = 0; e < RU; ++e)
-        if (E0 + ((16 - E0) * e) / RU == S) return e;
+        if (E0 + ((14 - E0) * e) / RU == S) return e;
     return -1;
 }
 constexpr bool schedule_complete()
 {
     for (int e = 0; e < RU; ++e) {
         int n = 0;
-        for (int S = 0; S < 16; ++S) n += unit_at(S) == e;
+        for (int S = 0; S < 16 - 2; ++S) n += unit_at(S) == e;
         if (n != 1) return false;
     }
     for (int h = 0; h < 2 * RU; ++h) {
         int n = 0;
-        for (int S = 0; S < 48; ++S) n += half_at(S) == h;
+        for (int S = 0; S < 48 - 6; ++S) n += half_at(S) == h;
         if (n != 1) return false;
     }
     return half_at(NPR) < 0 && unit_at((NPR + 1) / 2) < 0 && E0 > (NPR + 1) / 2;
@@ -337,13 +338,12 @@ struct StageCtx {
     float4 cs[RU];
 };
 
-template <int VAR, bool MASK, int S>
+template <int VAR, bool MASK, int S, int END>
 __device__ inline void slot(const unsigned* __restrict__ rv, const unsigned* __restrict__ ru,
                            unsigned* __restrict__ wv, unsigned* __restrict__ wu, StageCtx<VAR>& c)
 {
     constexpr bool THREE = Traits<VAR>::three;
-    constexpr int NM = mfma_per_stage<VAR>();
-    if constexpr (S < NM) {
+    if constexpr (S < END) {
         mfma_slot<VAR, S>(rv, ru, c);
         if constexpr (THREE && S < NPR) prep_piece<S, MASK>(c.P, c.t, c.R, c.f);
         if constexpr (!THREE && 2 * S < NPR) prep_piece<2 * S, MASK>(c.P, c.t, c.R, c.f);
@@ -358,7 +358,7 @@ __device__ inline void slot(const unsigned* __restrict__ rv, const unsigned* __r
             if constexpr (e >= 0) unit_store<VAR, e>(wv, wu, unit_value<e, MASK, true>(c.P, c.f, c.grp), c.p, c.grp);
         }
         __builtin_amdgcn_sched_barrier(0);
-        slot<VAR, MASK, S + 1>(rv, ru, wv, wu, c);
+        slot<VAR, MASK, S + 1, END>(rv, ru, wv, wu, c);
     }
 }
 
@@ -385,17 +385,39 @@ __device__ inline void prep_all(Pt& P, const Raw& r, const Frame& f)
 // One pipeline step: MFMAs of stage st (buffer RB), generation of stage st+1
 // (buffer RB^1) from R, then R <- stage st+2's point data.  Past the last
 // stage the generation runs on clamped, zero-amplitude data into the unused
-// buffer, which keeps the step branch-free.
+// buffer, which keeps the step branch-free.  The stage's first operands come
+// from cy; its barrier comes in the middle of the last MFMA group (whose
+// operands are in registers, and whose first half keeps the matrix pipe busy
+// while the wave waits), after which the next stage's first operands are read
+// into cy from buffer RB^1 while the rest of the group runs.
 template <int VAR, bool MASK, int RB>
 __device__ inline void step(Smem<Traits<VAR>::three>& sm, f32x16 (&acc)[4][2], Raw& R, Rsrc rs, const Frame& f,
-                            int pnext, int ke, int p, int grp, int lrow, int koff, int ucol)
+                            int pnext, int ke, int p, int grp, int lrow, int koff, int ucol, Carry& cy)
 {
+    constexpr bool THREE = Traits<VAR>::three;
+    constexpr int NM = mfma_per_stage<VAR>(), T = NM - tail_mfmas<VAR>() / 2;
     const unsigned* __restrict__ rv = &sm.v[RB][0][0][0];
     const unsigned* __restrict__ ru = &sm.u[RB][0][0][0];
+    unsigned* __restrict__ wv = &sm.v[RB ^ 1][0][0][0];
+    unsigned* __restrict__ wu = &sm.u[RB ^ 1][0][0][0];
     StageCtx<VAR> c{acc, R, rs, f, pnext, ke, p, grp, lrow, koff, ucol};
-    stage_operands<VAR>(rv, ru, c);
-    slot<VAR, MASK, 0>(rv, ru, &sm.v[RB ^ 1][0][0][0], &sm.u[RB ^ 1][0][0][0], c);
+    c.A[0][0] = cy.A[0][0];
+    c.Ub[0][0] = cy.Ub[0][0];
+    if constexpr (THREE) {
+        c.A[0][1] = cy.A[0][1];
+        c.Ub[0][1] = cy.Ub[0][1];
+    } else {
+        c.A[1][0] = cy.A[1][0];
+    }
+    xform_b(c, 0, 0);
+    if constexpr (THREE) xform_b(c, 0, 1);
+    slot<VAR, MASK, 0, T>(rv, ru, wv, wu, c);
     __syncthreads();
+    load_b<VAR>(wu, cy, 0);
+    load_a<VAR>(wv, cy, 0);
+    if constexpr (!THREE) load_a<VAR>(wv, cy, 1);
+    __builtin_amdgcn_sched_barrier(0);
+    slot<VAR, MASK, T, NM>(rv, ru, wv, wu, c);
 }
 
 // S = sum_p U V^T over the np records at rec (all matrix-core eligible, at
@@ -453,13 +475,23 @@ __global__ __launch_bounds__(NT, 2) void k_rs_gemm(Geometry g, const RsRec* __re
         load_raw<MASK>(R, rs, f.tm, kb + RP + p, ke);
     }
     __syncthreads();
+    Carry cy;
+    cy.lrow = lrow;
+    cy.koff = koff;
+    cy.ucol = ucol;
+    load_b<VAR>(&sm.u[0][0][0][0], cy, 0);
+    load_a<VAR>(&sm.v[0][0][0][0], cy, 0);
+    if constexpr (!Traits<VAR>::three) load_a<VAR>(&sm.v[0][0][0][0], cy, 1);
     f32x16 acc[4][2];
 #pragma unroll
     for (int i = 0; i < 4; ++i) acc[i][0] = acc[i][1] = f32x16{};
+    // no branch between the two steps, so that the next operands' reads stay
+    // ahead of the last MFMAs; for an odd stage count the second step runs one
+    // stage past the end, whose operands (generated from clamped, zero-amplitude
+    // point data) contribute exact zeros
     for (int st = 0; st < nst; st += 2) {
-        step<VAR, MASK, 0>(sm, acc, R, rs, f, kb + (st + 2) * RP + p, ke, p, grp, lrow, koff, ucol);
-        if (st + 1 >= nst) break;
-        step<VAR, MASK, 1>(sm, acc, R, rs, f, kb + (st + 3) * RP + p, ke, p, grp, lrow, koff, ucol);
+        step<VAR, MASK, 0>(sm, acc, R, rs, f, kb + (st + 2) * RP + p, ke, p, grp, lrow, koff, ucol, cy);
+        step<VAR, MASK, 1>(sm, acc, R, rs, f, kb + (st + 3) * RP + p, ke, p, grp, lrow, koff, ucol, cy);
     }
     store_partials(g, acc, part, split, x0, y0, ucol, lane);
 }

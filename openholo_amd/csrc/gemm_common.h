@@ -185,18 +185,6 @@ struct Carry {
     int lrow, koff, ucol;
 };
 
-// Operands of a stage's first MFMAs, loaded before slot 0 (PU: dwords of one
-// U plane).
-template <int VAR, int PU = PLANE_U, class C>
-__device__ inline void stage_operands(const unsigned* rv, const unsigned* ru, C& c)
-{
-    load_b<VAR, PU>(ru, c, 0);
-    load_a<VAR>(rv, c, 0);
-    if constexpr (!Traits<VAR>::three) load_a<VAR>(rv, c, 1);
-    xform_b(c, 0, 0);
-    if constexpr (Traits<VAR>::three) xform_b(c, 0, 1);
-}
-
 // Slot S of a stage: MFMA S and the operand loads / transforms due there.
 template <int VAR, int S, int PU = PLANE_U, class C>
 __device__ inline void mfma_slot(const unsigned* rv, const unsigned* ru, C& c)

@@ -277,64 +277,59 @@ struct StageCtx {
 };
 
 // The generation schedule: 3-term, half-element h (even: value of element
-// h/2, odd: its store) at slot H0 + (46 - H0) h / (2 NE); 1-term, element E
-// at slot E0 + (16 - E0) E / NE (two elements share a slot when needed).
+// h/2, odd: its store) at slot H0 + (42 - H0) h / (2 NE); 1-term, element E
+// at slot E0 + (14 - E0) E / NE (two elements share a slot when needed).
 // Return the half-element / element at slot S, or -1.
 // 3-term: prep piece K at slot K, amplitude/box and the next loads at slot
-// NPIECE, half-elements from slot NPIECE + 1 to 46.  1-term: prep pieces two
+// NPIECE, half-elements from slot NPIECE + 1 to 41.  1-term: prep pieces two
 // per slot (slots 0 .. NPIECE/2 - 1), amplitude/box at NPIECE/2, elements from
-// NPIECE/2 + 1 to 15.
-//
-// LATE (the unmasked instances): the stage's barrier comes within its last
-// MFMA group (tail_mfmas), which runs on operands already in registers while
-// the next stage's first operands are read; the generation ends before that
-// group.
+// NPIECE/2 + 1 to 13.  The generation ends before the stage's last MFMA group
+// (tail_mfmas), within which the barrier comes (stage_body).
 constexpr int H0 = NPIECE + 1, E0 = NPIECE / 2 + 1;
-template <int W, bool LATE>
+template <int W>
 constexpr int half_at(int S)
 {
-    constexpr int NE = Shape<W>::NE, END = LATE ? 48 - 6 : 46;
+    constexpr int NE = Shape<W>::NE, END = 48 - 6;
     for (int h = 0; h < 2 * NE; ++h)
         if (H0 + ((END - H0) * h) / (2 * NE) == S) return h;
     return -1;
 }
-template <int W, bool LATE>
+template <int W>
 constexpr int elem_at(int S, int n)   // n-th element at slot S (slots hold at most two)
 {
-    constexpr int NE = Shape<W>::NE, END = LATE ? 16 - 2 : 16;
+    constexpr int NE = Shape<W>::NE, END = 16 - 2;
     for (int e = 0; e < NE; ++e)
         if (E0 + ((END - E0) * e) / NE == S && n-- == 0) return e;
     return -1;
 }
 // every element (1-term) and half-element (3-term) lands in exactly one slot,
-// before the tail group when LATE
-template <int W, bool LATE>
+// before the tail group
+template <int W>
 constexpr bool schedule_complete()
 {
     constexpr int NE = Shape<W>::NE;
     for (int e = 0; e < NE; ++e) {
         int n = 0;
-        for (int S = 0; S < 16; ++S)
-            n += (elem_at<W, LATE>(S, 0) == e) + (elem_at<W, LATE>(S, 1) == e) + (elem_at<W, LATE>(S, 2) == e);
+        for (int S = 0; S < 16 - 2; ++S)
+            n += (elem_at<W>(S, 0) == e) + (elem_at<W>(S, 1) == e) + (elem_at<W>(S, 2) == e);
         if (n != 1) return false;
     }
     for (int S = 0; S < 16; ++S)
-        if (elem_at<W, LATE>(S, 2) >= 0 || (LATE && S >= 16 - 2 && elem_at<W, LATE>(S, 0) >= 0)) return false;
+        if (elem_at<W>(S, 2) >= 0) return false;
     for (int h = 0; h < 2 * NE; ++h) {
         int n = 0;
-        for (int S = 0; S < 48; ++S) n += half_at<W, LATE>(S) == h && !(LATE && S >= 48 - 6);
+        for (int S = 0; S < 48 - 6; ++S) n += half_at<W>(S) == h;
         if (n != 1) return false;
     }
     return true;
 }
-static_assert(schedule_complete<128, false>() && schedule_complete<256, false>() &&
-                  schedule_complete<128, true>() && schedule_complete<256, true>(),
-              "generation schedule must cover every element once");
+static_assert(schedule_complete<128>() && schedule_complete<256>(),
+              "generation schedule must cover every element once, before the last MFMA group");
 
 
 // Slot S of the pipelined stage: MFMA S, operand loads / transforms due at
 // this slot, and this slot's share of the next stage's generation.
-template <int VAR, int W, bool LATE, int S, int END>
+template <int VAR, int W, int S, int END>
 __device__ inline void slot(const unsigned* __restrict__ rv, const unsigned* __restrict__ ru,
                             unsigned* __restrict__ wv, unsigned* __restrict__ wu, StageCtx<VAR, W>& c)
 {
@@ -351,16 +346,16 @@ __device__ inline void slot(const unsigned* __restrict__ rv, const unsigned* __r
             load_raw(c.R, c.rs, c.pnext, c.ke);
         }
         if constexpr (THREE) {
-            constexpr int h = half_at<W, LATE>(S);
+            constexpr int h = half_at<W>(S);
             if constexpr (h >= 0 && (h & 1) == 0) c.cs[h >> 1] = elem_value<W, (h >> 1)>(c.P);
             if constexpr (h >= 0 && (h & 1) == 1) elem_store<VAR, W, (h >> 1)>(wv, wu, c.cs[h >> 1], c.p, c.grp);
         } else {
-            constexpr int e0 = elem_at<W, LATE>(S, 0), e1 = elem_at<W, LATE>(S, 1);
+            constexpr int e0 = elem_at<W>(S, 0), e1 = elem_at<W>(S, 1);
             if constexpr (e0 >= 0) elem_store<VAR, W, e0>(wv, wu, elem_value<W, e0>(c.P), c.p, c.grp);
             if constexpr (e1 >= 0) elem_store<VAR, W, e1>(wv, wu, elem_value<W, e1>(c.P), c.p, c.grp);
         }
         __builtin_amdgcn_sched_barrier(0);
-        slot<VAR, W, LATE, S + 1, END>(rv, ru, wv, wu, c);
+        slot<VAR, W, S + 1, END>(rv, ru, wv, wu, c);
     }
 }
 
@@ -370,50 +365,48 @@ __device__ inline void slot(const unsigned* __restrict__ rv, const unsigned* __r
 // elements), fenced by sched_barrier so that the VALU work issues while the
 // matrix core runs.  The restrict-qualified planes tell the compiler that the
 // generation's LDS writes never alias the operand reads.  Also loads the
-// point data of the stage after next into R; returns true when this lane's
-// point has a box that does not cover the tile (P then holds the box).
-// LATE: the stage's first operands come from cy (read by the previous stage);
-// the barrier comes in the middle of the last MFMA group (whose operands are in
-// registers, and whose first half keeps the matrix pipe busy while the wave
-// waits there); after it the next stage's first operands are read into cy from
-// wv / wu, which all waves have written by then, while the rest of the group
-// runs.  Otherwise the caller's barrier follows.
-template <int VAR, int W, bool LATE>
-__device__ inline bool stage_body(const unsigned* __restrict__ rv, const unsigned* __restrict__ ru,
-                                  unsigned* __restrict__ wv, unsigned* __restrict__ wu, f32x16 (&acc)[4][2],
-                                  Pt& P, Raw& R, Rsrc rs, int pnext, int ke, double ythr, double xthr, double dy,
-                                  double dx, int x0, int y0, int p, int grp, int lrow, int koff, int ucol, float sg,
-                                  Carry& cy)
+// point data of the stage after next into R.  The stage's first operands come
+// from cy (read by the previous stage); the barrier comes in the middle of the
+// last MFMA group (whose operands are in registers, and whose first half keeps
+// the matrix pipe busy while the wave waits there), after the masked
+// instances' zeroing of out-of-box elements (MASK: waves holding a point whose
+// box does not cover the tile); after it the next stage's first operands are
+// read into cy from wv / wu, which all waves have written by then, while the
+// rest of the group runs.
+template <int VAR, int W, bool MASK, int RB>
+__device__ inline void stage_body(Smem<Traits<VAR>::three, W>& sm, f32x16 (&acc)[4][2], Raw& R, Rsrc rs,
+                                  int pnext, int ke, double ythr, double xthr, double dy, double dx, int x0, int y0,
+                                  int p, int grp, int lrow, int koff, int ucol, float sg, Carry& cy)
 {
     constexpr bool THREE = Traits<VAR>::three;
     constexpr int NM = mfma_per_stage<VAR>(), PU = Shape<W>::PLANE_U;
+    const unsigned* __restrict__ rv = &sm.v[RB][0][0][0];
+    const unsigned* __restrict__ ru = &sm.u[RB][0][0][0];
+    unsigned* __restrict__ wv = &sm.v[RB ^ 1][0][0][0];
+    unsigned* __restrict__ wu = &sm.u[RB ^ 1][0][0][0];
     StageCtx<VAR, W> c{acc, R, rs, pnext, ke, ythr, xthr, dy, dx, x0, y0, p, grp, lrow, koff, ucol, sg};
     c.partial = false;
-    if constexpr (LATE) {
-        constexpr int T = NM - tail_mfmas<VAR>() / 2;   // half the tail group before the barrier
-        c.A[0][0] = cy.A[0][0];
-        c.Ub[0][0] = cy.Ub[0][0];
-        if constexpr (THREE) {
-            c.A[0][1] = cy.A[0][1];
-            c.Ub[0][1] = cy.Ub[0][1];
-        } else {
-            c.A[1][0] = cy.A[1][0];
-        }
-        xform_b(c, 0, 0);
-        if constexpr (THREE) xform_b(c, 0, 1);
-        slot<VAR, W, true, 0, T>(rv, ru, wv, wu, c);
-        __syncthreads();
-        load_b<VAR, PU>(wu, cy, 0);
-        load_a<VAR>(wv, cy, 0);
-        if constexpr (!THREE) load_a<VAR>(wv, cy, 1);
-        __builtin_amdgcn_sched_barrier(0);
-        slot<VAR, W, true, T, NM>(rv, ru, wv, wu, c);
+    constexpr int T = NM - tail_mfmas<VAR>() / 2;   // half the last group before the barrier
+    c.A[0][0] = cy.A[0][0];
+    c.Ub[0][0] = cy.Ub[0][0];
+    if constexpr (THREE) {
+        c.A[0][1] = cy.A[0][1];
+        c.Ub[0][1] = cy.Ub[0][1];
     } else {
-        stage_operands<VAR, PU>(rv, ru, c);
-        slot<VAR, W, false, 0, NM>(rv, ru, wv, wu, c);
+        c.A[1][0] = cy.A[1][0];
     }
-    P = c.P;
-    return c.partial;
+    xform_b(c, 0, 0);
+    if constexpr (THREE) xform_b(c, 0, 1);
+    slot<VAR, W, 0, T>(rv, ru, wv, wu, c);
+    if constexpr (MASK) {
+        if (__builtin_amdgcn_ballot_w64(c.partial)) mask_stage<VAR, W, RB ^ 1>(sm, c.P, p, grp);
+    }
+    __syncthreads();
+    load_b<VAR, PU>(wu, cy, 0);
+    load_a<VAR>(wv, cy, 0);
+    if constexpr (!THREE) load_a<VAR>(wv, cy, 1);
+    __builtin_amdgcn_sched_barrier(0);
+    slot<VAR, W, T, NM>(rv, ru, wv, wu, c);
 }
 
 // One pipeline step: MFMAs of stage st (buffer RB) and generation of stage
@@ -429,16 +422,8 @@ __device__ inline void step(Smem<Traits<VAR>::three, W>& sm, f32x16 (&acc)[4][2]
                             int kb, int ke, double ythr, double xthr, double dy, double dx, int x0, int y0, int p,
                             int grp, int lrow, int koff, int ucolx, float sg, Carry& cy)
 {
-    Pt P;
-    const bool partial = stage_body<VAR, W, !MASK>(&sm.v[RB][0][0][0], &sm.u[RB][0][0][0], &sm.v[RB ^ 1][0][0][0],
-                                                &sm.u[RB ^ 1][0][0][0], acc, P, R, rs, kb + (st + 2) * KP + p, ke,
-                                                ythr, xthr, dy, dx, x0, y0, p, grp, lrow, koff, ucolx, sg, cy);
-    if constexpr (MASK) {
-        if (__builtin_amdgcn_ballot_w64(partial)) mask_stage<VAR, W, RB ^ 1>(sm, P, p, grp);
-        __syncthreads();
-    } else {
-        (void)partial;   // the barrier was inside stage_body
-    }
+    stage_body<VAR, W, MASK, RB>(sm, acc, R, rs, kb + (st + 2) * KP + p, ke, ythr, xthr, dy, dx, x0, y0, p, grp,
+                                 lrow, koff, ucolx, sg, cy);
 }
 
 // One workgroup's tile and split: prologue, K loop, FP32 partials.
@@ -489,24 +474,19 @@ __device__ __attribute__((always_inline)) inline void fresnel_tile(Smem<Traits<V
     cy.lrow = lrow;
     cy.koff = koff;
     cy.ucol = ucolx;
-    if constexpr (!MASK) {
-        load_b<VAR, Sh::PLANE_U>(&sm.u[0][0][0][0], cy, 0);
-        load_a<VAR>(&sm.v[0][0][0][0], cy, 0);
-        if constexpr (!Traits<VAR>::three) load_a<VAR>(&sm.v[0][0][0][0], cy, 1);
-    }
+    load_b<VAR, Sh::PLANE_U>(&sm.u[0][0][0][0], cy, 0);
+    load_a<VAR>(&sm.v[0][0][0][0], cy, 0);
+    if constexpr (!Traits<VAR>::three) load_a<VAR>(&sm.v[0][0][0][0], cy, 1);
     f32x16 acc[4][2];
 #pragma unroll
     for (int i = 0; i < 4; ++i) acc[i][0] = acc[i][1] = f32x16{};
-    // LATE: no branch between the two steps, so that the next operands' reads
+    // no branch between the two steps, so that the next operands' reads
     // stay ahead of the last MFMA group; for an odd stage count the second step
     // then runs one stage past the end, whose operands (generated from clamped,
     // zero-amplitude point data) contribute exact zeros.
     for (int st = 0; st < nst; st += 2) {
         step<VAR, W, 0, MASK>(sm, acc, R, rs, st, kb, ke, ythr, xthr, dy, dx, x0, y0, p, grp, lrow, koff, ucolx,
                               sg, cy);
-        if constexpr (MASK) {
-            if (st + 1 >= nst) break;
-        }
         step<VAR, W, 1, MASK>(sm, acc, R, rs, st + 1, kb, ke, ythr, xthr, dy, dx, x0, y0, p, grp, lrow, koff,
                               ucolx, sg, cy);
     }

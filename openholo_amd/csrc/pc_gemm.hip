@@ -494,7 +494,8 @@ __device__ __attribute__((always_inline)) inline void fresnel_tile(Smem<Traits<V
     store_partials(g, acc, part, split, x0, y0, ucolx, lane);
 }
 
-// pconst: per-point constants of points kfirst .. klast-1 (k_gemm_consts).
+// pconst: the records of points kfirst .. klast-1 (k_gemm_consts; at most
+// MAX_REC of them) for this instance's tile shape.
 // Both MASK instances are launched; the one that does not match
 // *partial_flag (!= 0: some point has a box smaller than the hologram) returns
 // at once, so each keeps its own register allocation.
